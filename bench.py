@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark: factor + local update + 16-RHS solve of the 2D variable-coefficient operator.
+
+Workload (the 2D 1023^2 configuration): 5-point -div(a grad u) on the unit square, 1023 x 1023
+interior grid (n = 1,046,529), a = lognormal(0,1) iid per padded node (numpy PCG64, seed 2),
+h = 1/1024, nested dissection with maxSideLength 16 (31 x 31 leaf boxes, 6 levels), FP64
+plain form.  One step = Factor() + Update() (a multiplied by 100 on a random 32 x 32 block of
+nodes, modified DOFs by the reference's modDOF rule) + Solve() of 16 N(0,1) right-hand sides.
+All inputs (CSR values of both operators, right-hand sides) are resident in HBM before timing;
+the update's modified-DOF list is host data, as in the reference API.
+
+value = algorithmic TFLOP/s of the whole step: (F_factor + F_update + F_solve) / time, with
+F = sum over fronts of n^3/3 + n^2 m + n m^2 (factor, dirty fronts for update) and
+nrhs * sum (2 n^2 + 4 n m) (solve), n / m = pivot / boundary size of each front.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "updating-multifrontal_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "factor+update+solve s & TFLOP/s; FP32-MFMA+IR vs FP64 at 1e-10 backward err"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def make_problem(N, seed, box=32, scale=100.0, nrhs=16):
+    import umf
+    rng = np.random.default_rng(seed)
+    co = np.exp(rng.standard_normal((N + 2, N + 2)))
+    co2 = co.copy()
+    i0, j0 = (int(v) for v in rng.integers(0, N + 2 - box, size=2))
+    co2[i0:i0 + box, j0:j0 + box] *= scale
+    h = 1.0 / (N + 1)
+    rp, ci, v1 = umf.build_laplacian((N, N), co, 1, h)
+    _, _, v2 = umf.build_laplacian((N, N), co2, 1, h)
+    mod = np.unique(ci[v1 != v2]).astype(np.int32)  # modDOF rule (testMF2D.cpp:143-150)
+    B = rng.standard_normal((nrhs, N * N))  # row r = right-hand side r: column-major n x nrhs
+    return rp, ci, v1, v2, mod, B, (i0, j0)
+
+
+class Csr:
+    def __init__(self, rp, ci, val):
+        self.rp, self.ci, self.val = rp, ci, val
+
+
+def algorithmic_flops(plan, nrhs):
+    fl_solve = 0.0
+    for ell in range(plan.levels):
+        for b in range(plan.nboxes(ell)):
+            n = len(plan.box_dofs(ell, b, 0))
+            m = len(plan.box_dofs(ell, b, 1))
+            fl_solve += 2.0 * n * n + 4.0 * n * m
+    return fl_solve * nrhs
+
+
+def cpu_baseline(N=511, seed=2, nrhs=16, max_side=16):
+    """The reference itself (oracle/_ref, MKL sequential, 1 thread) on a bounded sample of the
+    same workload: the same generator at 511^2 (5 levels, the same 31^2 leaf fronts),
+    Factor + Update + nrhs single-vector Solve calls as the reference API requires."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    import umf
+    kind = "reference" if oracle_py.have_ref() else "port"
+    rp, ci, v1, v2, mod, B, _ = make_problem(N, seed, nrhs=nrhs)
+    A1, A2 = Csr(rp, ci, v1), Csr(rp, ci, v2)
+    P = umf.MF2D(A1, N, N, 1, max_side, analyze_only=True)
+    f_factor = P.stats()["flops_factor"]
+    P.analyze_update(mod)
+    f_update = P.stats()["flops_update"]
+    f_solve = algorithmic_flops(P, nrhs)
+    Ac1 = oracle_py.Csr(rp, ci, v1)
+    Ac2 = oracle_py.Csr(rp, ci, v2)
+    F = oracle_py.Factorization(Ac1, (N, N), 1, max_side, "ref" if kind == "reference" else "orc")
+    t0 = time.perf_counter()
+    F.factor(trace=False)
+    t1 = time.perf_counter()
+    F.update(Ac2, mod, trace=False)
+    t2 = time.perf_counter()
+    for r in range(nrhs):
+        F.solve(B[r])
+    t3 = time.perf_counter()
+    tot = t3 - t0
+    return {
+        "value": (f_factor + f_update + f_solve) / tot / 1e12,
+        "unit": "TFLOP/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"2D {N}x{N} lognormal(0,1), maxSideLength {max_side}: Factor + Update (x100 on 32x32) + "
+                  f"{nrhs} single-RHS Solve calls; reference LU form via MKL sequential",
+        "t_factor_s": t1 - t0,
+        "t_update_s": t2 - t1,
+        "t_solve_s": t3 - t2,
+        "seconds": tot,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1023)
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the sharded path is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import umf
+    torch.cuda.set_device(0)
+    N, nrhs, seed = args.n, args.nrhs, 2
+    rp, ci, v1, v2, mod, B, corner = make_problem(N, seed, nrhs=nrhs)
+    n = N * N
+    plan = umf.MF2D(Csr(rp, ci, v1), N, N, 1, 16)
+    d_v = [torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()]
+    d_B = torch.from_numpy(B).cuda()
+    d_X = torch.empty_like(d_B)
+    d_R = torch.empty_like(d_B)
+
+    state = {"cur": 0}  # index of the operator the plan currently holds
+
+    def step():
+        plan.Factor()
+        nxt = 1 - state["cur"]
+        plan.update_device(d_v[nxt].data_ptr(), mod)
+        state["cur"] = nxt
+        plan.solve_device(d_B.data_ptr(), d_X.data_ptr(), nrhs)
+
+    for _ in range(args.warmup):
+        step()
+    if not args.no_profile:
+        plan.profile(True)
+        plan.profile_reset()
+    tf = tu = ts = 0.0
+    flops = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = plan.stats()
+        tf += st["t_factor_ms"]
+        tu += st["t_update_ms"]
+        ts += st["t_solve_ms"]
+        flops += st["flops_factor"] + st["flops_update"] + st["flops_solve"]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.profile(False)
+    secs = t1 - t0
+    K = args.steps
+
+    # accuracy of the last step: normwise backward error max_j ||b - A x|| / (||A||_F ||x|| + ||b||)
+    plan.spmv_device(d_X.data_ptr(), d_R.data_ptr(), nrhs)
+    R = (d_B - d_R).cpu().numpy()
+    X = d_X.cpu().numpy()
+    vcur = (v1, v2)[state["cur"]]
+    normA = float(np.linalg.norm(vcur))
+    berr = max(np.linalg.norm(R[j]) / (normA * np.linalg.norm(X[j]) + np.linalg.norm(B[j])) for j in range(nrhs))
+
+    prof = plan.profile_read() if not args.no_profile else {}
+    roof = None
+    if prof:
+        dom = max(prof, key=lambda k: prof[k]["ms"])
+        d = prof[dom]
+        avg_ms = d["ms"] / max(d["launches"], 1)
+        if d["flops"] > 0:
+            ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+        else:
+            ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None}
+        roof.update({"kernel": dom, "launches": d["launches"], "avg_launch_ms": avg_ms,
+                     "share_of_kernel_time": d["ms"] / sum(v["ms"] for v in prof.values())})
+
+    st = plan.stats()
+    out = {
+        "metric": METRIC,
+        "value": flops / secs / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": secs / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D 5-point -div(a grad u), {N}x{N} grid (n={n}), a~lognormal(0,1) per node, "
+                        f"maxSideLength 16 ({plan.levels} levels, {plan.leaf_width(0)}^2 leaves), "
+                        f"Factor + Update (a x100 on 32x32 nodes at {corner}) + {nrhs}-RHS Solve, FP64",
+            "nrhs": nrhs,
+            "precision": "fp64",
+        },
+        "t_factor_ms": tf / K,
+        "t_update_ms": tu / K,
+        "t_solve_ms": ts / K,
+        "flops_per_step": flops / K,
+        "fronts": st["n_fronts"],
+        "fronts_update": st["n_fronts_update"],
+        "front_pool_gb": st["bytes_front_pool"] / 1e9,
+        "backward_error": berr,
+        "roofline": roof,
+        "kernels": {k: {"ms_per_step": v["ms"] / K, "launches_per_step": v["launches"] / K,
+                        "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] else None,
+                        "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 and v["bytes"] else None}
+                    for k, v in prof.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(nrhs=nrhs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

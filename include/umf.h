@@ -1,0 +1,155 @@
+/* umf.h -- C ABI of the MI355X multifrontal factor / update / solve library (libumf_hip.so).
+ *
+ * One plan = one nested-dissection factorization of a 2D/3D grid operator held in device
+ * memory (HBM) of the current HIP device.  Host arrays passed in are caller-owned and copied;
+ * device memory is plan-owned.  Every call is synchronous unless its name ends in _device,
+ * whose pointers are device pointers and whose work is ordered on the plan's stream.
+ * Calls on one plan must be serialized by the caller; distinct plans are independent.
+ *
+ * Return status: UMF_OK (0); negative = error (see umf_last_error()); positive = numerical
+ * failure: a non-positive pivot, with umf_last_error() naming the level, box and pivot column
+ * (the reference aborts on LAPACK info != 0, include/LinAlg/dense.hpp:187).
+ *
+ * Each entry point names the reference interface it stands in for.
+ */
+#ifndef UMF_H
+#define UMF_H 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UMF_OK 0
+#define UMF_EINVAL (-1)     /* bad argument */
+#define UMF_EGEOM (-2)      /* planner geometry error (reference throws std::runtime_error) */
+#define UMF_EDEVICE (-3)    /* HIP runtime error */
+#define UMF_ENOMEM (-4)     /* device allocation failed */
+#define UMF_EPATTERN (-5)   /* CSR pattern differs from the planned one */
+#define UMF_EUPDATEMAP (-6) /* a modified DOF maps outside the leaf box grid */
+#define UMF_ESTATE (-7)     /* call out of order (e.g. solve before factor) */
+
+typedef struct umf_plan umf_plan;
+
+/* Front-block selectors for umf_get_front_block (names of the reference's Box fields,
+ * include/DE/mf.hpp:45-55).  A22 is rebuilt as L*L^T from the stored Cholesky factor; XR, XL
+ * and S are formed from L and W = L^{-1} A21.  A11post is the update matrix handed to the
+ * parent; its diagonal separator blocks are never read by the reference (the parent replaces
+ * them from the separators, src/DE/mf2d.cpp:427-443) and hold S there instead, so they come
+ * back as NaN. */
+#define UMF_BLOCK_A22 0
+#define UMF_BLOCK_XL 1
+#define UMF_BLOCK_XR 2
+#define UMF_BLOCK_S 3
+#define UMF_BLOCK_A11POST 4
+#define UMF_BLOCK_L 5 /* lower Cholesky factor of A22 (no reference counterpart) */
+
+/* Replaces the constructors MF2D(A, Nx, Ny, width, maxSideLength) (include/DE/mf.hpp:67) and
+ * MF3D(A, Nx, Ny, Nz, width, maxSideLength) (include/DE/mf.hpp:173): plans the elimination
+ * tree from the grid, uploads the CSR operator (int32 indices, n x n, DOF id i + j*Nx [+ k*Nx*Ny])
+ * and allocates the device front pool.  dim = 2 or 3 (nz ignored for 2D).  The operator must be
+ * symmetric positive definite with a symmetric pattern and no duplicate entries. */
+int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_length, int n,
+                    const int* row_ptr, const int* col_ind, const double* val, umf_plan** plan);
+
+/* Symbolic analysis only (no device memory, no values): the elimination tree, DOF lists and
+ * assembly programs of umf_plan_create, for structure queries on hosts without a GPU.
+ * umf_factor / umf_update / umf_solve on such a plan return UMF_ESTATE. */
+int umf_plan_analyze(int dim, int nx, int ny, int nz, int width, int max_side_length, int n,
+                     const int* row_ptr, const int* col_ind, umf_plan** plan);
+
+/* Computes, without running it, the dirty sets an update with these modified DOFs would
+ * factor (MF2D::Update's marking, src/DE/mf2d.cpp:161-174, and the level recurrence of
+ * src/DE/mf2d.cpp:579-620); afterwards umf_dirty reports them.  Works on analyzed plans. */
+int umf_analyze_update(umf_plan* plan, int n_mod, const int* mod_dofs);
+
+/* Frees the plan and all its device memory (the reference's destructor). */
+int umf_plan_destroy(umf_plan* plan);
+
+/* Replaces MF2D::Factor / MF3D::Factor (src/DE/mf2d.cpp:449-500, src/DE/mf3d.cpp:705-766):
+ * factors every front, leaves to root, on the device. */
+int umf_factor(umf_plan* plan);
+
+/* Replaces MF2D::Update(A, modifiedDOFs) / MF3D::Update (src/DE/mf2d.cpp:150-181,
+ * src/DE/mf3d.cpp:208-248): installs the new CSR (same pattern as planned) and re-factors the
+ * reference's dirty fronts only: the leaf boxes holding the modified DOFs, then per level the
+ * parents of both neighbours of every touched separator, and the root. */
+int umf_update(umf_plan* plan, const int* row_ptr, const int* col_ind, const double* val, int n_mod,
+               const int* mod_dofs);
+
+/* Device-resident variant of umf_update: d_val holds the new CSR values (same pattern as
+ * planned, nnz doubles, device memory); the caller guarantees symmetry.  No reference
+ * counterpart (the reference copies a host Sparse). */
+int umf_update_device(umf_plan* plan, const double* d_val, int n_mod, const int* mod_dofs);
+
+/* Replaces MF2D::Solve / MF3D::Solve (src/DE/mf2d.cpp:976-1054, src/DE/mf3d.cpp:1237-1313) for
+ * nrhs right-hand sides at once: b and x are n x nrhs column-major host arrays with leading
+ * dimension ld >= n (x may equal b). */
+int umf_solve(umf_plan* plan, const double* b, double* x, int nrhs, int ld);
+
+/* Replaces MF2D::Apply / MF3D::Apply (src/DE/mf2d.cpp:904-974, src/DE/mf3d.cpp:1164-1235):
+ * applies the factored operator to nrhs columns (host arrays, as umf_solve). */
+int umf_apply(umf_plan* plan, const double* x, double* y, int nrhs, int ld);
+
+/* Device-pointer variants of umf_solve / umf_apply (no reference counterpart: the reference
+ * takes one host vector per call).  d_b / d_x are n x nrhs column-major in device memory. */
+int umf_solve_device(umf_plan* plan, const double* d_b, double* d_x, int nrhs, int ld);
+int umf_apply_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, int ld);
+
+/* CSR SpMV y = A x with the plan's current operator, device pointers (the reference's
+ * Sparse::operator*(Vec), src/LinAlg/sparse.cpp:63-77), for residuals. */
+int umf_spmv_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, int ld);
+
+/* Last error message of the calling thread. */
+const char* umf_last_error(void);
+
+/* ---- structure queries (parity with the reference's private state, include/DE/mf.hpp:103-139) */
+/* levels_ and the leaf box widths boxWidth{X,Y,Z}Bottom_ (axis 0..2). */
+int umf_levels(const umf_plan* plan);
+int umf_leaf_width(const umf_plan* plan, int axis);
+int umf_num_boxes(const umf_plan* plan, int level);
+/* Separators between boxes of `level` (1 <= level < levels), the reference's edgeData_[level-1]
+ * / faceData_[level-1]. */
+int umf_num_seps(const umf_plan* plan, int level);
+/* Box DOF lists: which = 0 intDOFs, 1 bdDOFs.  Writes to out when non-null; returns the count. */
+int umf_box_dofs(const umf_plan* plan, int level, int box, int which, int* out);
+int umf_sep_dofs(const umf_plan* plan, int level, int sep, int* out);
+/* Dirty sets of the last factor/update: which = 0 boxes factored at `level`, 1 separators that
+ * received Schur contributions at `level`, 2 separators rebuilt from their children at `level`.
+ * Sorted ascending.  Returns the count. */
+int umf_dirty(const umf_plan* plan, int level, int which, int* out);
+
+/* ---- factor-entry queries (parity); see UMF_BLOCK_*.  Writes column-major rows x cols. */
+int umf_get_front_block(umf_plan* plan, int level, int box, int which, double* out, int* rows, int* cols);
+/* The separator's A11 (k x k), symmetric, both triangles filled. */
+int umf_get_sep_block(umf_plan* plan, int level, int sep, double* out, int* k);
+
+/* ---- measurement */
+typedef struct umf_stats {
+  double t_factor_ms;   /* device time of the last umf_factor */
+  double t_update_ms;   /* device time of the last umf_update */
+  double t_solve_ms;    /* device time of the last umf_solve / umf_solve_device */
+  double flops_factor;  /* sum over fronts of n^3/3 + n^2 m + n m^2 */
+  double flops_update;  /* the same sum over the fronts of the last update */
+  double flops_solve;   /* nrhs * sum (2 n^2 + 4 n m) of the last solve */
+  double bytes_front_pool; /* device bytes held by fronts and separators */
+  int n_fronts;
+  int n_fronts_update;
+} umf_stats;
+int umf_get_stats(const umf_plan* plan, umf_stats* out);
+
+/* Per-kernel timing (no reference counterpart; the reference's HASCPP11TIMING timers are
+ * the nearest, src/DE/mf2d.cpp:463-499): when enabled, every kernel launch of the plan is
+ * bracketed by HIP events on the plan stream and its algorithmic flops / bytes are counted.
+ * umf_profile_read(plan, i, ...) fills entry i and returns the number of kernels seen. */
+int umf_profile_enable(umf_plan* plan, int on);
+int umf_profile_reset(umf_plan* plan);
+int umf_profile_read(const umf_plan* plan, int i, char* name, int name_len, double* ms_total, int* launches,
+                     double* flops, double* bytes);
+
+/* hipStream_t of the plan (for callers ordering their own work), returned as void*. */
+void* umf_stream(umf_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

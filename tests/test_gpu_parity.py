@@ -1,0 +1,173 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the same inputs.
+
+Tolerances: factor entries within 1e-11 relative (normwise per block),
+solution within 1e-10 relative 2-norm; structure (DOF lists, separators, dirty sets) bit-exact.
+The oracle factors with LU + partial pivoting as the reference does; the library factors with
+Cholesky, so parity is judged on quantities both define uniquely: A22, XR, XL, S, the update
+matrix A11post off its diagonal separator blocks, separator A11 blocks and the solution.
+"""
+import numpy as np
+import pytest
+
+from oracle_py import Factorization, coeff_field, laplacian, moddofs
+
+pytestmark = pytest.mark.gpu
+
+FACTOR_TOL = 1e-11
+SOLVE_TOL = 1e-10
+
+# (dims, maxSideLength, coefficient field, width)
+CASES = [
+    ((31, 31), 3, "uniform", 1),
+    ((23, 57), 3, "lognormal", 1),
+    ((127, 127), 8, "uniform", 1),   # the 2D 127^2 configuration's tree (levels 4, 15^2 leaves)
+    ((64, 47), 5, "lognormal", 1),
+    ((40, 33), 5, "rand", 2),
+    ((9, 9, 9), 3, "rand", 1),
+    ((15, 15, 15), 3, "lognormal", 1),
+    ((11, 14, 9), 2, "lognormal", 1),
+    ((17, 13, 15), 2, "rand", 2),
+]
+
+
+def ids(c):
+    return "x".join(map(str, c[0])) + f"-m{c[1]}-{c[2]}-w{c[3]}"
+
+
+def _mk(dims, maxs, kind, w, seed=11):
+    import umf
+    co = coeff_field(dims, kind, seed=seed, width=w)
+    A = laplacian(dims, co, width=w)
+    G = (umf.MF2D if len(dims) == 2 else umf.MF3D)(A, *dims, width=w, maxSideLength=maxs)
+    O = Factorization(A, dims, w, maxs, "orc")
+    return co, A, G, O
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _compare_blocks(G, O):
+    worst = {}
+    for ell in range(O.L):
+        for b in range(O.nboxes(ell)):
+            names = ["A22"] if ell == 0 else ["A22", "XR", "XL", "S", "A11post"]
+            for blk in names:
+                g = G.front_block(ell, b, blk)
+                o = O.box_block(ell, b, blk)
+                assert g.shape == o.shape, (ell, b, blk)
+                if blk == "A11post":
+                    mask = ~np.isnan(g)
+                    e = relerr(g[mask], o[mask]) if mask.any() else 0.0
+                else:
+                    e = relerr(g, o)
+                worst[blk] = max(worst.get(blk, 0.0), e)
+        if ell > 0:
+            for s in range(O.nseps(ell)):
+                worst["sepA11"] = max(worst.get("sepA11", 0.0), relerr(G.sep_block(ell, s), O.sep_block(ell, s)))
+    return worst
+
+
+@pytest.mark.parametrize("case", CASES, ids=ids)
+def test_factor_blocks_match_oracle(case):
+    dims, maxs, kind, w = case
+    _, A, G, O = _mk(dims, maxs, kind, w)
+    G.Factor()
+    O.factor()
+    worst = _compare_blocks(G, O)
+    print(worst)
+    for k, v in worst.items():
+        assert v <= FACTOR_TOL, (k, v)
+
+
+@pytest.mark.parametrize("case", CASES, ids=ids)
+def test_solve_apply_match_oracle(case):
+    dims, maxs, kind, w = case
+    _, A, G, O = _mk(dims, maxs, kind, w)
+    G.Factor()
+    O.factor()
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, A.n)
+    b = A.matvec(x)
+    xg = G.Solve(b)
+    xo = O.solve(b)
+    assert relerr(xg, xo) <= SOLVE_TOL
+    assert relerr(xg, x) <= 1e3 * SOLVE_TOL  # Solve(Ax) = x, the reference's own check
+    yg = G.Apply(x)
+    yo = O.apply(x)
+    assert relerr(yg, yo) <= SOLVE_TOL
+    assert relerr(yg, b) <= 1e-12  # Apply(x) = Ax
+
+
+@pytest.mark.parametrize("case", CASES[:4] + CASES[5:8], ids=ids)
+def test_update_matches_oracle(case):
+    dims, maxs, kind, w = case
+    co, A, G, O = _mk(dims, maxs, kind, w)
+    G.Factor()
+    O.factor()
+    rng = np.random.default_rng(7)
+    co2 = co.copy()
+    lo = [int(rng.integers(0, d - 3)) for d in dims]
+    co2[tuple(slice(l, l + 4) for l in lo)] *= 9.0
+    A2 = laplacian(dims, co2, width=w)
+    mod = moddofs(A, A2)
+    G.Update(A2, mod)
+    O.update(A2, mod)
+    for ell in range(O.L):
+        for which in (0, 1, 2):
+            assert np.array_equal(G.dirty(ell, which), O.trace(ell, which)), (ell, which)
+    worst = _compare_blocks(G, O)
+    for k, v in worst.items():
+        assert v <= FACTOR_TOL, (k, v)
+    x = rng.uniform(-1, 1, A.n)
+    b2 = A2.matvec(x)
+    assert relerr(G.Solve(b2), O.solve(b2)) <= SOLVE_TOL
+    assert relerr(G.Solve(b2), x) <= 1e3 * SOLVE_TOL
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 16, 21])
+def test_multi_rhs_columns_independent(nrhs):
+    dims = (63, 61)
+    _, A, G, O = _mk(dims, 4, "lognormal", 1)
+    G.Factor()
+    O.factor()
+    rng = np.random.default_rng(nrhs)
+    B = rng.standard_normal((A.n, nrhs))
+    X = G.Solve(B)
+    for j in range(nrhs):
+        assert relerr(X[:, j], O.solve(B[:, j])) <= SOLVE_TOL
+    Y = G.Apply(B)
+    for j in range(nrhs):
+        assert relerr(Y[:, j], O.apply(B[:, j])) <= SOLVE_TOL
+
+
+def test_non_spd_reports_pivot():
+    import umf
+    dims = (31, 31)
+    co = coeff_field(dims, "uniform")
+    A = laplacian(dims, co)
+    A.val = -A.val  # negative definite: the first pivot is negative
+    G = umf.MF2D(A, *dims, width=1, maxSideLength=3)
+    with pytest.raises(ArithmeticError, match="non-positive pivot"):
+        G.Factor()
+
+
+def test_geometry_error_message():
+    import umf
+    dims = (9, 9)
+    A = laplacian(dims, coeff_field(dims, "uniform"))
+    with pytest.raises(RuntimeError, match="recursion depth is 1 level"):
+        umf.MF2D(A, *dims, width=1, maxSideLength=8)
+
+
+def test_update_pattern_mismatch_rejected():
+    import umf
+    dims = (31, 31)
+    _, A, G, _ = _mk(dims, 3, "uniform", 1)
+    G.Factor()
+    B = laplacian(dims, coeff_field(dims, "uniform"), width=1)
+    B.ci = B.ci.copy()
+    B.ci[0], B.ci[1] = B.ci[1], B.ci[0]
+    with pytest.raises(ValueError):
+        G.Update(B, [0])

@@ -1,0 +1,34 @@
+// kernels.h -- device-side descriptors shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "plan.h"
+
+namespace umf {
+
+// One front as the kernels see it.  The front F is (n+m) x (n+m), column-major, ld = N = n+m,
+// at arena + off:  rows/cols [0,n) = intDOFs (pivot block A22), [n,N) = bdDOFs.
+// After factorization: L11 in the lower part of F[0:n,0:n], L21 = A12 L11^{-T} in F[n:N,0:n],
+// F[n:N,n:N] (lower) = A11 + S off the diagonal separator blocks and S = -L21 L21^T on them.
+struct FrontDev {
+  i64 off;      // arena offset of F
+  i64 int_off;  // DOF list offsets
+  i64 bd_off;
+  i64 x_off;    // solve scratch offset (N rows per right-hand-side column)
+  int n, m, N, nsep;
+  int sep_pos[6], sep_len[6];
+};
+struct SepDev {
+  i64 off;  // arena offset of the k x k A11
+  i64 dof_off;
+  int k;
+  int box[2];  // front ids
+  int pos[2];  // offsets inside each neighbour's bdDOFs
+};
+struct MatDev {
+  i64 off;
+  int ld;
+  int cols;
+};
+
+}  // namespace umf

@@ -1,0 +1,948 @@
+// runtime.hip -- device front pool, level scheduler and the C ABI of libumf_hip.so.
+//
+// One umf_plan owns: the planner output (host), one device arena holding every front and
+// separator matrix (planned once, 256-byte aligned), device copies of the DOF lists, assembly
+// programs and CSR operator, and one HIP stream.  Factor / Update walk the tree level by level
+// (leaves first), each stage one batched launch over the level's (dirty) fronts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/umf.h"
+#include "kernels.hip"
+#include "plan.h"
+
+using namespace umf;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(UMF_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+hipError_t upload(T** d, const std::vector<T>& h) {
+  *d = nullptr;
+  if (h.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void**)d, sizeof(T) * h.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+}  // namespace
+
+// Per-kernel event timing (enabled by umf_profile_enable).
+struct KStat {
+  std::string name;
+  double ms = 0, flops = 0, bytes = 0;
+  int launches = 0;
+};
+struct Prof {
+  bool on = false;
+  std::vector<KStat> stats;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Pending {
+    int k;
+    double flops, bytes;
+  };
+  std::vector<Pending> pend;
+  int find(const char* name) {
+    for (size_t i = 0; i < stats.size(); i++)
+      if (stats[i].name == name) return (int)i;
+    stats.push_back(KStat());
+    stats.back().name = name;
+    return (int)stats.size() - 1;
+  }
+};
+
+struct umf_plan {
+  Plan P;
+  Prof prof;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* arena = nullptr;
+  int* d_dofs = nullptr;
+  MatDev* d_mats = nullptr;
+  FrontDev* d_fronts = nullptr;
+  SepDev* d_seps = nullptr;
+  Triple* d_tri = nullptr;
+  i64* d_tri_off = nullptr;
+  BlockOp* d_ops = nullptr;
+  Tile* d_tiles = nullptr;
+  i64* d_tile_off = nullptr;
+  int* d_rp = nullptr;
+  int* d_ci = nullptr;
+  double* d_val = nullptr;
+  int* d_err = nullptr;
+  int* d_ids = nullptr;  // id lists of the current schedule
+  size_t ids_cap = 0;
+  double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
+  double* d_w = nullptr;   // solve work vector n x RB
+  i64 xs_elems = 0;
+  std::vector<FrontDev> fronts_h;
+  std::vector<SepDev> seps_h;
+  std::vector<double> val_h;  // current operator values
+  bool factored = false;
+  bool host_only = false;  // analyzed plan without device memory
+  Plan::Dirty last;
+  umf_stats stats{};
+  int max_tiles_per_mat = 1;
+};
+
+namespace {
+
+// Schedule of one factor/update pass: per level, the front ids to (re)factor, the separator ids
+// to rebuild and the separator ids receiving Schur contributions, all uploaded in one buffer.
+struct Sched {
+  struct Lvl {
+    int f_off, f_cnt;      // fronts (front ids == matrix ids)
+    int s_off, s_cnt;      // separators (separator ids)
+    int sm_off, sm_cnt;    // separator matrix ids (NF + id) for zero/assemble
+    int maxN, maxn, maxm;  // extents over the level's fronts
+    int max_k;
+    int f_grid, s_grid;    // assembly grid widths (tiles / triples per matrix)
+  };
+  std::vector<Lvl> lv;
+  std::vector<int> ids;
+};
+
+Sched make_sched(const Plan& P, const Plan::Dirty& D) {
+  Sched S;
+  const Grid& g = P.g;
+  S.lv.assign(g.L, {});
+  int NF = P.nfronts();
+  auto grid_of = [&](int mid) {
+    i64 tiles = P.tile_off[mid + 1] - P.tile_off[mid];
+    i64 tri = P.tri_off[mid + 1] - P.tri_off[mid];
+    return (int)std::min<i64>(std::max<i64>(std::max(tiles, (tri + 1023) / 1024), 1), 1024);
+  };
+  for (int ell = g.L - 1; ell >= 0; ell--) {
+    Sched::Lvl& L = S.lv[ell];
+    L.f_off = (int)S.ids.size();
+    L.maxN = L.maxn = L.maxm = 0;
+    L.f_grid = L.s_grid = 1;
+    for (int b : D.boxes[ell]) {
+      int f = P.box_gid[ell][b];
+      S.ids.push_back(f);
+      L.maxN = std::max(L.maxN, P.fronts[f].n + P.fronts[f].m);
+      L.maxn = std::max(L.maxn, P.fronts[f].n);
+      L.maxm = std::max(L.maxm, P.fronts[f].m);
+      L.f_grid = std::max(L.f_grid, grid_of(f));
+    }
+    L.f_cnt = (int)S.ids.size() - L.f_off;
+    L.s_off = (int)S.ids.size();
+    L.max_k = 0;
+    if (ell >= 1)
+      for (int e : D.seps[ell]) {
+        int s = P.sep_gid[ell][e];
+        S.ids.push_back(s);
+        L.max_k = std::max(L.max_k, P.seps[s].k);
+      }
+    L.s_cnt = (int)S.ids.size() - L.s_off;
+    L.sm_off = (int)S.ids.size();
+    if (ell >= 1)
+      for (int e : D.seps[ell]) {
+        S.ids.push_back(NF + P.sep_gid[ell][e]);
+        L.s_grid = std::max(L.s_grid, grid_of(NF + P.sep_gid[ell][e]));
+      }
+    L.sm_cnt = (int)S.ids.size() - L.sm_off;
+  }
+  return S;
+}
+
+int ensure_ids(umf_plan* p, size_t n) {
+  if (n <= p->ids_cap) return UMF_OK;
+  if (p->d_ids) (void)hipFree(p->d_ids);
+  p->d_ids = nullptr;
+  HIPCHK(hipMalloc((void**)&p->d_ids, sizeof(int) * std::max<size_t>(n, 1)));
+  p->ids_cap = n;
+  return UMF_OK;
+}
+
+// Launch on the plan stream; with profiling on, bracket it by a pair of pooled events and
+// remember its algorithmic work.
+template <class K, class... Args>
+void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, Args... args) {
+  Prof& pr = p->prof;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (pr.on) {
+    while (pr.pool.size() < pr.used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) break;
+      pr.pool.push_back(e);
+    }
+    if (pr.pool.size() >= pr.used + 2) {
+      a = pr.pool[pr.used];
+      b = pr.pool[pr.used + 1];
+      pr.used += 2;
+      pr.pend.push_back(Prof::Pending{pr.find(name), flops, bytes});
+      (void)hipEventRecord(a, p->stream);
+    }
+  }
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, p->stream, args...);
+  if (b) (void)hipEventRecord(b, p->stream);
+}
+
+// After the stream has been synchronized: fold the pending event pairs into the totals.
+void prof_collect(umf_plan* p) {
+  Prof& pr = p->prof;
+  for (size_t i = 0; i < pr.pend.size(); i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pr.pool[2 * i], pr.pool[2 * i + 1]) != hipSuccess) ms = 0;
+    KStat& k = pr.stats[pr.pend[i].k];
+    k.ms += ms;
+    k.launches++;
+    k.flops += pr.pend[i].flops;
+    k.bytes += pr.pend[i].bytes;
+  }
+  pr.pend.clear();
+  pr.used = 0;
+}
+
+// max 64x64 tiles of the lower region rows [s, rend) x cols [s, cend)
+inline int tiles_of(int s, int rend, int cend) {
+  if (s >= cend) return 0;
+  int T = (cend - s + 63) / 64, RT = (rend - s + 63) / 64;
+  return T * (T + 1) / 2 + (RT - T) * T;
+}
+
+// Run the factorization over the dirty schedule on the plan stream.
+int run_factor(umf_plan* p, const Plan::Dirty& D) {
+  const Plan& P = p->P;
+  const Grid& g = P.g;
+  Sched S = make_sched(P, D);
+  int rc = ensure_ids(p, S.ids.size());
+  if (rc) return rc;
+  if (!S.ids.empty())
+    HIPCHK(hipMemcpyAsync(p->d_ids, S.ids.data(), sizeof(int) * S.ids.size(), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemsetAsync(p->d_err, 0, sizeof(int) * 4, p->stream));
+  const int* ids = p->d_ids;
+  const int NF = P.nfronts();
+  auto lowerN = [](double N) { return N * (N + 1) / 2; };
+  for (int ell = g.L - 1; ell >= 0; ell--) {
+    const Sched::Lvl& L = S.lv[ell];
+    // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
+    // then per triple (12 B index + 8 B value read + 8 B write), per copied element 16 B
+    auto asm_work = [&](int off, int cnt, double& zb, double& ab) {
+      zb = ab = 0;
+      for (int i = 0; i < cnt; i++) {
+        int mid = S.ids[off + i];
+        zb += 8.0 * lowerN(P.mats[mid].rows);
+        ab += 28.0 * (P.tri_off[mid + 1] - P.tri_off[mid]);
+        for (i64 t = P.tile_off[mid]; t < P.tile_off[mid + 1]; t++) {
+          const BlockOp& op = P.ops[P.tiles[t].op];
+          int r = std::min(64, op.nr - P.tiles[t].r0), c = std::min(64, op.nc - P.tiles[t].c0);
+          ab += 16.0 * r * c;
+        }
+      }
+    };
+    // 1. separators of this level rebuilt from their children (upper levels) or from the CSR
+    //    operator (leaf level): zero + scatter
+    if (L.sm_cnt > 0) {
+      double zb, ab;
+      asm_work(L.sm_off, L.sm_cnt, zb, ab);
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower, dim3(std::min(std::max(L.max_k, 1), 256), L.sm_cnt),
+             ids + L.sm_off, (const MatDev*)p->d_mats, p->arena);
+      launch(p, "k_assemble", 0, ab, k_assemble, dim3(L.s_grid, L.sm_cnt), ids + L.sm_off, (const MatDev*)p->d_mats,
+             p->arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
+             (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+    }
+    if (L.f_cnt == 0) continue;
+    // 2. front assembly
+    {
+      double zb, ab;
+      asm_work(L.f_off, L.f_cnt, zb, ab);
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
+             (const MatDev*)p->d_mats, p->arena);
+      launch(p, "k_assemble", 0, ab, k_assemble, dim3(L.f_grid, L.f_cnt), ids + L.f_off, (const MatDev*)p->d_mats,
+             p->arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
+             (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+    }
+    // 3. blocked partial Cholesky of the pivot columns
+    for (int k0 = 0; k0 < L.maxn; k0 += NB) {
+      int rows = 0, tiles = 0;
+      bool partial = false;
+      double fpot = 0, ftrs = 0, fgem = 0, bgem = 0, btrs = 0;
+      for (int i = 0; i < L.f_cnt; i++) {
+        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+        if (k0 >= F.n) continue;
+        int kb = std::min(NB, F.n - k0), N = F.n + F.m;
+        partial |= kb < NB;
+        rows = std::max(rows, N - k0 - kb);
+        tiles = std::max(tiles, tiles_of(k0 + kb, N, F.n));
+        double s0 = k0 + kb, nc = F.n - s0, nr = N - s0;
+        fpot += (double)kb * kb * kb / 3.0;
+        ftrs += nr * kb * kb;
+        btrs += 16.0 * nr * kb;
+        if (nc > 0) {
+          double elems = nc * nr - nc * (nc - 1) / 2;  // lower part of the trailing region
+          fgem += 2.0 * kb * elems;
+          bgem += 16.0 * elems + 8.0 * kb * (nr + nc);
+        }
+      }
+      launch(p, "k_potrf_diag", fpot, 0, k_potrf_diag, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+             p->arena, k0, p->d_err);
+      if (rows > 0) {
+        launch(p, "k_trsm_panel", ftrs, btrs, k_trsm_panel<true>, dim3((rows + 255) / 256, L.f_cnt), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, p->arena, k0);
+        if (partial)
+          launch(p, "k_trsm_panel_tail", 0, 0, k_trsm_panel<false>, dim3((rows + 255) / 256, L.f_cnt),
+                 ids + L.f_off, (const FrontDev*)p->d_fronts, p->arena, k0);
+      }
+      if (tiles > 0)
+        launch(p, "k_gemm_panel", fgem, bgem, k_gemm<0>, dim3(tiles, L.f_cnt), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, p->arena, k0);
+    }
+    // 4. Schur complement into the trailing block
+    if (L.maxm > 0) {
+      int tiles = 0;
+      double fl = 0, by = 0;
+      for (int i = 0; i < L.f_cnt; i++) {
+        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+        tiles = std::max(tiles, tiles_of(F.n, F.n + F.m, F.n + F.m));
+        fl += (double)F.n * F.m * (F.m + 1);
+        by += 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
+      }
+      if (tiles > 0)
+        launch(p, "k_gemm_schur", fl, by, k_gemm<1>, dim3(tiles, L.f_cnt), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, p->arena, 0);
+    }
+    // 5. separators of this level receive both neighbours' Schur blocks
+    if (L.s_cnt > 0) {
+      double by = 0;
+      for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
+      launch(p, "k_sep_add", 0, by, k_sep_add, dim3(std::min(std::max(L.max_k, 1), 256), L.s_cnt), ids + L.s_off,
+             (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, p->arena);
+    }
+  }
+  (void)NF;
+  HIPCHK(hipGetLastError());
+  return UMF_OK;
+}
+
+int check_err(umf_plan* p) {
+  int h[4];
+  HIPCHK(hipMemcpy(h, p->d_err, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[0]) {
+    const FrontInfo& F = p->P.fronts[h[1]];
+    char buf[256];
+    snprintf(buf, sizeof buf,
+             "non-positive pivot: level %d box %d pivot column %d (front of n=%d): operator not SPD", F.level, F.box,
+             h[2], F.n);
+    g_err = buf;
+    return 1 + h[2];
+  }
+  return UMF_OK;
+}
+
+int ensure_solve_scratch(umf_plan* p) {
+  if (p->d_xs) return UMF_OK;
+  HIPCHK(hipMalloc((void**)&p->d_xs, sizeof(double) * std::max<i64>(p->xs_elems, 1)));
+  HIPCHK(hipMalloc((void**)&p->d_w, sizeof(double) * (size_t)p->P.n * RB));
+  return UMF_OK;
+}
+
+// Level lists of all fronts / separators (solve and apply touch every front).
+struct AllLists {
+  std::vector<int> ids;
+  std::vector<int> f_off, f_cnt, s_off, s_cnt;
+};
+AllLists all_lists(const Plan& P) {
+  AllLists A;
+  int L = P.g.L;
+  A.f_off.assign(L, 0); A.f_cnt.assign(L, 0); A.s_off.assign(L, 0); A.s_cnt.assign(L, 0);
+  for (int ell = 0; ell < L; ell++) {
+    A.f_off[ell] = (int)A.ids.size();
+    for (int f : P.box_gid[ell]) A.ids.push_back(f);
+    A.f_cnt[ell] = (int)A.ids.size() - A.f_off[ell];
+    A.s_off[ell] = (int)A.ids.size();
+    if (ell >= 1)
+      for (int s : P.sep_gid[ell]) A.ids.push_back(s);
+    A.s_cnt[ell] = (int)A.ids.size() - A.s_off[ell];
+  }
+  return A;
+}
+
+// Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the work vector d_w (ld = n).
+int sweep(umf_plan* p, int mode, int nr) {
+  const Plan& P = p->P;
+  int L = P.g.L;
+  AllLists A = all_lists(P);
+  int rc = ensure_ids(p, A.ids.size());
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(p->d_ids, A.ids.data(), sizeof(int) * A.ids.size(), hipMemcpyHostToDevice, p->stream));
+  const int* ids = p->d_ids;
+  int n = P.n;
+  const FrontDev* fr = p->d_fronts;
+  // per level: solve flops nr*(2n^2 + 4nm) split evenly over the two sweeps; bytes = factor read
+  auto work = [&](int ell, double& fl, double& by) {
+    fl = by = 0;
+    for (int f : P.box_gid[ell]) {
+      double nn = P.fronts[f].n, mm = P.fronts[f].m;
+      fl += nr * (nn * nn + 2 * nn * mm);
+      by += 8.0 * (nn * (nn + 1) / 2 + nn * mm) + 16.0 * nr * (nn + mm);
+    }
+  };
+  double fl, by;
+  if (mode == 0) {
+    for (int ell = L - 1; ell >= 1; ell--) {
+      work(ell, fl, by);
+      launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+      launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
+             (const int*)p->d_dofs, p->d_w, n, (const double*)p->d_xs, nr);
+    }
+    work(0, fl, by);
+    launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(1), ids + A.f_off[0], fr, (const double*)p->arena,
+           (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+    for (int ell = 0; ell < L; ell++) {
+      work(ell, fl, by);
+      launch(p, "k_solve_bwd", fl, by, k_solve_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+    }
+  } else {
+    for (int ell = L - 1; ell >= 0; ell--)
+      launch(p, "k_apply_up", 0, 0, k_apply_up, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const double*)p->arena,
+             (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+    for (int ell = 1; ell < L; ell++) {
+      launch(p, "k_apply_down", 0, 0, k_apply_down, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const double*)p->arena, p->d_xs, nr);
+      launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
+             (const int*)p->d_dofs, p->d_w, n, (const double*)p->d_xs, nr);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return UMF_OK;
+}
+
+// Shared body of the solve/apply entry points: columns in chunks of RB through d_w.
+int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, int ld, bool device) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (!p->factored) return fail(UMF_ESTATE, "factor before solve/apply");
+  if (nrhs < 0 || ld < p->P.n) return fail(UMF_EINVAL, "bad nrhs or ld");
+  if (nrhs == 0) return UMF_OK;
+  int rc = ensure_solve_scratch(p);
+  if (rc) return rc;
+  int n = p->P.n;
+  hipMemcpyKind kin = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  hipMemcpyKind kout = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  for (int c0 = 0; c0 < nrhs; c0 += RB) {
+    int nr = std::min(RB, nrhs - c0);
+    HIPCHK(hipMemcpy2DAsync(p->d_w, sizeof(double) * n, in + (size_t)c0 * ld, sizeof(double) * ld,
+                            sizeof(double) * n, nr, kin, p->stream));
+    rc = sweep(p, mode, nr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy2DAsync(out + (size_t)c0 * ld, sizeof(double) * ld, p->d_w, sizeof(double) * n,
+                            sizeof(double) * n, nr, kout, p->stream));
+  }
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  prof_collect(p);
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  if (mode == 0) {
+    p->stats.t_solve_ms = ms;
+    double fl = 0;
+    for (const FrontInfo& F : p->P.fronts) fl += 2.0 * F.n * F.n + 4.0 * F.n * F.m;
+    p->stats.flops_solve = fl * nrhs;
+  }
+  return UMF_OK;
+}
+
+void free_plan(umf_plan* p) {
+  void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
+                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_ids, p->d_xs, p->d_w};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+}
+
+// Symmetric pattern, symmetric values, no duplicate entries (Cholesky fronts need all three).
+std::string check_operator(int n, const int* rp, const int* ci, const double* v) {
+  if (rp[0] != 0) return "row_ptr[0] must be 0";
+  for (int i = 0; i < n; i++)
+    if (rp[i + 1] < rp[i]) return "row_ptr must be non-decreasing";
+  int nnz = rp[n];
+  std::vector<int> cnt(n + 1, 0);
+  for (int k = 0; k < nnz; k++) {
+    if (ci[k] < 0 || ci[k] >= n) return "column index out of range";
+    cnt[ci[k] + 1]++;
+  }
+  for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> trow(nnz);
+  std::vector<double> tval(nnz);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+      int j = ci[k];
+      trow[fill[j]] = i;
+      tval[fill[j]] = v ? v[k] : 0.0;
+      fill[j]++;
+    }
+  // row i of A (sorted copy) must equal column i of A = row i of A^T (already sorted by row)
+  std::vector<std::pair<int, double>> row;
+  for (int i = 0; i < n; i++) {
+    row.clear();
+    for (int k = rp[i]; k < rp[i + 1]; k++) row.push_back({ci[k], v ? v[k] : 0.0});
+    std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+      return a.first < b.first;
+    });
+    for (size_t q = 1; q < row.size(); q++)
+      if (row[q].first == row[q - 1].first) return "duplicate entry in row " + std::to_string(i);
+    if ((int)row.size() != cnt[i + 1] - cnt[i]) return "pattern is not symmetric (row " + std::to_string(i) + ")";
+    for (size_t q = 0; q < row.size(); q++) {
+      int k = cnt[i] + (int)q;
+      if (trow[k] != row[q].first) return "pattern is not symmetric (row " + std::to_string(i) + ")";
+      if (v && tval[k] != row[q].second) return "values are not symmetric (row " + std::to_string(i) + ")";
+    }
+  }
+  return "";
+}
+
+}  // namespace
+
+// ============================================================================== C ABI
+extern "C" {
+
+const char* umf_last_error(void) { return g_err.c_str(); }
+
+int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_length, int n, const int* row_ptr,
+                    const int* col_ind, const double* val, umf_plan** out) {
+  if (!out || !row_ptr || !col_ind || !val) return fail(UMF_EINVAL, "null argument");
+  *out = nullptr;
+  if (n <= 0) return fail(UMF_EINVAL, "n must be positive");
+  std::string e = check_operator(n, row_ptr, col_ind, val);
+  if (!e.empty()) return fail(UMF_EINVAL, e);
+  umf_plan* p = new umf_plan();
+  bool geom = false;
+  e = build_plan(p->P, dim, nx, ny, nz, width, max_side_length, n, row_ptr, col_ind, geom);
+  if (!e.empty()) {
+    delete p;
+    return fail(geom ? UMF_EGEOM : UMF_EINVAL, e);
+  }
+  Plan& P = p->P;
+  p->val_h.assign(val, val + row_ptr[n]);
+  // device descriptors
+  std::vector<MatDev> mats;
+  for (const MatDesc& M : P.mats) mats.push_back(MatDev{M.off, M.rows, M.cols});
+  i64 xo = 0;
+  for (const FrontInfo& F : P.fronts) {
+    FrontDev D{};
+    D.off = P.mats[&F - &P.fronts[0]].off;
+    D.int_off = F.int_off;
+    D.bd_off = F.bd_off;
+    D.n = F.n;
+    D.m = F.m;
+    D.N = F.n + F.m;
+    D.nsep = F.nsep;
+    for (int i = 0; i < 6; i++) {
+      D.sep_pos[i] = i < F.nsep ? F.sep_pos[i] : 0;
+      D.sep_len[i] = i < F.nsep ? F.sep_len[i] : 0;
+    }
+    D.x_off = xo;
+    xo += (i64)D.N * RB;
+    p->fronts_h.push_back(D);
+  }
+  p->xs_elems = xo;
+  for (const SepInfo& S : P.seps) {
+    SepDev D{};
+    D.off = P.mats[P.nfronts() + (&S - &P.seps[0])].off;
+    D.dof_off = S.dof_off;
+    D.k = S.k;
+    D.box[0] = S.box[0]; D.box[1] = S.box[1];
+    D.pos[0] = S.pos[0]; D.pos[1] = S.pos[1];
+    p->seps_h.push_back(D);
+  }
+  int mt = 1;
+  for (size_t m = 0; m + 1 < P.tile_off.size(); m++) mt = std::max<i64>(mt, P.tile_off[m + 1] - P.tile_off[m]);
+  p->max_tiles_per_mat = mt;
+
+  auto bail = [&](int code, const std::string& msg) {
+    free_plan(p);
+    delete p;
+    return fail(code, msg);
+  };
+  hipError_t he;
+#define UP(d, h)                                                         \
+  if ((he = upload(&(d), h)) != hipSuccess)                              \
+    return bail(he == hipErrorOutOfMemory ? UMF_ENOMEM : UMF_EDEVICE, std::string("upload: ") + hipGetErrorString(he));
+  if ((he = hipStreamCreate(&p->stream)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipEventCreate(&p->ev0)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipEventCreate(&p->ev1)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMalloc((void**)&p->arena, sizeof(double) * std::max<i64>(P.arena_elems, 1))) != hipSuccess)
+    return bail(UMF_ENOMEM, std::string("front pool of ") + std::to_string(P.arena_elems * 8.0 / 1e9) +
+                                " GB: " + hipGetErrorString(he));
+  UP(p->d_dofs, P.dofs);
+  UP(p->d_mats, mats);
+  UP(p->d_fronts, p->fronts_h);
+  UP(p->d_seps, p->seps_h);
+  UP(p->d_tri, P.triples);
+  UP(p->d_tri_off, P.tri_off);
+  UP(p->d_ops, P.ops);
+  UP(p->d_tiles, P.tiles);
+  UP(p->d_tile_off, P.tile_off);
+  UP(p->d_rp, P.rp);
+  UP(p->d_ci, P.ci);
+  UP(p->d_val, p->val_h);
+#undef UP
+  if ((he = hipMalloc((void**)&p->d_err, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  double fl = 0;
+  for (int f = 0; f < P.nfronts(); f++) fl += P.front_flops(f);
+  p->stats.flops_factor = fl;
+  p->stats.bytes_front_pool = 8.0 * P.arena_elems;
+  p->stats.n_fronts = P.nfronts();
+  *out = p;
+  return UMF_OK;
+}
+
+int umf_plan_analyze(int dim, int nx, int ny, int nz, int width, int max_side_length, int n, const int* row_ptr,
+                     const int* col_ind, umf_plan** out) {
+  if (!out || !row_ptr || !col_ind) return fail(UMF_EINVAL, "null argument");
+  *out = nullptr;
+  if (n <= 0) return fail(UMF_EINVAL, "n must be positive");
+  std::string e = check_operator(n, row_ptr, col_ind, nullptr);
+  if (!e.empty()) return fail(UMF_EINVAL, e);
+  umf_plan* p = new umf_plan();
+  bool geom = false;
+  e = build_plan(p->P, dim, nx, ny, nz, width, max_side_length, n, row_ptr, col_ind, geom);
+  if (!e.empty()) {
+    delete p;
+    return fail(geom ? UMF_EGEOM : UMF_EINVAL, e);
+  }
+  p->host_only = true;
+  p->last = p->P.all_dirty();
+  double fl = 0;
+  for (int f = 0; f < p->P.nfronts(); f++) fl += p->P.front_flops(f);
+  p->stats.flops_factor = fl;
+  p->stats.bytes_front_pool = 8.0 * p->P.arena_elems;
+  p->stats.n_fronts = p->P.nfronts();
+  *out = p;
+  return UMF_OK;
+}
+
+int umf_analyze_update(umf_plan* p, int n_mod, const int* mod_dofs) {
+  if (!p || (n_mod > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  std::vector<int> mod(mod_dofs, mod_dofs + std::max(n_mod, 0));
+  Plan::Dirty D;
+  std::string e;
+  if (!p->P.dirty_from_dofs(mod, D, e)) return fail(UMF_EUPDATEMAP, e);
+  p->last = D;
+  double fl = 0;
+  int nf = 0;
+  for (int ell = 0; ell < p->P.g.L; ell++)
+    for (int b : D.boxes[ell]) {
+      fl += p->P.front_flops(p->P.box_gid[ell][b]);
+      nf++;
+    }
+  p->stats.flops_update = fl;
+  p->stats.n_fronts_update = nf;
+  return UMF_OK;
+}
+
+int umf_plan_destroy(umf_plan* p) {
+  if (!p) return UMF_OK;
+  free_plan(p);
+  delete p;
+  return UMF_OK;
+}
+
+int umf_factor(umf_plan* p) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  Plan::Dirty D = p->P.all_dirty();
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  int rc = run_factor(p, D);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  prof_collect(p);
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  p->stats.t_factor_ms = ms;
+  p->last = D;
+  rc = check_err(p);
+  p->factored = rc == UMF_OK;
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+// Shared tail of the update entry points: d_val already holds (or is about to receive, in
+// stream order) the new values.
+int do_update(umf_plan* p, const double* src_val, bool src_device, int n_mod, const int* mod_dofs) {
+  const Plan& P = p->P;
+  Plan::Dirty D;
+  std::string e;
+  std::vector<int> mod(mod_dofs, mod_dofs + std::max(n_mod, 0));
+  if (!P.dirty_from_dofs(mod, D, e)) return fail(UMF_EUPDATEMAP, e);
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  HIPCHK(hipMemcpyAsync(p->d_val, src_val, sizeof(double) * P.ci.size(),
+                        src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+  double fl = 0;
+  int nf = 0;
+  if (D.any) {
+    int rc = run_factor(p, D);
+    if (rc) return rc;
+    for (int ell = 0; ell < P.g.L; ell++)
+      for (int b : D.boxes[ell]) {
+        fl += P.front_flops(P.box_gid[ell][b]);
+        nf++;
+      }
+  }
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  prof_collect(p);
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  p->stats.t_update_ms = ms;
+  p->stats.flops_update = fl;
+  p->stats.n_fronts_update = nf;
+  p->last = D;
+  int rc = check_err(p);
+  p->factored = rc == UMF_OK;
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int umf_update(umf_plan* p, const int* row_ptr, const int* col_ind, const double* val, int n_mod,
+               const int* mod_dofs) {
+  if (!p || !row_ptr || !col_ind || !val || (n_mod > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  if (p->host_only || !p->factored) return fail(UMF_ESTATE, "factor before update");
+  const Plan& P = p->P;
+  int n = P.n;
+  if (row_ptr[n] != (int)P.ci.size()) return fail(UMF_EPATTERN, "nnz differs from the planned operator");
+  for (int i = 0; i <= n; i++)
+    if (row_ptr[i] != P.rp[i]) return fail(UMF_EPATTERN, "row_ptr differs from the planned operator");
+  for (size_t k = 0; k < P.ci.size(); k++)
+    if (col_ind[k] != P.ci[k]) return fail(UMF_EPATTERN, "col_ind differs from the planned operator");
+  std::string e = check_operator(n, row_ptr, col_ind, val);
+  if (!e.empty()) return fail(UMF_EINVAL, e);
+  p->val_h.assign(val, val + row_ptr[n]);
+  return do_update(p, p->val_h.data(), false, n_mod, mod_dofs);
+}
+
+int umf_update_device(umf_plan* p, const double* d_val, int n_mod, const int* mod_dofs) {
+  if (!p || !d_val || (n_mod > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  if (p->host_only || !p->factored) return fail(UMF_ESTATE, "factor before update");
+  return do_update(p, d_val, true, n_mod, mod_dofs);
+}
+
+int umf_solve(umf_plan* p, const double* b, double* x, int nrhs, int ld) {
+  if (!b || !x) return fail(UMF_EINVAL, "null argument");
+  return solve_apply(p, 0, b, x, nrhs, ld, false);
+}
+int umf_apply(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
+  if (!x || !y) return fail(UMF_EINVAL, "null argument");
+  return solve_apply(p, 1, x, y, nrhs, ld, false);
+}
+int umf_solve_device(umf_plan* p, const double* b, double* x, int nrhs, int ld) {
+  if (!b || !x) return fail(UMF_EINVAL, "null argument");
+  return solve_apply(p, 0, b, x, nrhs, ld, true);
+}
+int umf_apply_device(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
+  if (!x || !y) return fail(UMF_EINVAL, "null argument");
+  return solve_apply(p, 1, x, y, nrhs, ld, true);
+}
+int umf_spmv_device(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
+  if (!p || !x || !y) return fail(UMF_EINVAL, "null argument");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  if (ld < p->P.n || nrhs < 0) return fail(UMF_EINVAL, "bad nrhs or ld");
+  if (nrhs == 0) return UMF_OK;
+  hipLaunchKernelGGL(k_spmv, dim3((p->P.n + 255) / 256), dim3(256), 0, p->stream, p->P.n, p->d_rp, p->d_ci,
+                     p->d_val, x, ld, y, ld, nrhs);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return UMF_OK;
+}
+
+int umf_levels(const umf_plan* p) { return p ? p->P.g.L : UMF_EINVAL; }
+int umf_leaf_width(const umf_plan* p, int a) {
+  if (!p || a < 0 || a >= p->P.g.dim) return fail(UMF_EINVAL, "bad axis");
+  return p->P.g.bw[a];
+}
+int umf_num_boxes(const umf_plan* p, int level) {
+  if (!p || level < 0 || level >= p->P.g.L) return fail(UMF_EINVAL, "bad level");
+  return p->P.g.nboxes(level);
+}
+int umf_num_seps(const umf_plan* p, int level) {
+  if (!p || level < 1 || level >= p->P.g.L) return fail(UMF_EINVAL, "bad level");
+  return p->P.g.nseps(level);
+}
+int umf_box_dofs(const umf_plan* p, int level, int box, int which, int* out) {
+  if (!p || level < 0 || level >= p->P.g.L || box < 0 || box >= p->P.g.nboxes(level))
+    return fail(UMF_EINVAL, "bad box");
+  const FrontInfo& F = p->P.fronts[p->P.box_gid[level][box]];
+  i64 off = which == 0 ? F.int_off : F.bd_off;
+  int cnt = which == 0 ? F.n : F.m;
+  if (out) std::copy(p->P.dofs.begin() + off, p->P.dofs.begin() + off + cnt, out);
+  return cnt;
+}
+int umf_sep_dofs(const umf_plan* p, int level, int sep, int* out) {
+  if (!p || level < 1 || level >= p->P.g.L || sep < 0 || sep >= p->P.g.nseps(level))
+    return fail(UMF_EINVAL, "bad separator");
+  const SepInfo& S = p->P.seps[p->P.sep_gid[level][sep]];
+  if (out) std::copy(p->P.dofs.begin() + S.dof_off, p->P.dofs.begin() + S.dof_off + S.k, out);
+  return S.k;
+}
+int umf_dirty(const umf_plan* p, int level, int which, int* out) {
+  if (!p || level < 0 || level >= p->P.g.L || (int)p->last.boxes.size() != p->P.g.L)
+    return fail(UMF_EINVAL, "bad level or nothing factored");
+  const Plan& P = p->P;
+  std::vector<int> v;
+  if (which == 0) v = p->last.boxes[level];
+  else if (which == 1) v = p->last.seps[level];
+  else if (level < P.g.L - 1) v = p->last.seps[level];  // rebuilt from children == receive adds
+  if (out) std::copy(v.begin(), v.end(), out);
+  return (int)v.size();
+}
+
+int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out, int* rows, int* cols) {
+  if (!p || !rows || !cols || level < 0 || level >= p->P.g.L || box < 0 || box >= p->P.g.nboxes(level))
+    return fail(UMF_EINVAL, "bad block query");
+  if (!p->factored) return fail(UMF_ESTATE, "factor before querying blocks");
+  int f = p->P.box_gid[level][box];
+  const FrontDev& F = p->fronts_h[f];
+  int n = F.n, m = F.m, N = F.N;
+  std::vector<double> h((size_t)N * N);
+  HIPCHK(hipMemcpy(h.data(), p->arena + F.off, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  auto at = [&](int i, int j) { return h[(size_t)j * N + i]; };
+  std::vector<double> r;
+  int R = 0, Cc = 0;
+  if (which == UMF_BLOCK_L || which == UMF_BLOCK_A22) {
+    R = Cc = n;
+    std::vector<double> Lm((size_t)n * n, 0.0);
+    for (int j = 0; j < n; j++)
+      for (int i = j; i < n; i++) Lm[(size_t)j * n + i] = at(i, j);
+    if (which == UMF_BLOCK_L) r = Lm;
+    else {
+      r.assign((size_t)n * n, 0.0);
+      for (int j = 0; j < n; j++)
+        for (int i = j; i < n; i++) {
+          double s = 0;
+          for (int k = 0; k <= j; k++) s += Lm[(size_t)k * n + i] * Lm[(size_t)k * n + j];
+          r[(size_t)j * n + i] = s;
+          r[(size_t)i * n + j] = s;
+        }
+    }
+  } else if (which == UMF_BLOCK_XR || which == UMF_BLOCK_XL) {
+    // XR = L11^{-T} W with W = L21^T (n x m)
+    std::vector<double> X((size_t)n * m);
+    for (int c = 0; c < m; c++) {
+      double* x = &X[(size_t)c * n];
+      for (int i = 0; i < n; i++) x[i] = at(n + c, i);
+      for (int j = n - 1; j >= 0; j--) {
+        double s = x[j];
+        for (int i = j + 1; i < n; i++) s -= at(i, j) * x[i];
+        x[j] = s / at(j, j);
+      }
+    }
+    if (which == UMF_BLOCK_XR) { R = n; Cc = m; r = X; }
+    else {
+      R = m; Cc = n;
+      r.assign((size_t)m * n, 0.0);
+      for (int c = 0; c < m; c++)
+        for (int i = 0; i < n; i++) r[(size_t)i * m + c] = X[(size_t)c * n + i];
+    }
+  } else if (which == UMF_BLOCK_S) {
+    R = Cc = m;
+    r.assign((size_t)m * m, 0.0);
+    for (int j = 0; j < m; j++)
+      for (int i = j; i < m; i++) {
+        double s = 0;
+        for (int k = 0; k < n; k++) s += at(n + i, k) * at(n + j, k);
+        r[(size_t)j * m + i] = -s;
+        r[(size_t)i * m + j] = -s;
+      }
+  } else if (which == UMF_BLOCK_A11POST) {
+    R = Cc = m;
+    r.assign((size_t)m * m, 0.0);
+    std::vector<int> run(m, -1);
+    for (int q = 0; q < F.nsep; q++)
+      for (int i = 0; i < F.sep_len[q]; i++) run[F.sep_pos[q] + i] = q;
+    for (int j = 0; j < m; j++)
+      for (int i = j; i < m; i++) {
+        double v = run[i] == run[j] ? NAN : at(n + i, n + j);
+        r[(size_t)j * m + i] = v;
+        r[(size_t)i * m + j] = v;
+      }
+  } else {
+    return fail(UMF_EINVAL, "unknown block selector");
+  }
+  *rows = R;
+  *cols = Cc;
+  if (out && !r.empty()) std::memcpy(out, r.data(), sizeof(double) * r.size());
+  return UMF_OK;
+}
+
+int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
+  if (!p || !k || level < 1 || level >= p->P.g.L || sep < 0 || sep >= p->P.g.nseps(level))
+    return fail(UMF_EINVAL, "bad separator");
+  const SepDev& S = p->seps_h[p->P.sep_gid[level][sep]];
+  *k = S.k;
+  if (!out) return UMF_OK;
+  std::vector<double> h((size_t)S.k * S.k);
+  if (!h.empty()) HIPCHK(hipMemcpy(h.data(), p->arena + S.off, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  for (int j = 0; j < S.k; j++)
+    for (int i = j; i < S.k; i++) {
+      out[(size_t)j * S.k + i] = h[(size_t)j * S.k + i];
+      out[(size_t)i * S.k + j] = h[(size_t)j * S.k + i];
+    }
+  return UMF_OK;
+}
+
+int umf_get_stats(const umf_plan* p, umf_stats* out) {
+  if (!p || !out) return fail(UMF_EINVAL, "null argument");
+  *out = p->stats;
+  return UMF_OK;
+}
+
+void* umf_stream(umf_plan* p) { return p ? (void*)p->stream : nullptr; }
+
+int umf_profile_enable(umf_plan* p, int on) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  p->prof.on = on != 0;
+  return UMF_OK;
+}
+int umf_profile_reset(umf_plan* p) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  p->prof.stats.clear();
+  return UMF_OK;
+}
+int umf_profile_read(const umf_plan* p, int i, char* name, int name_len, double* ms, int* launches, double* flops,
+                     double* bytes) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  const std::vector<KStat>& st = p->prof.stats;
+  if (i >= 0 && i < (int)st.size()) {
+    if (name && name_len > 0) {
+      std::strncpy(name, st[i].name.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (ms) *ms = st[i].ms;
+    if (launches) *launches = st[i].launches;
+    if (flops) *flops = st[i].flops;
+    if (bytes) *bytes = st[i].bytes;
+  }
+  return (int)st.size();
+}
+
+}  // extern "C"

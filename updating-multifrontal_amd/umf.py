@@ -1,0 +1,323 @@
+"""Python host mirror of the reference's multifrontal interface, over libumf_hip.so (ctypes).
+
+`MF2D` / `MF3D` keep the reference's constructor arguments and method names
+(include/DE/mf.hpp:67-80, :173-188): ``Factor()``, ``Update(A, modifiedDOFs)``, ``Solve(b)``,
+``Apply(x)``.  ``Solve``/``Apply`` also accept an n x nrhs array.  Errors follow the
+reference: geometry problems raise RuntimeError with the reference's message; a non-SPD pivot
+raises ArithmeticError (the reference asserts on LAPACK info != 0).
+
+The compute path is the HIP library only: if libumf_hip.so is missing this module raises at
+import time rather than falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libumf_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C csrc)")
+
+_lib = C.CDLL(LIB_PATH)
+
+UMF_OK = 0
+UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE = range(-1, -8, -1)
+BLOCKS = {"A22": 0, "XL": 1, "XR": 2, "S": 3, "A11post": 4, "L": 5}
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class Stats(C.Structure):
+    _fields_ = [("t_factor_ms", C.c_double), ("t_update_ms", C.c_double), ("t_solve_ms", C.c_double),
+                ("flops_factor", C.c_double), ("flops_update", C.c_double), ("flops_solve", C.c_double),
+                ("bytes_front_pool", C.c_double), ("n_fronts", C.c_int), ("n_fronts_update", C.c_int)]
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_create = _sig("umf_plan_create", C.c_int, [C.c_int] * 7 + [_ip, _ip, _dp, C.POINTER(_vp)])
+_analyze = _sig("umf_plan_analyze", C.c_int, [C.c_int] * 7 + [_ip, _ip, C.POINTER(_vp)])
+_analyze_update = _sig("umf_analyze_update", C.c_int, [_vp, C.c_int, _ip])
+_destroy = _sig("umf_plan_destroy", C.c_int, [_vp])
+_factor = _sig("umf_factor", C.c_int, [_vp])
+_update = _sig("umf_update", C.c_int, [_vp, _ip, _ip, _dp, C.c_int, _ip])
+_update_dev = _sig("umf_update_device", C.c_int, [_vp, _vp, C.c_int, _ip])
+_prof_enable = _sig("umf_profile_enable", C.c_int, [_vp, C.c_int])
+_prof_reset = _sig("umf_profile_reset", C.c_int, [_vp])
+_prof_read = _sig("umf_profile_read", C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp, _ip, _dp, _dp])
+_solve = _sig("umf_solve", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
+_apply = _sig("umf_apply", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
+_solve_dev = _sig("umf_solve_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_apply_dev = _sig("umf_apply_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_spmv_dev = _sig("umf_spmv_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_last_error = _sig("umf_last_error", C.c_char_p, [])
+_levels = _sig("umf_levels", C.c_int, [_vp])
+_leaf_width = _sig("umf_leaf_width", C.c_int, [_vp, C.c_int])
+_num_boxes = _sig("umf_num_boxes", C.c_int, [_vp, C.c_int])
+_num_seps = _sig("umf_num_seps", C.c_int, [_vp, C.c_int])
+_box_dofs = _sig("umf_box_dofs", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip])
+_sep_dofs = _sig("umf_sep_dofs", C.c_int, [_vp, C.c_int, C.c_int, _ip])
+_dirty = _sig("umf_dirty", C.c_int, [_vp, C.c_int, C.c_int, _ip])
+_front_block = _sig("umf_get_front_block", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip])
+_sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip])
+_stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
+
+EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
+            "umf_update_device", "umf_profile_enable", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
+            "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
+            "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
+            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream"]
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc == UMF_OK:
+        return
+    msg = _last_error().decode()
+    if rc > 0:
+        raise ArithmeticError(msg)
+    if rc == UMF_EGEOM:
+        raise RuntimeError(msg)
+    if rc in (UMF_EINVAL, UMF_EPATTERN, UMF_EUPDATEMAP):
+        raise ValueError(msg)
+    if rc == UMF_ENOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(f"umf error {rc}: {msg}")
+
+
+def _csr(A):
+    """Accepts an object with rp/ci/val (int32/int32/float64) or a scipy CSR matrix."""
+    if hasattr(A, "indptr"):
+        rp, ci, val = A.indptr, A.indices, A.data
+    else:
+        rp, ci, val = A.rp, A.ci, A.val
+    return (np.ascontiguousarray(rp, dtype=np.int32), np.ascontiguousarray(ci, dtype=np.int32),
+            np.ascontiguousarray(val, dtype=np.float64))
+
+
+class _MF:
+    def __init__(self, A, dims, width, maxSideLength, analyze_only=False):
+        rp, ci, val = _csr(A)
+        self.n = len(rp) - 1
+        self.dims = tuple(dims)
+        h = _vp()
+        nz = dims[2] if len(dims) == 3 else 1
+        if analyze_only:  # host-side symbolic analysis only, no device memory
+            _check(_analyze(len(dims), dims[0], dims[1], nz, width, maxSideLength, self.n, _i(rp), _i(ci),
+                            C.byref(h)))
+        else:
+            _check(_create(len(dims), dims[0], dims[1], nz, width, maxSideLength, self.n, _i(rp), _i(ci),
+                           _d(val), C.byref(h)))
+        self._h = h
+
+    def analyze_update(self, modifiedDOFs):
+        """Dirty sets an Update with these DOFs would factor (then read with dirty())."""
+        mod = np.ascontiguousarray(sorted(set(int(d) for d in modifiedDOFs)), dtype=np.int32)
+        _check(_analyze_update(self._h, len(mod), _i(mod) if len(mod) else None))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _destroy(h)
+            self._h = None
+
+    # ---- the reference's public API
+    def Factor(self):
+        _check(_factor(self._h))
+
+    def Update(self, A, modifiedDOFs):
+        rp, ci, val = _csr(A)
+        mod = np.ascontiguousarray(sorted(set(int(d) for d in modifiedDOFs)), dtype=np.int32)
+        _check(_update(self._h, _i(rp), _i(ci), _d(val), len(mod), _i(mod) if len(mod) else None))
+
+    def _sweep(self, fn, b):
+        b = np.asarray(b, dtype=np.float64)
+        vec = b.ndim == 1
+        B = np.asfortranarray(b.reshape(self.n, -1))
+        X = np.empty_like(B, order="F")
+        _check(fn(self._h, _d(B), _d(X), B.shape[1], self.n))
+        return X[:, 0].copy() if vec else X
+
+    def Solve(self, b):
+        return self._sweep(_solve, b)
+
+    def Apply(self, x):
+        return self._sweep(_apply, x)
+
+    # ---- device-resident variants (raw device pointers), column-major n x nrhs
+    def update_device(self, d_val, modifiedDOFs):
+        """Update with new CSR values already in device memory (same pattern as planned)."""
+        mod = np.ascontiguousarray(sorted(set(int(d) for d in modifiedDOFs)), dtype=np.int32)
+        _check(_update_dev(self._h, d_val, len(mod), _i(mod) if len(mod) else None))
+
+    def solve_device(self, d_b, d_x, nrhs, ld=None):
+        _check(_solve_dev(self._h, d_b, d_x, nrhs, ld or self.n))
+
+    def apply_device(self, d_x, d_y, nrhs, ld=None):
+        _check(_apply_dev(self._h, d_x, d_y, nrhs, ld or self.n))
+
+    def spmv_device(self, d_x, d_y, nrhs, ld=None):
+        _check(_spmv_dev(self._h, d_x, d_y, nrhs, ld or self.n))
+
+    # ---- structure / parity queries
+    @property
+    def levels(self):
+        return _levels(self._h)
+
+    def leaf_width(self, axis):
+        return _leaf_width(self._h, axis)
+
+    def nboxes(self, ell):
+        return _num_boxes(self._h, ell)
+
+    def nseps(self, ell):
+        return _num_seps(self._h, ell)
+
+    def _ints(self, fn, *args):
+        k = fn(self._h, *args, None)
+        if k < 0:
+            _check(k)
+        out = np.empty(max(k, 1), np.int32)
+        fn(self._h, *args, _i(out))
+        return out[:k].copy()
+
+    def box_dofs(self, ell, box, which):
+        return self._ints(_box_dofs, ell, box, which)
+
+    def sep_dofs(self, ell, sep):
+        return self._ints(_sep_dofs, ell, sep)
+
+    def dirty(self, ell, which):
+        return self._ints(_dirty, ell, which)
+
+    def front_block(self, ell, box, which):
+        r = C.c_int()
+        c = C.c_int()
+        _check(_front_block(self._h, ell, box, BLOCKS[which], None, C.byref(r), C.byref(c)))
+        out = np.empty(max(r.value * c.value, 1))
+        _check(_front_block(self._h, ell, box, BLOCKS[which], _d(out), C.byref(r), C.byref(c)))
+        return out[: r.value * c.value].reshape(c.value, r.value).T.copy()
+
+    def sep_block(self, ell, sep):
+        k = C.c_int()
+        _check(_sep_block(self._h, ell, sep, None, C.byref(k)))
+        out = np.empty(max(k.value * k.value, 1))
+        _check(_sep_block(self._h, ell, sep, _d(out), C.byref(k)))
+        return out[: k.value * k.value].reshape(k.value, k.value).T.copy()
+
+    def profile(self, on=True):
+        _check(_prof_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        _check(_prof_reset(self._h))
+
+    def profile_read(self):
+        """{kernel: {ms, launches, flops, bytes}} accumulated while profiling was on."""
+        k = _prof_read(self._h, -1, None, 0, None, None, None, None)
+        out = {}
+        for i in range(k):
+            name = C.create_string_buffer(64)
+            ms = C.c_double(); la = C.c_int(); fl = C.c_double(); by = C.c_double()
+            _prof_read(self._h, i, name, 64, C.byref(ms), C.byref(la), C.byref(fl), C.byref(by))
+            out[name.value.decode()] = {"ms": ms.value, "launches": la.value, "flops": fl.value, "bytes": by.value}
+        return out
+
+    def stats(self):
+        s = Stats()
+        _check(_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+
+class MF2D(_MF):
+    """MF2D(A, Nx, Ny, width=1, maxSideLength=4) -- include/DE/mf.hpp:67."""
+
+    def __init__(self, A, Nx, Ny, width=1, maxSideLength=4, analyze_only=False):
+        super().__init__(A, (Nx, Ny), width, maxSideLength, analyze_only)
+
+
+class MF3D(_MF):
+    """MF3D(A, Nx, Ny, Nz, width=1, maxSideLength=4) -- include/DE/mf.hpp:173."""
+
+    def __init__(self, A, Nx, Ny, Nz, width=1, maxSideLength=4, analyze_only=False):
+        super().__init__(A, (Nx, Ny, Nz), width, maxSideLength, analyze_only)
+
+
+def build_laplacian(dims, coeff, width=1, h=1.0):
+    """Variable-coefficient -div(a grad u) operator on the padded nodal field `coeff`
+    ((N+2w)^d, indexed [i][j][k]), face coefficient = mean of the two nodal values, Dirichlet
+    boundary through the padding -- the reference harness operator
+    (test_include/DE/buildLaplacian.hpp:10-132, :244-391).  Vectorized numpy; returns
+    (row_ptr, col_ind, val) with ascending columns per row."""
+    coeff = np.asarray(coeff, dtype=np.float64)
+    dim = len(dims)
+    cf = {1: [0.0, -1.0], 2: [0.0, -4.0 / 3.0, 1.0 / 12.0], 3: [0.0, -1.5, 3.0 / 20.0, -1.0 / 90.0]}[width]
+    w = width
+    N = tuple(dims)
+    n = int(np.prod(N))
+    hh = h * h
+    # DOF id = i + j*Nx (+ k*Nx*Ny); grids indexed [i, j(, k)]
+    idx = np.arange(n).reshape(N[::-1]).transpose()  # idx[i,j(,k)]
+    inner = tuple(slice(w, w + N[a]) for a in range(dim))
+    diag = np.zeros(N)
+
+    def shifted(a, s):
+        sl = list(inner)
+        sl[a] = slice(w + s, w + s + N[a])
+        return coeff[tuple(sl)]
+
+    # entries in ascending column order: for axes from slowest (last) to fastest, negative
+    # offsets far-to-near, then the diagonal, then positive offsets near-to-far
+    order_neg = [(a, -k) for a in reversed(range(dim)) for k in range(w, 0, -1)]
+    order_pos = [(a, k) for a in range(dim) for k in range(1, w + 1)]
+    # the diagonal sums per k (far to near) in the harness order x, y (, z) inside each k
+    for k in range(w, 0, -1):
+        for a in range(dim):
+            a1 = 0.5 * (shifted(a, -k) + shifted(a, -k + 1))
+            a2 = 0.5 * (shifted(a, k) + shifted(a, k - 1))
+            diag = diag - cf[k] * (a1 + a2) / hh
+    entries = []
+    for (a, s) in order_neg + [(None, 0)] + order_pos:
+        if a is None:
+            valid = np.ones(N, bool)
+            entries.append((idx, diag, valid))
+            continue
+        k = abs(s)
+        if s < 0:
+            ah = 0.5 * (shifted(a, -k) + shifted(a, -k + 1))
+        else:
+            ah = 0.5 * (shifted(a, k) + shifted(a, k - 1))
+        v = cf[k] * ah / hh
+        pos = np.indices(N)[a] + s
+        valid = (pos >= 0) & (pos < N[a])
+        nb = idx + s * (1 if a == 0 else int(np.prod(N[:a])))
+        entries.append((nb, v, valid))
+    # assemble row by row in the entry order above (already ascending in column)
+    flat = [(e[0].transpose().ravel(), e[1].transpose().ravel(), e[2].transpose().ravel()) for e in entries]
+    cnt = sum(f[2].astype(np.int32) for f in flat)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(cnt, out=rp[1:])
+    ci = np.empty(rp[-1], np.int32)
+    val = np.empty(rp[-1])
+    fill = rp[:-1].copy()
+    for c_, v_, m_ in flat:
+        rows = np.nonzero(m_)[0]
+        ci[fill[rows]] = c_[rows]
+        val[fill[rows]] = v_[rows]
+        fill[rows] += 1
+    return rp, ci, val
