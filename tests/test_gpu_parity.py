@@ -22,11 +22,11 @@ CASES = [
     ((23, 57), 3, "lognormal", 1),
     ((127, 127), 8, "uniform", 1),   # the 2D 127^2 configuration's tree (levels 4, 15^2 leaves)
     ((64, 47), 5, "lognormal", 1),
-    ((40, 33), 5, "rand", 2),
+    ((40, 33), 5, "uniform", 2),  # width > 1 with a variable field is unsymmetric (see below)
     ((9, 9, 9), 3, "rand", 1),
     ((15, 15, 15), 3, "lognormal", 1),
     ((11, 14, 9), 2, "lognormal", 1),
-    ((17, 13, 15), 2, "rand", 2),
+    ((17, 13, 15), 2, "uniform", 2),
 ]
 
 
@@ -171,3 +171,29 @@ def test_update_pattern_mismatch_rejected():
     B.ci[0], B.ci[1] = B.ci[1], B.ci[0]
     with pytest.raises(ValueError):
         G.Update(B, [0])
+
+
+@pytest.mark.parametrize("args", [["2", "127", "1", "8"], ["3", "15", "1", "4"], ["2", "63", "1", "4"]])
+def test_dropin_cpp_driver(args):
+    """The reference's test flow (testMF2D/testMF3D) through the drop-in C++ headers."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "updating-multifrontal_amd",
+                       "examples", "testMF")
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+def test_unsymmetric_operator_rejected():
+    """buildLaplacian with width 2 and a variable field averages different node pairs for
+    (p, p-2) and (p-2, p) (test_include/DE/buildLaplacian.hpp:64-74 vs :115-124), so the matrix
+    is pattern-symmetric but unsymmetric; the reference factors it with LU.  The Cholesky
+    fronts here need symmetry and refuse it up front."""
+    import umf
+    dims = (40, 33)
+    A = laplacian(dims, coeff_field(dims, "rand", seed=1, width=2), width=2)
+    D = A.to_scipy()
+    assert abs(D - D.T).max() > 0
+    with pytest.raises(ValueError, match="not symmetric"):
+        umf.MF2D(A, *dims, width=2, maxSideLength=5)

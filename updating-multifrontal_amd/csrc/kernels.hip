@@ -217,8 +217,9 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         Q = F[cols, k0:k0+kb], C = F[s:N, s:n] lower, s = k0+kb.
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-// The MFMA A operand is Q and B is P, so each lane's four accumulators run down one column of
-// C and the 16 lanes of a group cover 16 consecutive rows: stores are coalesced.
+// The MFMA A operand is Q and B is P.  v_mfma_f64_16x16x4 leaves D[(lane/16) + 4r][lane%16]
+// in accumulator r of a lane, so the 16 lanes of a group cover 16 consecutive rows of C
+// (coalesced stores) and accumulator r is column (lane/16) + 4r of the 16-wide block.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                               double* __restrict__ A, int k0) {
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = row0 + wr * 32 + bi * 16 + (lane & 15);
-        const int j = col0 + wc * 32 + bj * 16 + 4 * (lane >> 4) + r;
+        const int j = col0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * r;
         if (i >= rend || j >= cend || i < j) continue;
         double* c = C + (i64)j * F.N + i;
         if (MODE == 0) {

@@ -89,6 +89,7 @@ struct umf_plan {
   int* d_err = nullptr;
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
+  std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
   double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
   double* d_w = nullptr;   // solve work vector n x RB
   i64 xs_elems = 0;
@@ -165,6 +166,7 @@ Sched make_sched(const Plan& P, const Plan::Dirty& D) {
 
 int ensure_ids(umf_plan* p, size_t n) {
   if (n <= p->ids_cap) return UMF_OK;
+  HIPCHK(hipStreamSynchronize(p->stream));
   if (p->d_ids) (void)hipFree(p->d_ids);
   p->d_ids = nullptr;
   HIPCHK(hipMalloc((void**)&p->d_ids, sizeof(int) * std::max<size_t>(n, 1)));
@@ -226,8 +228,11 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
   Sched S = make_sched(P, D);
   int rc = ensure_ids(p, S.ids.size());
   if (rc) return rc;
-  if (!S.ids.empty())
-    HIPCHK(hipMemcpyAsync(p->d_ids, S.ids.data(), sizeof(int) * S.ids.size(), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
+  p->ids_h = S.ids;
+  if (!p->ids_h.empty())
+    HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                          p->stream));
   HIPCHK(hipMemsetAsync(p->d_err, 0, sizeof(int) * 4, p->stream));
   const int* ids = p->d_ids;
   const int NF = P.nfronts();
@@ -383,7 +388,10 @@ int sweep(umf_plan* p, int mode, int nr) {
   AllLists A = all_lists(P);
   int rc = ensure_ids(p, A.ids.size());
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(p->d_ids, A.ids.data(), sizeof(int) * A.ids.size(), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->ids_h = A.ids;
+  HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                        p->stream));
   const int* ids = p->d_ids;
   int n = P.n;
   const FrontDev* fr = p->d_fronts;
