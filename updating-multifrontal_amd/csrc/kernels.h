@@ -14,7 +14,8 @@ struct FrontDev {
   i64 off;      // arena offset of F
   i64 int_off;  // DOF list offsets
   i64 bd_off;
-  i64 x_off;    // solve scratch offset (N rows per right-hand-side column)
+  i64 x_off;    // solve scratch offset: N x 16 right-hand-side block, row-major
+  i64 dinv_off; // inverted 64x64 diagonal blocks of L11, column-major, zero above the diagonal
   int n, m, N, nsep;
   int sep_pos[6], sep_len[6];
 };

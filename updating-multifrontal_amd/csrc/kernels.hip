@@ -3,8 +3,8 @@
 // Every kernel works on a batch: a list of front (or separator) ids of one tree level, one
 // front per blockIdx.y (or blockIdx.x), so a whole level runs in one launch per stage.
 //   assembly   k_zero_lower, k_assemble, k_sep_add          (HBM-bound gathers / scatters)
-//   factor     k_potrf_diag, k_trsm_panel, k_gemm<0>, k_gemm<1>   (right-looking blocked
-//              Cholesky of the pivot columns, then the Schur update on FP64 MFMA)
+//   factor     k_chol_diag, k_chol_panel, k_gemm<0>, k_gemm<1>   (blocked Cholesky of the
+//              pivot columns, left- or right-looking, then the Schur update; FP64 MFMA)
 //   solve      k_solve_fwd, k_solve_bwd, k_sep_acc           (multi-RHS level sweeps)
 //   apply      k_apply_up, k_apply_down
 //   residual   k_spmv
@@ -80,115 +80,303 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
     }
 }
 
-// ----------------------------------------------------------------------------- factorization
-// Unblocked Cholesky of the kb x kb diagonal block at (k0,k0) of each front, in LDS.
-// A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
-__global__ __launch_bounds__(256) void k_potrf_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    double* __restrict__ A, int k0, int* __restrict__ err) {
-  const int fid = ids[blockIdx.x];
-  const FrontDev F = fr[fid];
-  if (k0 >= F.n) return;
-  const int kb = min(NB, F.n - k0);
-  __shared__ double T[NB * (NB + 1)];
-  double* a = A + F.off + (i64)k0 * F.N + k0;
-  const int t = threadIdx.x;
-  for (int e = t; e < NB * NB; e += 256) {
-    int r = e & 63, c = e >> 6;
-    T[c * (NB + 1) + r] = (r < kb && c < kb && r >= c) ? a[(i64)c * F.N + r] : 0.0;
-  }
-  for (int j = 0; j < kb; j++) {
-    __syncthreads();
-    const double d = T[j * (NB + 1) + j];
-    if (t == 0 && !(d > 0.0)) {
-      if (atomicCAS(err, 0, 1) == 0) { err[1] = fid; err[2] = k0 + j; }
+// ----------------------------------------------------------------------------- MFMA tile GEMM
+// acc += P Q^T for one 64 x 64 tile with v_mfma_f64_16x16x4 (4 waves as 2 x 2, each 32 x 32 =
+// 2 x 2 blocks of 16 x 16).  P(i,k) = P[k*ldp + i], Q(j,k) = Q[k*ldq + j]; rows i >= pn and
+// j >= qn read as zero; k in [0, K).  The MFMA A operand is Q and B is P, so accumulator r of a
+// lane holds C(i, j) with i = 16-block row + lane%16 and j = 16-block col + lane/16 + 4r
+// (v_mfma_f64_16x16x4 leaves D[(lane/16) + 4r][lane%16] in accumulator r), and the 16 lanes of a
+// group cover 16 consecutive rows of C.
+struct TileSmem {
+  double As[16][64];
+  double Bs[16][64];
+};
+
+__device__ __forceinline__ void tile_gemm_nt(d4 (&acc)[2][2], const double* __restrict__ P, int ldp, int pn,
+                                             const double* __restrict__ Q, int ldq, int qn, int K, TileSmem& sm) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int li = t & 63, lk = t >> 6;
+  const bool prow = li < pn, qrow = li < qn;
+  double pa[4], qb[4];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = kc + lk + 4 * q;
+      const bool kin = k < K;
+      pa[q] = (kin && prow) ? P[(i64)k * ldp + li] : 0.0;
+      qb[q] = (kin && qrow) ? Q[(i64)k * ldq + li] : 0.0;
     }
-    const double sd = sqrt(d);
+  };
+  if (K <= 0) return;
+  fetch(0);
+  for (int kc = 0; kc < K; kc += 16) {
     __syncthreads();
-    for (int i = j + 1 + t; i < kb; i += 256) T[j * (NB + 1) + i] /= sd;
-    if (t == 0) T[j * (NB + 1) + j] = sd;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sm.As[lk + 4 * q][li] = pa[q];
+      sm.Bs[lk + 4 * q][li] = qb[q];
+    }
     __syncthreads();
-    const int cnt = kb - j - 1;
-    for (int e = t; e < cnt * cnt; e += 256) {
-      int rr = e % cnt, cc = e / cnt;
-      if (rr >= cc) T[(j + 1 + cc) * (NB + 1) + j + 1 + rr] -= T[j * (NB + 1) + j + 1 + rr] * T[j * (NB + 1) + j + 1 + cc];
+    if (kc + 16 < K) fetch(kc + 16);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+      const int kr = k4 * 4 + (lane >> 4);
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        const double bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) {
+          const double av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+        }
+      }
     }
   }
   __syncthreads();
-  for (int e = t; e < NB * NB; e += 256) {
-    int r = e & 63, c = e >> 6;
-    if (r < kb && c < kb && r >= c) a[(i64)c * F.N + r] = T[c * (NB + 1) + r];
+}
+
+// tile-local coordinates of accumulator (bi, bj, r) of this thread
+__device__ __forceinline__ int acc_row(int bi) { return ((threadIdx.x >> 6) >> 1) * 32 + bi * 16 + (threadIdx.x & 15); }
+__device__ __forceinline__ int acc_col(int bj, int r) {
+  return ((threadIdx.x >> 6) & 1) * 32 + bj * 16 + ((threadIdx.x & 63) >> 4) + 4 * r;
+}
+
+// ----------------------------------------------------------------------------- factorization
+// Blocked partial Cholesky of the pivot columns [0, n) of each front, 64 columns per block.
+//   k_chol_diag:  the diagonal block (already carrying every earlier block's update) is
+//                 factored in LDS, T = L_jj L_jj^T, together with Linv_jj = L_jj^{-1}.
+//   k_chol_panel: rows below the block: X = (A(rows, jb block) - L(rows, 0:K) L(jb.., 0:K)^T)
+//                 Linv_jj^T, i.e. the panel TRSM as a product with the inverted diagonal block.
+//                 Left-looking (K = jb): the off-diagonal tiles are formed once, from the
+//                 untouched input; the workgroup then applies its rank-64 update to the one
+//                 future diagonal tile it owns (rows = cols = its row range).  Right-looking
+//                 (K = 0): the trailing region was updated by k_gemm<0>.
+// A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
+// acc += P Q^T for 16 x 16 blocks held in LDS, one wave, v_mfma_f64_16x16x4 (K = 16):
+// P(i,k) = P[i*psi + k*psk], Q(j,k) = Q[j*qsj + k*qsk]; accumulator r of a lane is
+// C(lane%16, lane/16 + 4r).
+__device__ __forceinline__ void mma16(d4& acc, const double* P, int psi, int psk, const double* Q, int qsj, int qsk) {
+  const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int k4 = 0; k4 < 4; k4++) {
+    const int k = 4 * k4 + kq;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Q[i * qsj + k * qsk], P[i * psi + k * psk], acc, 0, 0, 0);
   }
 }
 
-// Panel TRSM: rows [k0+kb, N) of columns [k0, k0+kb) become X = A L_kk^{-T} (forward
-// substitution per row, one row per thread, the factored diagonal block in LDS).  Covers the
-// rows below the block inside A22 and all rows of A12, so L21 is produced along the way.
-// FULL = every listed front has a full 64-wide panel at k0 (unconditional, unrolled stores);
-// otherwise only fronts whose last panel is narrower are handled (stores by a short loop:
-// unrolled conditional stores make the compiler spill the whole row).
-template <bool FULL>
-__global__ __launch_bounds__(256) void k_trsm_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    double* __restrict__ A, int k0) {
-  const FrontDev F = fr[ids[blockIdx.y]];
-  if (k0 >= F.n) return;
-  const int kb = min(NB, F.n - k0);
-  if (FULL != (kb == NB)) return;
-  __shared__ double Ld[NB * (NB + 1)];
-  __shared__ double Dinv[NB];
-  const double* dblk = A + F.off + (i64)k0 * F.N + k0;
-  const int t = threadIdx.x;
+// Cholesky of the 64 x 64 diagonal block at (jb, jb) of each front with its inverse, one
+// workgroup per front, as 4 x 4 blocks of 16 x 16 in LDS:
+//   for k: factor + invert D_kk (one thread per element, 16 barrier steps); panel
+//   L_ik = A_ik D_kk^{-T} and trailing A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave;
+//   then the off-diagonal blocks of the inverse, Linv_ij = -D_ii^{-1} sum_k L_ik Linv_kj,
+//   by distance i - j.
+// T[c*LD + r] holds L (lower), U[c*LD + r] holds Linv (lower).
+#define LD (NB + 1)
+__global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                   double* __restrict__ A, int jb, int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  if (jb >= F.n) return;
+  const int kb = min(NB, F.n - jb);
+  __shared__ double T[NB * LD];
+  __shared__ double U[NB * LD];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  double* blk = A + F.off + (i64)jb * F.N + jb;
   for (int e = t; e < NB * NB; e += 256) {
-    int r = e & 63, c = e >> 6;
-    double v = (r < kb && c < kb && r >= c) ? dblk[(i64)c * F.N + r] : 0.0;
-    if (r == c && r >= kb) v = 1.0;  // identity padding keeps the unrolled solve branch-free
-    Ld[c * (NB + 1) + r] = v;
+    const int i = e & 63, j = e >> 6;
+    double v;
+    if (i < kb && j < kb) v = i >= j ? blk[(i64)j * F.N + i] : 0.0;
+    else v = i == j ? 1.0 : 0.0;  // identity padding
+    T[j * LD + i] = v;
+    U[j * LD + i] = 0.0;
   }
   __syncthreads();
-  if (t < NB) Dinv[t] = 1.0 / Ld[t * (NB + 1) + t];
-  __syncthreads();
-  const int row = k0 + kb + blockIdx.x * 256 + t;
-  if (row >= F.N) return;
-  // uniform column base + 32-bit per-thread offset
-  double* p = A + F.off + (i64)k0 * F.N;
-  const unsigned ld = F.N;
-  // two 32-column halves: solve the first, eliminate it from the second, solve the second
-  double x[32], y[32];
+  const int er = t & 15, ec = t >> 4;  // element of a 16 x 16 block
+  for (int k = 0; k < 4; k++) {
+    const int K0 = 16 * k;
+    double* Dk = T + K0 * LD + K0;
+    double* Vk = U + K0 * LD + K0;
+    if (ec == er) Vk[ec * LD + er] = 1.0;
+    // (a) unblocked Cholesky of D_kk on [D | I]
+    for (int j = 0; j < 16; j++) {
+      __syncthreads();
+      const double d = Dk[j * LD + j];
+      if (t == 0 && K0 + j < kb && !(d > 0.0)) {
+        if (atomicCAS(err, 0, 1) == 0) { err[1] = fid; err[2] = jb + K0 + j; }
+      }
+      const double sd = sqrt(d), inv = 1.0 / sd;
+      const double lr = Dk[j * LD + er] * inv, lc = Dk[j * LD + ec] * inv;
+      const double ajc = Vk[ec * LD + j] * inv;  // row j of the inverse, scaled
+      __syncthreads();
+      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= lr * lc;
+      if (er > j && ec <= j) Vk[ec * LD + er] -= lr * ajc;
+      if (er == j && ec <= j) Vk[ec * LD + j] = ajc;
+      if (ec == j) {
+        if (er > j) Dk[j * LD + er] = lr;
+        else if (er == j) Dk[j * LD + j] = sd;
+      }
+    }
+    __syncthreads();
+    // (b) panel blocks below: L_ik = A_ik D_kk^{-T}, wave w -> block i = k+1+w
+    {
+      const int i = k + 1 + w;
+      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+      if (i < 4) mma16(acc, T + K0 * LD + 16 * i, 1, LD, Vk, 1, LD);
+      __syncthreads();
+      if (i < 4)
 #pragma unroll
-  for (int c = 0; c < 32; c++) x[c] = (FULL || c < kb) ? p[c * ld + row] : 0.0;
+        for (int r = 0; r < 4; r++) T[(K0 + (lane >> 4) + 4 * r) * LD + 16 * i + (lane & 15)] = acc[r];
+    }
+    __syncthreads();
+    // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
+    for (int p = w; p < 6; p += 4) {
+      int cnt = 0, bi = -1, bj = -1;
+      for (int i2 = k + 1; i2 < 4; i2++)
+        for (int j2 = k + 1; j2 <= i2; j2++) {
+          if (cnt == p) { bi = i2; bj = j2; }
+          cnt++;
+        }
+      if (bi < 0) continue;
+      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+      mma16(acc, T + K0 * LD + 16 * bi, 1, LD, T + K0 * LD + 16 * bj, 1, LD);
 #pragma unroll
-  for (int j = 0; j < 32; j++) {
-    double v = x[j];
-#pragma unroll
-    for (int i = 0; i < j; i++) v -= x[i] * Ld[i * (NB + 1) + j];
-    x[j] = v * Dinv[j];
+      for (int r = 0; r < 4; r++) T[(16 * bj + (lane >> 4) + 4 * r) * LD + 16 * bi + (lane & 15)] -= acc[r];
+    }
+    __syncthreads();
   }
+  // inverse off-diagonal blocks, by distance: Linv_ij = -D_ii^{-1} S, S = sum_{k=j}^{i-1} L_ik Linv_kj
+  __shared__ double Sbuf[4][16 * 17];
+  for (int dist = 1; dist < 4; dist++) {
+    const int i = w + dist, j = w;
+    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+    if (i < 4)
+      for (int kk = j; kk < i; kk++)  // S += L_i,kk Linv_kk,j: P = L_i,kk (i,k), Q(j',k) = Linv_kk,j(k, j')
+        mma16(acc, T + 16 * kk * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * kk, LD, 1);
+    if (i < 4)
 #pragma unroll
-  for (int c = 0; c < 32; c++) y[c] = (FULL || c + 32 < kb) ? p[(c + 32) * ld + row] : 0.0;
+      for (int r = 0; r < 4; r++) Sbuf[w][((lane >> 4) + 4 * r) * 17 + (lane & 15)] = acc[r];  // S(i', j') at [j'*17+i']
+    __syncthreads();
+    d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+    if (i < 4)  // Linv_ij = -(D_ii^{-1} S): P = D_ii^{-1} (i', k), Q(j', k) = S(k, j')
+      mma16(a2, U + 16 * i * LD + 16 * i, 1, LD, Sbuf[w], 17, 1);
+    if (i < 4)
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
-    const double xi = x[i];
-#pragma unroll
-    for (int j = 0; j < 32; j++) y[j] -= xi * Ld[i * (NB + 1) + 32 + j];
+      for (int r = 0; r < 4; r++) U[(16 * j + (lane >> 4) + 4 * r) * LD + 16 * i + (lane & 15)] = -a2[r];
+    __syncthreads();
   }
-#pragma unroll
-  for (int j = 0; j < 32; j++) {
-    double v = y[j];
-#pragma unroll
-    for (int i = 0; i < j; i++) v -= y[i] * Ld[(32 + i) * (NB + 1) + 32 + j];
-    y[j] = v * Dinv[32 + j];
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = T[j * LD + i];
   }
-  if (FULL) {
-#pragma unroll
-    for (int c = 0; c < 32; c++) p[c * ld + row] = x[c];
-#pragma unroll
-    for (int c = 0; c < 32; c++) p[(c + 32) * ld + row] = y[c];
-  } else {
-    for (int c = 0; c < kb && c < 32; c++) p[c * ld + row] = x[c];
-    for (int c = 32; c < kb; c++) p[c * ld + row] = y[c - 32];
-  }
+  double* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+  for (int e = t; e < NB * NB; e += 256) dinv[e] = U[(e >> 6) * LD + (e & 63)];  // column-major, zero above
 }
-template __global__ void k_trsm_panel<true>(const int*, const FrontDev*, double*, int);
-template __global__ void k_trsm_panel<false>(const int*, const FrontDev*, double*, int);
+
+__global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                    double* __restrict__ A, int jb, int left) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  if (jb >= F.n) return;
+  const int kb = min(NB, F.n - jb);
+  const int row0 = jb + kb + NB * blockIdx.x;
+  if (row0 >= F.N) return;
+  const int nrow = min(NB, F.N - row0);
+  __shared__ union {
+    TileSmem g;
+    struct {
+      double X[NB * (NB + 1)];
+    } c;
+  } sm;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wr = (t >> 6) >> 1, wc = (t >> 6) & 1;
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+  const int K = left ? jb : 0;
+  tile_gemm_nt(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
+  double* pan = A + F.off + (i64)jb * F.N + row0;
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col(bj, r);
+        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : 0.0;
+      }
+  const double* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
+  __syncthreads();
+  // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k)
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < NB / 4; k4++) {
+    const int k = k4 * 4 + (lane >> 4);
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+      const double bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const double av = dinv[k * NB + wc * 32 + bj * 16 + (lane & 15)];
+        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col(bj, r);
+        if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
+      }
+  if (!left || row0 >= F.n) return;
+  // rank-64 update of the future diagonal tile D = A(row0.., row0..): D -= Y Y^T (lower)
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col(bj, r);
+        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : 0.0;
+      }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < NB / 4; k4++) {
+    const int k = k4 * 4 + (lane >> 4);
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+      const double bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const double av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + (lane & 15)];
+        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+      }
+    }
+  }
+  const int dn = min(NB, F.n - row0);
+  double* D = A + F.off + (i64)row0 * F.N + row0;
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col(bj, r);
+        if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] -= acc[bi][bj][r];
+      }
+}
 
 // Lower-triangular tile enumeration of the region rows [s, rend) x cols [s, cend), 64x64
 // tiles: first the T(T+1)/2 tiles of the square part, then the full rows below it.
@@ -212,14 +400,11 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
   return true;
 }
 
-// C -= P Q^T on 64x64 tiles with v_mfma_f64_16x16x4 (4 waves, 2x2 of 16x16 blocks each).
-// MODE 0: trailing update of the pivot columns after panel k0: P = F[rows, k0:k0+kb],
+// C -= P Q^T on the lower part of a region, one 64x64 tile per workgroup.
+// MODE 0: right-looking trailing update after panel k0: P = F[rows, k0:k0+kb],
 //         Q = F[cols, k0:k0+kb], C = F[s:N, s:n] lower, s = k0+kb.
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-// The MFMA A operand is Q and B is P.  v_mfma_f64_16x16x4 leaves D[(lane/16) + 4r][lane%16]
-// in accumulator r of a lane, so the 16 lanes of a group cover 16 consecutive rows of C
-// (coalesced stores) and accumulator r is column (lane/16) + 4r of the 16-wide block.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                               double* __restrict__ A, int k0) {
@@ -237,56 +422,14 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
   }
   int row0, col0;
   if (!tile_of(blockIdx.x, s, rend, cend, row0, col0)) return;
-
-  __shared__ double As[16][64];
-  __shared__ double Bs[16][64];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = w >> 1, wc = w & 1;
-  const double* base = A + F.off;
+  __shared__ TileSmem sm;
   d4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
-
-  // each thread stages 4 elements of each operand per 16-wide k chunk
-  const int li = t & 63, lk = t >> 6;  // row within tile, k offset (0..3), +4q
-  const bool prow = row0 + li < rend, qrow = col0 + li < cend;
-  double pa[4], qb[4];
-  auto fetch = [&](int kc) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      int k = kc + lk + 4 * q;
-      bool kin = k < K;
-      i64 colk = (i64)(kbase + k) * F.N;
-      pa[q] = (kin && prow) ? base[colk + row0 + li] : 0.0;
-      qb[q] = (kin && qrow) ? base[colk + col0 + li] : 0.0;
-    }
-  };
-  fetch(0);
-  for (int kc = 0; kc < K; kc += 16) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      As[lk + 4 * q][li] = pa[q];
-      Bs[lk + 4 * q][li] = qb[q];
-    }
-    __syncthreads();
-    if (kc + 16 < K) fetch(kc + 16);
-#pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) {
-      const int kr = k4 * 4 + (lane >> 4);
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) {
-        const double b = As[kr][wr * 32 + bi * 16 + (lane & 15)];
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
-          const double a = Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
-          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[bi][bj], 0, 0, 0);
-        }
-      }
-    }
-  }
+  const double* base = A + F.off + (i64)kbase * F.N;
+  tile_gemm_nt(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   double* C = A + F.off;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -294,8 +437,7 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = row0 + wr * 32 + bi * 16 + (lane & 15);
-        const int j = col0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * r;
+        const int i = row0 + acc_row(bi), j = col0 + acc_col(bj, r);
         if (i >= rend || j >= cend || i < j) continue;
         double* c = C + (i64)j * F.N + i;
         if (MODE == 0) {
@@ -320,182 +462,184 @@ template __global__ void k_gemm<0>(const int*, const FrontDev*, double*, int);
 template __global__ void k_gemm<1>(const int*, const FrontDev*, double*, int);
 
 // ----------------------------------------------------------------------------- solve
-// Work vector W is n_dof x R column-major (ld = ldw); per-front scratch X is N x R (ld = N).
-#define RB 16  // right-hand sides per sweep pass
+// The sweeps work on W, an n_dof x 16 row-major right-hand-side block (W[dof*16 + r]), and on a
+// per-front scratch X (N x 16, row-major).  All block products run on v_mfma_f64_16x16x4 with a
+// 16-row block of L (or Linv) as the A operand and a 4 x 16 slice of X as B, so accumulator r
+// of a lane is row (lane/16) + 4r, right-hand side lane%16.
+#define RB 16
+#define GU 4  // 16-row groups in flight per wave
 
-// Forward sweep of one front: X[0:n] <- L11^{-1} W[int], X[n:N] <- -L21 X[0:n];
-// writes X[0:n] back to W[int] (the reference's A22^{-1} b_rd is finished by the backward sweep).
+__device__ __forceinline__ void gather_rhs(const FrontDev& F, double* __restrict__ X, const double* __restrict__ W,
+                                           const int* __restrict__ di, const int* __restrict__ db) {
+  for (int e = threadIdx.x; e < F.N * RB; e += 256) {
+    const int i = e >> 4, r = e & 15;
+    const int d = i < F.n ? di[i] : (db ? db[i - F.n] : -1);
+    X[e] = d >= 0 ? W[(i64)d * RB + r] : 0.0;
+  }
+}
+
+// Forward sweep of one front: X[0:n] <- L11^{-1} W[int] block by block (x_j = Linv_j b_j, then
+// b_below -= L(below, j) x_j), X[n:N] <- -L21 X[0:n]; X[0:n] goes back to W[int].
 __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    const double* __restrict__ A, const int* __restrict__ dofs,
-                                                   double* __restrict__ W, int ldw, double* __restrict__ xs, int nr) {
+                                                   double* __restrict__ W, double* __restrict__ xs) {
   const FrontDev F = fr[ids[blockIdx.x]];
   const double* L = A + F.off;
   double* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int e = t; e < F.N * nr; e += 256) {
-    int i = e % F.N, r = e / F.N;
-    X[(i64)r * F.N + i] = i < F.n ? W[(i64)r * ldw + di[i]] : 0.0;
-  }
-  __shared__ double Ld[NB * (NB + 1)];
-  __shared__ double Xd[NB][RB];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  gather_rhs(F, X, W, di, nullptr);
+  __shared__ double xb[NB][RB];
   for (int jb = 0; jb < F.n; jb += NB) {
     const int kb = min(NB, F.n - jb);
     __syncthreads();
-    for (int e = t; e < NB * NB; e += 256) {
-      int r = e & 63, c = e >> 6;
-      double v = (r < kb && c < kb && r >= c) ? L[(i64)(jb + c) * F.N + jb + r] : 0.0;
-      if (r == c && r >= kb) v = 1.0;
-      Ld[c * (NB + 1) + r] = v;
-    }
-    for (int e = t; e < NB * RB; e += 256) {
-      int i = e & 63, r = e >> 6;
-      Xd[i][r] = (i < kb && r < nr) ? X[(i64)r * F.N + jb + i] : 0.0;
+    {  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
+      const double* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        const int k = 4 * k4 + q;
+        const double av = Li[k * NB + 16 * w + c16];
+        const double bv = k < kb ? X[(i64)(jb + k) * RB + c16] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + q + 4 * r;
+        xb[i][c16] = acc[r];
+        if (i < kb) X[(i64)(jb + i) * RB + c16] = acc[r];
+      }
     }
     __syncthreads();
-    if (wv == 0) {
-      // lane = (rhs r, quarter q): rows q, q+4, ..., q+60
-      const int r = lane & 15, q = lane >> 4;
-      double x[16];
+    // rows below, GU 16-row groups per wave in flight; out-of-range rows read row N-1 and are
+    // never written
+    for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * GU) {
+      double av[GU][NB / 4];
 #pragma unroll
-      for (int s = 0; s < 16; s++) x[s] = Xd[4 * s + q][r];
+      for (int u = 0; u < GU; u++) {
+        const int row = min(g0 + 64 * u + c16, F.N - 1);
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        const int sj = j >> 2, qj = j & 3;
-        if (q == qj) x[sj] /= Ld[j * (NB + 1) + j];
-        const double xj = __shfl(x[sj], qj * 16 + r, 64);
-#pragma unroll
-        for (int s = 0; s < 16; s++)
-          if (4 * s + q > j) x[s] -= Ld[j * (NB + 1) + 4 * s + q] * xj;
+        for (int k4 = 0; k4 < NB / 4; k4++) {
+          const int k = 4 * k4 + q;
+          av[u][k4] = k < kb ? L[(i64)(jb + k) * F.N + row] : 0.0;
+        }
       }
 #pragma unroll
-      for (int s = 0; s < 16; s++) Xd[4 * s + q][r] = x[s];
-    }
-    __syncthreads();
-    for (int e = t; e < kb * nr; e += 256) {
-      int i = e % kb, r = e / kb;
-      X[(i64)r * F.N + jb + i] = Xd[i][r];
-    }
-    // rows below: X[i] -= L[i, jb:jb+kb] Xd
-    for (int i = jb + kb + t; i < F.N; i += 256) {
-      double acc[RB];
+      for (int u = 0; u < GU; u++) {
+        d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int r = 0; r < RB; r++) acc[r] = 0.0;
-      for (int c = 0; c < kb; c++) {
-        const double l = L[(i64)(jb + c) * F.N + i];
+        for (int k4 = 0; k4 < NB / 4; k4++)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], xb[4 * k4 + q][c16], acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < RB; r++) acc[r] += l * Xd[c][r];
+        for (int r = 0; r < 4; r++) {
+          const int i = g0 + 64 * u + q + 4 * r;
+          if (i < F.N) X[(i64)i * RB + c16] -= acc[r];
+        }
       }
-      for (int r = 0; r < nr; r++) X[(i64)r * F.N + i] -= acc[r];
     }
   }
   __syncthreads();
-  for (int e = t; e < F.n * nr; e += 256) {
-    int i = e % F.n, r = e / F.n;
-    W[(i64)r * ldw + di[i]] = X[(i64)r * F.N + i];
-  }
+  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
 
 // Separator accumulation of per-box contributions X[n+pos : n+pos+k] of both neighbour boxes
-// into the work vector, box 0 first: W = (W + X_b0) + X_b1.
+// into the work block, box 0 first: W = (W + X_b0) + X_b1.
 __global__ __launch_bounds__(256) void k_sep_acc(const int* __restrict__ ids, const SepDev* __restrict__ seps,
                                                  const FrontDev* __restrict__ fr, const int* __restrict__ dofs,
-                                                 double* __restrict__ W, int ldw, const double* __restrict__ xs,
-                                                 int nr) {
+                                                 double* __restrict__ W, const double* __restrict__ xs) {
   const SepDev S = seps[ids[blockIdx.x]];
   const FrontDev B0 = fr[S.box[0]], B1 = fr[S.box[1]];
   const int* d = dofs + S.dof_off;
-  for (int e = threadIdx.x; e < S.k * nr; e += 256) {
-    int i = e % S.k, r = e / S.k;
-    double* w = W + (i64)r * ldw + d[i];
-    double v = *w + xs[B0.x_off + (i64)r * B0.N + B0.n + S.pos[0] + i];
-    *w = v + xs[B1.x_off + (i64)r * B1.N + B1.n + S.pos[1] + i];
+  const double* x0 = xs + B0.x_off + (i64)(B0.n + S.pos[0]) * RB;
+  const double* x1 = xs + B1.x_off + (i64)(B1.n + S.pos[1]) * RB;
+  for (int e = threadIdx.x; e < S.k * RB; e += 256) {
+    double* wp = W + (i64)d[e >> 4] * RB + (e & 15);
+    const double v = *wp + x0[e];
+    *wp = v + x1[e];
   }
 }
 
 // Backward sweep of one front: X[0:n] = W[int] (forward result y), X[n:N] = W[bd];
-// X[0:n] <- L11^{-T} (y - L21^T X[n:N]) = A22^{-1} b_rd - XR b_sk; writes X[0:n] to W[int].
+// block by block from the last: z_j = y_j - L(below, j)^T X(below), x_j = Linv_j^T z_j
+// (= A22^{-1} b_rd - XR b_sk overall); X[0:n] goes back to W[int].
 __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    const double* __restrict__ A, const int* __restrict__ dofs,
-                                                   double* __restrict__ W, int ldw, double* __restrict__ xs, int nr) {
+                                                   double* __restrict__ W, double* __restrict__ xs) {
   const FrontDev F = fr[ids[blockIdx.x]];
   const double* L = A + F.off;
   double* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int e = t; e < F.N * nr; e += 256) {
-    int i = e % F.N, r = e / F.N;
-    X[(i64)r * F.N + i] = W[(i64)r * ldw + (i < F.n ? di[i] : db[i - F.n])];
-  }
-  __shared__ double Lt[NB * (NB + 1)];
-  __shared__ double Xt[NB][RB];
-  __shared__ double Xd[NB][RB];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  gather_rhs(F, X, W, di, db);
+  __shared__ double zb[NB][RB];
   const int nblk = (F.n + NB - 1) / NB;
   for (int bk = nblk - 1; bk >= 0; bk--) {
     const int jb = bk * NB, kb = min(NB, F.n - jb);
-    // T = sum over rows below the block of L[i, jb+c] X[i, r]; thread -> (c, 4 rhs)
-    const int c = t & 63, rg = (t >> 6) * 4;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i0 = jb + kb; i0 < F.N; i0 += NB) {
-      const int h = min(NB, F.N - i0);
-      __syncthreads();
-      for (int e = t; e < NB * NB; e += 256) {
-        int ii = e & 63, cc = e >> 6;
-        Lt[cc * (NB + 1) + ii] = (ii < h && cc < kb) ? L[(i64)(jb + cc) * F.N + i0 + ii] : 0.0;
-      }
-      for (int e = t; e < NB * RB; e += 256) {
-        int ii = e & 63, r = e >> 6;
-        Xt[ii][r] = (ii < h && r < nr) ? X[(i64)r * F.N + i0 + ii] : 0.0;
-      }
-      __syncthreads();
-      for (int ii = 0; ii < NB; ii++) {
-        const double l = Lt[c * (NB + 1) + ii];
+    __syncthreads();
+    // T(c, r) = sum_{i >= jb+kb} L(i, jb+c) X(i, r); wave w owns columns c in [16w, 16w+16),
+    // streaming 4-row slices straight from the column-major L (lanes: column c16, row q)
+    const int col = 16 * w + c16;
+    const double* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
+    const bool cin = col < kb;
+    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int i0 = jb + kb; i0 < F.N; i0 += 16 * GU) {
+      double av[GU][4], bv[GU][4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[q] += l * Xt[ii][rg + q];
-      }
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) {
+          const int i = i0 + 16 * u + 4 * k4 + q;
+          const bool ok = cin && i < F.N;
+          const int ic = min(i, F.N - 1);
+          av[u][k4] = ok ? Lc[ic] : 0.0;
+          bv[u][k4] = X[(i64)ic * RB + c16];
+        }
+#pragma unroll
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], bv[u][k4], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int c = 16 * w + q + 4 * r;
+      zb[c][c16] = c < kb ? X[(i64)(jb + c) * RB + c16] - acc[r] : 0.0;
     }
     __syncthreads();
-    for (int e = t; e < NB * NB; e += 256) {
-      int r = e & 63, cc = e >> 6;
-      double v = (r < kb && cc < kb && r >= cc) ? L[(i64)(jb + cc) * F.N + jb + r] : 0.0;
-      if (r == cc && r >= kb) v = 1.0;
-      Lt[cc * (NB + 1) + r] = v;
-    }
+    {  // x_j = Linv_j^T z: x(i, r) = sum_c Linv(c, i) z(c, r); wave w -> rows 16w..16w+15
+      const double* Li = A + F.dinv_off + (i64)bk * NB * NB;
+      d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int r = rg + q;
-      Xd[c][r] = (c < kb && r < nr) ? X[(i64)r * F.N + jb + c] - acc[q] : 0.0;
-    }
-    __syncthreads();
-    if (wv == 0) {
-      const int r = lane & 15, q = lane >> 4;
-      double x[16];
-#pragma unroll
-      for (int s = 0; s < 16; s++) x[s] = Xd[4 * s + q][r];
-#pragma unroll
-      for (int j = NB - 1; j >= 0; j--) {
-        const int sj = j >> 2, qj = j & 3;
-        if (q == qj) x[sj] /= Lt[j * (NB + 1) + j];
-        const double xj = __shfl(x[sj], qj * 16 + r, 64);
-#pragma unroll
-        for (int s = 0; s < 16; s++)
-          if (4 * s + q < j) x[s] -= Lt[(4 * s + q) * (NB + 1) + j] * xj;
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        const int c = 4 * k4 + q;
+        a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[(16 * w + c16) * NB + c], zb[c][c16], a2, 0, 0, 0);
       }
 #pragma unroll
-      for (int s = 0; s < 16; s++) Xd[4 * s + q][r] = x[s];
-    }
-    __syncthreads();
-    for (int e = t; e < kb * nr; e += 256) {
-      int i = e % kb, r = e / kb;
-      X[(i64)r * F.N + jb + i] = Xd[i][r];
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + q + 4 * r;
+        if (i < kb) X[(i64)(jb + i) * RB + c16] = a2[r];
+      }
     }
   }
   __syncthreads();
-  for (int e = t; e < F.n * nr; e += 256) {
-    int i = e % F.n, r = e / F.n;
-    W[(i64)r * ldw + di[i]] = X[(i64)r * F.N + i];
-  }
+  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+}
+
+// Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.
+__global__ __launch_bounds__(256) void k_to_rows(int n, const double* __restrict__ B, int ld, int nr,
+                                                 double* __restrict__ W) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (i64)n * RB) return;
+  const int i = (int)(e >> 4), r = (int)(e & 15);
+  W[e] = r < nr ? B[(i64)r * ld + i] : 0.0;
+}
+__global__ __launch_bounds__(256) void k_from_rows(int n, const double* __restrict__ W, double* __restrict__ B,
+                                                   int ld, int nr) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (i64)n * nr) return;
+  const int r = (int)(e / n), i = (int)(e % n);
+  B[(i64)r * ld + i] = W[(i64)i * RB + r];
 }
 
 // ----------------------------------------------------------------------------- apply
@@ -503,31 +647,28 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
 // b_rd <- L11 u.  u is kept in X[0:n] for the downward step.
 __global__ __launch_bounds__(256) void k_apply_up(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                   const double* __restrict__ A, const int* __restrict__ dofs,
-                                                  double* __restrict__ W, int ldw, double* __restrict__ xs, int nr) {
+                                                  double* __restrict__ W, double* __restrict__ xs, int nr) {
   const FrontDev F = fr[ids[blockIdx.x]];
   const double* L = A + F.off;
   double* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   const int t = threadIdx.x;
-  for (int e = t; e < F.N * nr; e += 256) {
-    int i = e % F.N, r = e / F.N;
-    X[(i64)r * F.N + i] = W[(i64)r * ldw + (i < F.n ? di[i] : db[i - F.n])];
-  }
+  gather_rhs(F, X, W, di, db);
   __syncthreads();
-  // u_c = sum_{i >= c} L[i][c] X[i]: one column per wave-lane group via a strided reduction
+  // u_c = sum_{i >= c} L[i][c] X[i]: one column at a time, block reduction
   __shared__ double red[256];
   for (int c = 0; c < F.n; c++) {
     for (int r = 0; r < nr; r++) {
       double s = 0.0;
-      for (int i = c + t; i < F.N; i += 256) s += L[(i64)c * F.N + i] * X[(i64)r * F.N + i];
+      for (int i = c + t; i < F.N; i += 256) s += L[(i64)c * F.N + i] * X[(i64)i * RB + r];
       red[t] = s;
       __syncthreads();
       for (int h = 128; h > 0; h >>= 1) {
         if (t < h) red[t] += red[t + h];
         __syncthreads();
       }
-      if (t == 0) X[(i64)r * F.N + c] = red[0];  // row c is no longer read by later columns
+      if (t == 0) X[(i64)c * RB + r] = red[0];  // row c is no longer read by later columns
       __syncthreads();
     }
   }
@@ -535,8 +676,8 @@ __global__ __launch_bounds__(256) void k_apply_up(const int* __restrict__ ids, c
   for (int i = t; i < F.n; i += 256)
     for (int r = 0; r < nr; r++) {
       double s = 0.0;
-      for (int c = 0; c <= i; c++) s += L[(i64)c * F.N + i] * X[(i64)r * F.N + c];
-      W[(i64)r * ldw + di[i]] = s;
+      for (int c = 0; c <= i; c++) s += L[(i64)c * F.N + i] * X[(i64)c * RB + r];
+      W[(i64)di[i] * RB + r] = s;
     }
 }
 
@@ -547,10 +688,11 @@ __global__ __launch_bounds__(256) void k_apply_down(const int* __restrict__ ids,
   const double* L = A + F.off;
   double* X = xs + F.x_off;
   for (int i = F.n + threadIdx.x; i < F.N; i += 256)
-    for (int r = 0; r < nr; r++) {
+    for (int r = 0; r < RB; r++) {
       double s = 0.0;
-      for (int c = 0; c < F.n; c++) s += L[(i64)c * F.N + i] * X[(i64)r * F.N + c];
-      X[(i64)r * F.N + i] = s;
+      if (r < nr)
+        for (int c = 0; c < F.n; c++) s += L[(i64)c * F.N + i] * X[(i64)c * RB + r];
+      X[(i64)i * RB + r] = s;
     }
 }
 

@@ -594,6 +594,10 @@ std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int 
   };
   for (int f = 0; f < P.nfronts(); f++) place(P.fronts[f].n + P.fronts[f].m, P.fronts[f].n + P.fronts[f].m);
   for (int s = 0; s < P.nseps_total(); s++) place(P.seps[s].k, P.seps[s].k);
+  for (int f = 0; f < P.nfronts(); f++) {
+    P.fronts[f].dinv_off = off;
+    off += (i64)((P.fronts[f].n + 63) / 64) * 64 * 64;
+  }
   P.arena_elems = off;
 
   // assembly programs

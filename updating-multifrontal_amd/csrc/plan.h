@@ -69,6 +69,7 @@ struct FrontInfo {
   int sep[6];         // global separator ids of the neighbours, in bd order
   int sep_pos[6];     // their offsets within bdDOFs
   int sep_len[6];
+  i64 dinv_off;  // arena offset of the inverted 64x64 diagonal blocks of L11 (ceil(n/64) of them)
 };
 struct SepInfo {
   int level, idx;

@@ -91,7 +91,9 @@ struct umf_plan {
   size_t ids_cap = 0;
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
   double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
-  double* d_w = nullptr;   // solve work vector n x RB
+  double* d_w = nullptr;   // solve work block n x RB, row-major
+  double* d_stage = nullptr;  // device copy of host right-hand sides
+  int stage_cols = 0;
   i64 xs_elems = 0;
   std::vector<FrontDev> fronts_h;
   std::vector<SepDev> seps_h;
@@ -276,38 +278,40 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
              p->arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
              (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
-    // 3. blocked partial Cholesky of the pivot columns
+    // 3. blocked partial Cholesky of the pivot columns, 64 at a time.  Levels with many fronts
+    //    go left-looking (each block column is formed once by an MFMA GEMM over the finished
+    //    columns, then factored; no trailing read-modify-write); levels with few large fronts go
+    //    right-looking (factor the block, then one wide trailing update) for parallelism.
+    const bool left = L.f_cnt >= 32;
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       int rows = 0, tiles = 0;
       bool partial = false;
-      double fpot = 0, ftrs = 0, fgem = 0, bgem = 0, btrs = 0;
+      double fdiag = 0, fpan = 0, fgem = 0, bgem = 0, bpan = 0;
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
         int kb = std::min(NB, F.n - k0), N = F.n + F.m;
         partial |= kb < NB;
+        double K = left ? k0 : 0;
         rows = std::max(rows, N - k0 - kb);
         tiles = std::max(tiles, tiles_of(k0 + kb, N, F.n));
         double s0 = k0 + kb, nc = F.n - s0, nr = N - s0;
-        fpot += (double)kb * kb * kb / 3.0;
-        ftrs += nr * kb * kb;
-        btrs += 16.0 * nr * kb;
-        if (nc > 0) {
+        fdiag += (double)kb * kb * kb / 3.0;
+        // TRSM (substitution count) + left-looking GEMM + the future diagonal tile's update
+        fpan += nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
+        bpan += 16.0 * nr * kb + 8.0 * nr * K;
+        if (!left && nc > 0) {
           double elems = nc * nr - nc * (nc - 1) / 2;  // lower part of the trailing region
           fgem += 2.0 * kb * elems;
           bgem += 16.0 * elems + 8.0 * kb * (nr + nc);
         }
       }
-      launch(p, "k_potrf_diag", fpot, 0, k_potrf_diag, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+      launch(p, "k_chol_diag", fdiag, 0, k_chol_diag, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
              p->arena, k0, p->d_err);
-      if (rows > 0) {
-        launch(p, "k_trsm_panel", ftrs, btrs, k_trsm_panel<true>, dim3((rows + 255) / 256, L.f_cnt), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, p->arena, k0);
-        if (partial)
-          launch(p, "k_trsm_panel_tail", 0, 0, k_trsm_panel<false>, dim3((rows + 255) / 256, L.f_cnt),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, p->arena, k0);
-      }
-      if (tiles > 0)
+      if (rows > 0)
+        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel, dim3((rows + NB - 1) / NB, L.f_cnt), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, p->arena, k0, left ? 1 : 0);
+      if (!left && tiles > 0)
         launch(p, "k_gemm_panel", fgem, bgem, k_gemm<0>, dim3(tiles, L.f_cnt), ids + L.f_off,
                (const FrontDev*)p->d_fronts, p->arena, k0);
     }
@@ -409,27 +413,27 @@ int sweep(umf_plan* p, int mode, int nr) {
     for (int ell = L - 1; ell >= 1; ell--) {
       work(ell, fl, by);
       launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
       launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
-             (const int*)p->d_dofs, p->d_w, n, (const double*)p->d_xs, nr);
+             (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
     }
     work(0, fl, by);
     launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(1), ids + A.f_off[0], fr, (const double*)p->arena,
-           (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+           (const int*)p->d_dofs, p->d_w, p->d_xs);
     for (int ell = 0; ell < L; ell++) {
       work(ell, fl, by);
       launch(p, "k_solve_bwd", fl, by, k_solve_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
     }
   } else {
     for (int ell = L - 1; ell >= 0; ell--)
       launch(p, "k_apply_up", 0, 0, k_apply_up, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const double*)p->arena,
-             (const int*)p->d_dofs, p->d_w, n, p->d_xs, nr);
+             (const int*)p->d_dofs, p->d_w, p->d_xs, nr);
     for (int ell = 1; ell < L; ell++) {
       launch(p, "k_apply_down", 0, 0, k_apply_down, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
              (const double*)p->arena, p->d_xs, nr);
       launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
-             (const int*)p->d_dofs, p->d_w, n, (const double*)p->d_xs, nr);
+             (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
     }
   }
   HIPCHK(hipGetLastError());
@@ -445,18 +449,36 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   int rc = ensure_solve_scratch(p);
   if (rc) return rc;
   int n = p->P.n;
-  hipMemcpyKind kin = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  hipMemcpyKind kout = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
+  // host operands are staged through a device column block first
+  double* d_in = const_cast<double*>(in);
+  double* d_out = out;
+  if (!device) {
+    if (!p->d_stage || p->stage_cols < nrhs) {
+      if (p->d_stage) HIPCHK(hipFree(p->d_stage));
+      p->d_stage = nullptr;
+      HIPCHK(hipMalloc((void**)&p->d_stage, sizeof(double) * (size_t)n * nrhs));
+      p->stage_cols = nrhs;
+    }
+    HIPCHK(hipMemcpy2DAsync(p->d_stage, sizeof(double) * n, in, sizeof(double) * ld, sizeof(double) * n, nrhs,
+                            hipMemcpyHostToDevice, p->stream));
+    d_in = d_out = p->d_stage;
+  }
+  const int ldd = device ? ld : n;
   for (int c0 = 0; c0 < nrhs; c0 += RB) {
     int nr = std::min(RB, nrhs - c0);
-    HIPCHK(hipMemcpy2DAsync(p->d_w, sizeof(double) * n, in + (size_t)c0 * ld, sizeof(double) * ld,
-                            sizeof(double) * n, nr, kin, p->stream));
+    unsigned gb = (unsigned)(((i64)n * RB + 255) / 256);
+    launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows, dim3(gb), n, (const double*)(d_in + (size_t)c0 * ldd), ldd,
+           nr, p->d_w);
     rc = sweep(p, mode, nr);
     if (rc) return rc;
-    HIPCHK(hipMemcpy2DAsync(out + (size_t)c0 * ld, sizeof(double) * ld, p->d_w, sizeof(double) * n,
-                            sizeof(double) * n, nr, kout, p->stream));
+    unsigned go = (unsigned)(((i64)n * nr + 255) / 256);
+    launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows, dim3(go), n, (const double*)p->d_w,
+           d_out + (size_t)c0 * ldd, ldd, nr);
   }
+  if (!device)
+    HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * ld, p->d_stage, sizeof(double) * n, sizeof(double) * n, nrhs,
+                            hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   prof_collect(p);
@@ -473,7 +495,7 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
 
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
-                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_ids, p->d_xs, p->d_w};
+                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_ids, p->d_xs, p->d_w, p->d_stage};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -565,6 +587,7 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
       D.sep_len[i] = i < F.nsep ? F.sep_len[i] : 0;
     }
     D.x_off = xo;
+    D.dinv_off = F.dinv_off;
     xo += (i64)D.N * RB;
     p->fronts_h.push_back(D);
   }
