@@ -142,6 +142,9 @@ int umf_get_stats(const umf_plan* plan, umf_stats* out);
  * bracketed by HIP events on the plan stream and its algorithmic flops / bytes are counted.
  * umf_profile_read(plan, i, ...) fills entry i and returns the number of kernels seen. */
 int umf_profile_enable(umf_plan* plan, int on);
+/* Restrict the timing to launches of one kernel (base name, e.g. "k_chol_panel"), so that a
+ * timed region carries events only around the kernel being measured; NULL or "" = all. */
+int umf_profile_select(umf_plan* plan, const char* kernel);
 int umf_profile_reset(umf_plan* plan);
 int umf_profile_read(const umf_plan* plan, int i, char* name, int name_len, double* ms_total, int* launches,
                      double* flops, double* bytes);

@@ -18,6 +18,7 @@ struct FrontDev {
   i64 dinv_off; // inverted 64x64 diagonal blocks of L11, column-major, zero above the diagonal
   int n, m, N, nsep;
   int sep_pos[6], sep_len[6];
+  int flag_off;  // first of the ceil(n/64) block flags of the blocked solve sweeps
 };
 struct SepDev {
   i64 off;  // arena offset of the k x k A11

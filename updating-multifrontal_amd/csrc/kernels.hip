@@ -155,23 +155,43 @@ __device__ __forceinline__ int acc_col(int bj, int r) {
 // A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
 // acc += P Q^T for 16 x 16 blocks held in LDS, one wave, v_mfma_f64_16x16x4 (K = 16):
 // P(i,k) = P[i*psi + k*psk], Q(j,k) = Q[j*qsj + k*qsk]; accumulator r of a lane is
-// C(lane%16, lane/16 + 4r).
+// C(lane%16, lane/16 + 4r).  PM / QM mark a triangular operand whose other triangle is not
+// stored as zeros: 1 = entries with k > row index read as zero, 2 = entries with k < row index.
+template <int PM = 0, int QM = 0>
 __device__ __forceinline__ void mma16(d4& acc, const double* P, int psi, int psk, const double* Q, int qsj, int qsk) {
   const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int k4 = 0; k4 < 4; k4++) {
     const int k = 4 * k4 + kq;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Q[i * qsj + k * qsk], P[i * psi + k * psk], acc, 0, 0, 0);
+    double pv = P[i * psi + k * psk], qv = Q[i * qsj + k * qsk];
+    if ((PM == 1 && k > i) || (PM == 2 && k < i)) pv = 0.0;
+    if ((QM == 1 && k > i) || (QM == 2 && k < i)) qv = 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qv, pv, acc, 0, 0, 0);
   }
+}
+
+// 1/sqrt(d) from the hardware estimate and two Newton steps, y <- y (3/2 - d y^2 / 2), which
+// carry the estimate to full double precision.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-0.5 * d * y, y, 0.5);
+    y = fma(y, e, y);
+  }
+  return y;
 }
 
 // Cholesky of the 64 x 64 diagonal block at (jb, jb) of each front with its inverse, one
 // workgroup per front, as 4 x 4 blocks of 16 x 16 in LDS:
-//   for k: factor + invert D_kk (one thread per element, 16 barrier steps); panel
-//   L_ik = A_ik D_kk^{-T} and trailing A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave;
-//   then the off-diagonal blocks of the inverse, Linv_ij = -D_ii^{-1} sum_k L_ik Linv_kj,
-//   by distance i - j.
-// T[c*LD + r] holds L (lower), U[c*LD + r] holds Linv (lower).
+//   for k: eliminate D_kk on [D_kk | I] without scaling the pivot rows (one thread per element,
+//   one barrier per column: step j reads column j and row j of the inverse part and writes only
+//   later columns / rows), then scale: L_kk = D' diag(r), Linv_kk = diag(r) V with r_j =
+//   1/sqrt(pivot j); panel L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one
+//   16-block per wave; then the off-diagonal blocks of the inverse,
+//   Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance i - j.
+// T[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
+// triangle: Linv(r, c) at U[r*LD + c], U = T + LD, for r >= c.
 #define LD (NB + 1)
 __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    double* __restrict__ A, int jb, int* __restrict__ err) {
@@ -179,54 +199,56 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
   const FrontDev F = fr[fid];
   if (jb >= F.n) return;
   const int kb = min(NB, F.n - jb);
-  __shared__ double T[NB * LD];
-  __shared__ double U[NB * LD];
+  __shared__ double T[LD * LD];
+  __shared__ double rs[NB];
+  double* U = T + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   double* blk = A + F.off + (i64)jb * F.N + jb;
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e & 63, j = e >> 6;
-    double v;
-    if (i < kb && j < kb) v = i >= j ? blk[(i64)j * F.N + i] : 0.0;
-    else v = i == j ? 1.0 : 0.0;  // identity padding
+    double v = 0.0;  // strict upper part: the inverse, initially zero
+    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? 1.0 : 0.0);  // identity padding
     T[j * LD + i] = v;
-    U[j * LD + i] = 0.0;
   }
+  if (t < NB) T[NB * LD + t] = 0.0;
   __syncthreads();
+  if (t < NB) U[t * LD + t] = 1.0;
   const int er = t & 15, ec = t >> 4;  // element of a 16 x 16 block
   for (int k = 0; k < 4; k++) {
     const int K0 = 16 * k;
-    double* Dk = T + K0 * LD + K0;
-    double* Vk = U + K0 * LD + K0;
-    if (ec == er) Vk[ec * LD + er] = 1.0;
-    // (a) unblocked Cholesky of D_kk on [D | I]
+    double* Dk = T + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
+    double* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
     for (int j = 0; j < 16; j++) {
       __syncthreads();
       const double d = Dk[j * LD + j];
-      if (t == 0 && K0 + j < kb && !(d > 0.0)) {
-        if (atomicCAS(err, 0, 1) == 0) { err[1] = fid; err[2] = jb + K0 + j; }
+      const double ar = Dk[j * LD + er], ac = Dk[j * LD + ec], vj = Vk[j * LD + ec];
+      const double r = rsqrt_nr(d);
+      if (t == 0) {
+        rs[K0 + j] = r;
+        if (K0 + j < kb && !(d > 0.0) && atomicCAS(err, 0, 1) == 0) {
+          err[1] = fid;
+          err[2] = jb + K0 + j;
+        }
       }
-      const double sd = sqrt(d), inv = 1.0 / sd;
-      const double lr = Dk[j * LD + er] * inv, lc = Dk[j * LD + ec] * inv;
-      const double ajc = Vk[ec * LD + j] * inv;  // row j of the inverse, scaled
-      __syncthreads();
-      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= lr * lc;
-      if (er > j && ec <= j) Vk[ec * LD + er] -= lr * ajc;
-      if (er == j && ec <= j) Vk[ec * LD + j] = ajc;
-      if (ec == j) {
-        if (er > j) Dk[j * LD + er] = lr;
-        else if (er == j) Dk[j * LD + j] = sd;
-      }
+      const double lr = ar * r;
+      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= lr * (ac * r);
+      if (er > j && ec <= j) Vk[er * LD + ec] -= (lr * r) * vj;
     }
     __syncthreads();
-    // (b) panel blocks below: L_ik = A_ik D_kk^{-T}, wave w -> block i = k+1+w
+    if (er >= ec) {
+      Dk[ec * LD + er] *= rs[K0 + ec];
+      Vk[er * LD + ec] *= rs[K0 + er];
+    }
+    __syncthreads();
+    // (b) panel blocks below: L_ik = A_ik Linv_kk^T, wave w -> block i = k+1+w
     {
       const int i = k + 1 + w;
       d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
-      if (i < 4) mma16(acc, T + K0 * LD + 16 * i, 1, LD, Vk, 1, LD);
+      if (i < 4) mma16<0, 1>(acc, T + K0 * LD + 16 * i, 1, LD, Vk, LD, 1);
       __syncthreads();
       if (i < 4)
 #pragma unroll
-        for (int r = 0; r < 4; r++) T[(K0 + (lane >> 4) + 4 * r) * LD + 16 * i + (lane & 15)] = acc[r];
+        for (int q = 0; q < 4; q++) T[(K0 + (lane >> 4) + 4 * q) * LD + 16 * i + (lane & 15)] = acc[q];
     }
     __syncthreads();
     // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
@@ -240,29 +262,35 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
       if (bi < 0) continue;
       d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
       mma16(acc, T + K0 * LD + 16 * bi, 1, LD, T + K0 * LD + 16 * bj, 1, LD);
+      // a diagonal block keeps its strict upper triangle, which holds the inverse
 #pragma unroll
-      for (int r = 0; r < 4; r++) T[(16 * bj + (lane >> 4) + 4 * r) * LD + 16 * bi + (lane & 15)] -= acc[r];
+      for (int q = 0; q < 4; q++)
+        if (bi != bj || (lane & 15) >= (lane >> 4) + 4 * q)
+          T[(16 * bj + (lane >> 4) + 4 * q) * LD + 16 * bi + (lane & 15)] -= acc[q];
     }
-    __syncthreads();
   }
-  // inverse off-diagonal blocks, by distance: Linv_ij = -D_ii^{-1} S, S = sum_{k=j}^{i-1} L_ik Linv_kj
-  __shared__ double Sbuf[4][16 * 17];
+  __syncthreads();
+  // inverse off-diagonal blocks, by distance; S = sum_kk L_i,kk Linv_kk,j is staged in the
+  // (not yet computed) destination block Linv_ij.
   for (int dist = 1; dist < 4; dist++) {
     const int i = w + dist, j = w;
+    double* Sij = U + 16 * i * LD + 16 * j;  // S(i', j') at Sij[i'*LD + j']
     d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
-    if (i < 4)
-      for (int kk = j; kk < i; kk++)  // S += L_i,kk Linv_kk,j: P = L_i,kk (i,k), Q(j',k) = Linv_kk,j(k, j')
-        mma16(acc, T + 16 * kk * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * kk, LD, 1);
+    if (i < 4) {  // P(i', k) = L(16i+i', 16kk+k), Q(j', k) = Linv(16kk+k, 16j+j'), lower for kk = j
+      mma16<0, 2>(acc, T + 16 * j * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * j, 1, LD);
+      for (int kk = j + 1; kk < i; kk++) mma16(acc, T + 16 * kk * LD + 16 * i, 1, LD, U + 16 * kk * LD + 16 * j, 1, LD);
+    }
     if (i < 4)
 #pragma unroll
-      for (int r = 0; r < 4; r++) Sbuf[w][((lane >> 4) + 4 * r) * 17 + (lane & 15)] = acc[r];  // S(i', j') at [j'*17+i']
+      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + (lane >> 4) + 4 * q] = acc[q];
     __syncthreads();
     d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
-    if (i < 4)  // Linv_ij = -(D_ii^{-1} S): P = D_ii^{-1} (i', k), Q(j', k) = S(k, j')
-      mma16(a2, U + 16 * i * LD + 16 * i, 1, LD, Sbuf[w], 17, 1);
+    if (i < 4)  // P(i', k) = Linv_ii(i', k), Q(j', k) = S(k, j')
+      mma16<1, 0>(a2, U + 16 * i * LD + 16 * i, LD, 1, Sij, 1, LD);
+    __syncthreads();
     if (i < 4)
 #pragma unroll
-      for (int r = 0; r < 4; r++) U[(16 * j + (lane >> 4) + 4 * r) * LD + 16 * i + (lane & 15)] = -a2[r];
+      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + (lane >> 4) + 4 * q] = -a2[q];
     __syncthreads();
   }
   for (int e = t; e < NB * NB; e += 256) {
@@ -270,7 +298,10 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
     if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = T[j * LD + i];
   }
   double* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
-  for (int e = t; e < NB * NB; e += 256) dinv[e] = U[(e >> 6) * LD + (e & 63)];  // column-major, zero above
+  for (int e = t; e < NB * NB; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    dinv[e] = r >= c ? U[r * LD + c] : 0.0;  // column-major, zero above the diagonal
+  }
 }
 
 __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
@@ -624,6 +655,184 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
   }
   __syncthreads();
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+}
+
+// ---- blocked sweeps for large fronts: one workgroup per 64-row block, blocks of a front
+// hand their results to one another through per-block flags (set to the launch's epoch).
+// A workgroup only waits for blocks owned by workgroups with smaller ids, which the hardware
+// dispatches first, and every wait is bounded: past the limit err[3] is set and the
+// workgroup goes on, so the launch always drains.
+#define SPIN_LIMIT (1 << 22)
+// The flag is polled with relaxed agent-scope loads (no cache maintenance per poll); one
+// acquire fence follows the successful poll.  Publication: every wave's stores are released at
+// agent scope before the barrier, then one relaxed store sets the flag.
+__device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err) {
+  if (threadIdx.x == 0) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      if (++spins > SPIN_LIMIT) {
+        atomicCAS(err + 3, 0, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void post_block(int* flag, int epoch) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forward sweep of one front by blocks (blockIdx.x = block, blockIdx.y = front): interior block
+// b waits for the blocks j < b, forms v = b_b - sum_j L_bj x_j and publishes x_b = Linv_b v
+// (to X and to W[int]); boundary block b accumulates over every interior block and leaves
+// X[rows] = -sum_j L_bj x_j, the contribution gathered by k_sep_acc.
+__global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 const double* __restrict__ A, const int* __restrict__ dofs,
+                                                 double* __restrict__ W, double* __restrict__ xs, int* flags, int epoch,
+                                                 int* err) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  const int nbi = (F.n + NB - 1) / NB, nbt = nbi + (F.m + NB - 1) / NB;
+  const int b = blockIdx.x;
+  if (b >= nbt) return;
+  const bool interior = b < nbi;
+  const int r0 = interior ? NB * b : F.n + NB * (b - nbi);
+  const int rend = min(interior ? F.n : F.N, r0 + NB);
+  const double* L = A + F.off;
+  double* X = xs + F.x_off;
+  const int* di = dofs + F.int_off;
+  int* fl = flags + F.flag_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  __shared__ double vb[NB][RB];
+  const int row = r0 + 16 * w + c16;  // L row supplied by this lane
+  const bool rok = row < rend;
+  const int rowc = rok ? row : r0;
+  d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+  const int jmax = interior ? b : nbi;
+  for (int j = 0; j < jmax; j++) {
+    wait_block(fl + j, epoch, err);
+    const int jb = NB * j, kb = min(NB, F.n - jb);
+    double av[NB / 4], bv[NB / 4];
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = 4 * k4 + q;
+      const bool kin = k < kb;
+      av[k4] = (rok && kin) ? L[(i64)(jb + k) * F.N + rowc] : 0.0;
+      bv[k4] = kin ? X[(i64)(jb + k) * RB + c16] : 0.0;
+    }
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], bv[k4], acc, 0, 0, 0);
+  }
+  if (!interior) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = r0 + 16 * w + q + 4 * r;
+      if (i < rend) X[(i64)i * RB + c16] = -acc[r];
+    }
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = 16 * w + q + 4 * r;
+    vb[i][c16] = r0 + i < rend ? W[(i64)di[r0 + i] * RB + c16] - acc[r] : 0.0;
+  }
+  __syncthreads();
+  const double* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
+  d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < NB / 4; k4++) {
+    const int c = 4 * k4 + q;
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[c * NB + 16 * w + c16], vb[c][c16], a2, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = r0 + 16 * w + q + 4 * r;
+    if (i < rend) {
+      X[(i64)i * RB + c16] = a2[r];
+      W[(i64)di[i] * RB + c16] = a2[r];
+    }
+  }
+  post_block(fl + b, epoch);
+}
+
+// Backward sweep of one front by blocks, last block first (workgroup b owns block
+// nbi-1-b): z = y_i - L(rows below, i)^T x(rows below), boundary rows (read from W[bd]) first,
+// then the interior blocks below as they are published; x_i = Linv_i^T z goes to X and W[int].
+__global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 const double* __restrict__ A, const int* __restrict__ dofs,
+                                                 double* __restrict__ W, double* __restrict__ xs, int* flags, int epoch,
+                                                 int* err) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  const int nbi = (F.n + NB - 1) / NB;
+  if ((int)blockIdx.x >= nbi) return;
+  const int ib = nbi - 1 - blockIdx.x;
+  const int jb = NB * ib, kb = min(NB, F.n - jb);
+  const double* L = A + F.off;
+  double* X = xs + F.x_off;
+  const int* di = dofs + F.int_off;
+  const int* db = dofs + F.bd_off;
+  int* fl = flags + F.flag_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  __shared__ double zb[NB][RB];
+  const int col = 16 * w + c16;  // column of the block supplied by this lane
+  const bool cin = col < kb;
+  const double* Lc = L + (i64)(jb + (cin ? col : 0)) * F.N;
+  d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int i0 = F.n; i0 < F.N; i0 += 16 * GU) {  // boundary rows: x = W[bd]
+    double av[GU][4], bv[GU][4];
+#pragma unroll
+    for (int u = 0; u < GU; u++)
+#pragma unroll
+      for (int k4 = 0; k4 < 4; k4++) {
+        const int i = i0 + 16 * u + 4 * k4 + q;
+        const bool ok = i < F.N;
+        av[u][k4] = (ok && cin) ? Lc[i] : 0.0;
+        bv[u][k4] = ok ? W[(i64)db[i - F.n] * RB + c16] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < GU; u++)
+#pragma unroll
+      for (int k4 = 0; k4 < 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], bv[u][k4], acc, 0, 0, 0);
+  }
+  for (int kbk = nbi - 1; kbk > ib; kbk--) {  // interior blocks below, in publication order
+    wait_block(fl + kbk, epoch, err);
+    const int k0 = NB * kbk, kn = min(NB, F.n - k0);
+    double av[NB / 4], bv[NB / 4];
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = 4 * k4 + q;
+      const bool kin = k < kn;
+      av[k4] = (kin && cin) ? Lc[k0 + k] : 0.0;
+      bv[k4] = kin ? X[(i64)(k0 + k) * RB + c16] : 0.0;
+    }
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], bv[k4], acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int c = 16 * w + q + 4 * r;
+    zb[c][c16] = c < kb ? W[(i64)di[jb + c] * RB + c16] - acc[r] : 0.0;
+  }
+  __syncthreads();
+  const double* Li = A + F.dinv_off + (i64)ib * NB * NB;
+  d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < NB / 4; k4++) {
+    const int c = 4 * k4 + q;
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[(16 * w + c16) * NB + c], zb[c][c16], a2, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = 16 * w + q + 4 * r;
+    if (i < kb) {
+      X[(i64)(jb + i) * RB + c16] = a2[r];
+      W[(i64)di[jb + i] * RB + c16] = a2[r];
+    }
+  }
+  post_block(fl + ib, epoch);
 }
 
 // Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.

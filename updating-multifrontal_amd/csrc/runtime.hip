@@ -51,6 +51,7 @@ struct KStat {
 };
 struct Prof {
   bool on = false;
+  std::string only;  // when non-empty, only launches of this kernel are timed
   std::vector<KStat> stats;
   std::vector<hipEvent_t> pool;
   size_t used = 0;
@@ -87,6 +88,9 @@ struct umf_plan {
   int* d_ci = nullptr;
   double* d_val = nullptr;
   int* d_err = nullptr;
+  int* d_flags = nullptr;  // block flags of the blocked solve sweeps
+  int n_flags = 0;
+  int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
@@ -178,11 +182,20 @@ int ensure_ids(umf_plan* p, size_t n) {
 
 // Launch on the plan stream; with profiling on, bracket it by a pair of pooled events and
 // remember its algorithmic work.
+// tree level of the launches that follow (-1: none), appended to kernel names when profiling
+thread_local int g_level = -1;
+
 template <class K, class... Args>
 void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, Args... args) {
   Prof& pr = p->prof;
   hipEvent_t a = nullptr, b = nullptr;
-  if (pr.on) {
+  char tagged[64];
+  const bool timed = pr.on && (pr.only.empty() || pr.only == name);
+  if (timed && g_level >= 0) {
+    snprintf(tagged, sizeof tagged, "%s@L%d", name, g_level);
+    name = tagged;
+  }
+  if (timed) {
     while (pr.pool.size() < pr.used + 2) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) break;
@@ -241,6 +254,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
   auto lowerN = [](double N) { return N * (N + 1) / 2; };
   for (int ell = g.L - 1; ell >= 0; ell--) {
     const Sched::Lvl& L = S.lv[ell];
+    g_level = ell;
     // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
     // then per triple (12 B index + 8 B value read + 8 B write), per copied element 16 B
     auto asm_work = [&](int off, int cnt, double& zb, double& ab) {
@@ -338,6 +352,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
     }
   }
   (void)NF;
+  g_level = -1;
   HIPCHK(hipGetLastError());
   return UMF_OK;
 }
@@ -385,6 +400,12 @@ AllLists all_lists(const Plan& P) {
   return A;
 }
 
+// Epoch of the next blocked sweep launch: flags still holding an earlier epoch read as unset.
+int next_epoch(umf_plan* p) {
+  p->epoch = p->epoch == 0x7fffffff ? 1 : p->epoch + 1;
+  return p->epoch;
+}
+
 // Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the work vector d_w (ld = n).
 int sweep(umf_plan* p, int mode, int nr) {
   const Plan& P = p->P;
@@ -409,21 +430,49 @@ int sweep(umf_plan* p, int mode, int nr) {
     }
   };
   double fl, by;
-  if (mode == 0) {
-    for (int ell = L - 1; ell >= 1; ell--) {
-      work(ell, fl, by);
+  // Levels of few (<= 16), large fronts are swept by 64-row blocks, one workgroup each
+  // (k_fwd_blk / k_bwd_blk); levels of many fronts keep one workgroup per front.
+  auto blocked = [&](int ell) { return A.f_cnt[ell] <= 16; };
+  auto blocks = [&](int ell, int& nbi, int& nbt) {
+    nbi = nbt = 1;
+    for (int f : P.box_gid[ell]) {
+      const FrontInfo& F = P.fronts[f];
+      nbi = std::max(nbi, (F.n + NB - 1) / NB);
+      nbt = std::max(nbt, (F.n + NB - 1) / NB + (F.m + NB - 1) / NB);
+    }
+  };
+  auto fwd = [&](int ell) {
+    work(ell, fl, by);
+    g_level = ell;
+    if (!blocked(ell)) {
       launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
              (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
+    } else {
+      int nbi, nbt;
+      blocks(ell, nbi, nbt);
+      launch(p, "k_fwd_blk", fl, by, k_fwd_blk, dim3(nbt, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs, p->d_flags, next_epoch(p), p->d_err);
+    }
+  };
+  if (mode == 0) {
+    for (int ell = L - 1; ell >= 1; ell--) {
+      fwd(ell);
       launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
              (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
     }
-    work(0, fl, by);
-    launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(1), ids + A.f_off[0], fr, (const double*)p->arena,
-           (const int*)p->d_dofs, p->d_w, p->d_xs);
+    fwd(0);
     for (int ell = 0; ell < L; ell++) {
       work(ell, fl, by);
-      launch(p, "k_solve_bwd", fl, by, k_solve_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
+      g_level = ell;
+      if (!blocked(ell)) {
+        launch(p, "k_solve_bwd", fl, by, k_solve_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+               (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
+      } else {
+        int nbi, nbt;
+        blocks(ell, nbi, nbt);
+        launch(p, "k_bwd_blk", fl, by, k_bwd_blk, dim3(nbi, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+               (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs, p->d_flags, next_epoch(p), p->d_err);
+      }
     }
   } else {
     for (int ell = L - 1; ell >= 0; ell--)
@@ -436,6 +485,7 @@ int sweep(umf_plan* p, int mode, int nr) {
              (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
     }
   }
+  g_level = -1;
   HIPCHK(hipGetLastError());
   return UMF_OK;
 }
@@ -485,6 +535,12 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
   if (mode == 0) {
+    int serr = 0;
+    HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
+    if (serr) {
+      HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
+      return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
+    }
     p->stats.t_solve_ms = ms;
     double fl = 0;
     for (const FrontInfo& F : p->P.fronts) fl += 2.0 * F.n * F.n + 4.0 * F.n * F.m;
@@ -495,7 +551,7 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
 
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
-                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_ids, p->d_xs, p->d_w, p->d_stage};
+                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -588,6 +644,8 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
     }
     D.x_off = xo;
     D.dinv_off = F.dinv_off;
+    D.flag_off = p->n_flags;
+    p->n_flags += (F.n + NB - 1) / NB;
     xo += (i64)D.N * RB;
     p->fronts_h.push_back(D);
   }
@@ -634,6 +692,10 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
   UP(p->d_val, p->val_h);
 #undef UP
   if ((he = hipMalloc((void**)&p->d_err, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMalloc((void**)&p->d_flags, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMemset(p->d_flags, 0, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
   double fl = 0;
   for (int f = 0; f < P.nfronts(); f++) fl += P.front_flops(f);
   p->stats.flops_factor = fl;
@@ -952,6 +1014,11 @@ void* umf_stream(umf_plan* p) { return p ? (void*)p->stream : nullptr; }
 int umf_profile_enable(umf_plan* p, int on) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   p->prof.on = on != 0;
+  return UMF_OK;
+}
+int umf_profile_select(umf_plan* p, const char* kernel) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  p->prof.only = kernel ? kernel : "";
   return UMF_OK;
 }
 int umf_profile_reset(umf_plan* p) {

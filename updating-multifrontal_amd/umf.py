@@ -53,6 +53,7 @@ _update = _sig("umf_update", C.c_int, [_vp, _ip, _ip, _dp, C.c_int, _ip])
 _update_dev = _sig("umf_update_device", C.c_int, [_vp, _vp, C.c_int, _ip])
 _prof_enable = _sig("umf_profile_enable", C.c_int, [_vp, C.c_int])
 _prof_reset = _sig("umf_profile_reset", C.c_int, [_vp])
+_prof_select = _sig("umf_profile_select", C.c_int, [_vp, C.c_char_p])
 _prof_read = _sig("umf_profile_read", C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp, _ip, _dp, _dp])
 _solve = _sig("umf_solve", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
 _apply = _sig("umf_apply", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
@@ -72,7 +73,7 @@ _sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip
 _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 
 EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
-            "umf_update_device", "umf_profile_enable", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
+            "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream"]
@@ -223,6 +224,10 @@ class _MF:
 
     def profile(self, on=True):
         _check(_prof_enable(self._h, 1 if on else 0))
+
+    def profile_select(self, kernel=None):
+        """Time only launches of `kernel` (base name); None = every kernel."""
+        _check(_prof_select(self._h, (kernel or "").encode()))
 
     def profile_reset(self):
         _check(_prof_reset(self._h))
