@@ -101,6 +101,48 @@ def cpu_baseline(N=511, seed=2, nrhs=16, max_side=16):
     }
 
 
+def backward_error(plan, d_B, d_X, d_R, vals, B, nrhs):
+    """max_j ||b_j - A x_j|| / (||A||_F ||x_j|| + ||b_j||), residual by the FP64 SpMV."""
+    plan.spmv_device(d_X.data_ptr(), d_R.data_ptr(), nrhs)
+    R = (d_B - d_R).cpu().numpy()
+    X = d_X.cpu().numpy()
+    normA = float(np.linalg.norm(vals))
+    return max(np.linalg.norm(R[j]) / (normA * np.linalg.norm(X[j]) + np.linalg.norm(B[j])) for j in range(nrhs))
+
+
+def fp32_form(plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
+    """The same step with FP32 fronts on FP32 MFMA and the 16-RHS Solve refined in FP64 to
+    backward error <= --ir-tol; same algorithmic flop numerator as the FP64 form."""
+    import torch
+    plan.SetPrecision("fp32", tol=args.ir_tol, max_iter=50)
+    plan.profile(False)
+    for _ in range(max(args.warmup, 1)):
+        step()
+    tf = tu = ts = flops = 0.0
+    iters = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = plan.stats()
+        tf += st["t_factor_ms"]
+        tu += st["t_update_ms"]
+        ts += st["t_solve_ms"]
+        iters.append(st["ir_iters"])
+        flops += st["flops_factor"] + st["flops_update"] + st["flops_solve"]
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    K = args.steps
+    st = plan.stats()
+    be = backward_error(plan, d_B, d_X, d_R, vals[state["cur"]], B, nrhs)
+    plan.SetPrecision("fp64")
+    return {"value": flops / secs / 1e12, "unit": "TFLOP/s", "ms_per_step": secs / K * 1e3,
+            "t_factor_ms": tf / K, "t_update_ms": tu / K, "t_solve_ms": ts / K,
+            "ir_iters": iters, "ir_tol": args.ir_tol, "backward_error": be, "ir_berr": st["ir_berr"],
+            "front_pool_gb": st["bytes_front_pool"] / 1e9,
+            "dtype": "f32 fronts (FP32 MFMA) + f64 refinement"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +152,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA + FP64-refinement form")
+    ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -175,12 +219,7 @@ def main():
     K = args.steps
 
     # accuracy of the last step: normwise backward error max_j ||b - A x|| / (||A||_F ||x|| + ||b||)
-    plan.spmv_device(d_X.data_ptr(), d_R.data_ptr(), nrhs)
-    R = (d_B - d_R).cpu().numpy()
-    X = d_X.cpu().numpy()
-    vcur = (v1, v2)[state["cur"]]
-    normA = float(np.linalg.norm(vcur))
-    berr = max(np.linalg.norm(R[j]) / (normA * np.linalg.norm(X[j]) + np.linalg.norm(B[j])) for j in range(nrhs))
+    berr = backward_error(plan, d_B, d_X, d_R, (v1, v2)[state["cur"]], B, nrhs)
 
     roof = None
     if dom:
@@ -238,6 +277,8 @@ def main():
         "kernels_by_level": {k: round(v["ms"], 4) for k, v in sorted(prof_lv.items())},
         "cpu_baseline": None,
     }
+    if not args.no_fp32:
+        out["fp32_ir"] = fp32_form(plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nrhs=nrhs)
     print(json.dumps(out), flush=True)

@@ -27,6 +27,7 @@ extern "C" {
 #define UMF_EPATTERN (-5)   /* CSR pattern differs from the planned one */
 #define UMF_EUPDATEMAP (-6) /* a modified DOF maps outside the leaf box grid */
 #define UMF_ESTATE (-7)     /* call out of order (e.g. solve before factor) */
+#define UMF_ENOCONV (-8)    /* FP32 form: refinement did not reach the requested backward error */
 
 typedef struct umf_plan umf_plan;
 
@@ -134,8 +135,22 @@ typedef struct umf_stats {
   double bytes_front_pool; /* device bytes held by fronts and separators */
   int n_fronts;
   int n_fronts_update;
+  int ir_iters;         /* FP32 form: refinement iterations of the last solve (max over columns) */
+  double ir_berr;       /* FP32 form: final normwise backward error (max over columns) */
 } umf_stats;
 int umf_get_stats(const umf_plan* plan, umf_stats* out);
+
+/* Precision form (no reference counterpart; the reference is FP64 throughout).
+ *   UMF_PREC_FP64: fronts stored and factored in FP64 (the drop-in form, the default).
+ *   UMF_PREC_FP32: fronts stored and factored in FP32 on v_mfma_f32_16x16x4 (half the front
+ *     memory); umf_solve then runs preconditioned conjugate gradients in FP64 (FP64 operator
+ *     and residuals, the FP32 multifrontal solve as preconditioner) until every column reaches
+ *     the normwise backward error ||b - A x|| / (||A||_F ||x|| + ||b||) <= tol, at most max_iter
+ *     iterations (UMF_ENOCONV when a column does not); umf_apply applies the FP32 factors.
+ * Takes effect at the next umf_factor (the plan must be factored again). */
+#define UMF_PREC_FP64 0
+#define UMF_PREC_FP32 1
+int umf_set_precision(umf_plan* plan, int precision, double tol, int max_iter);
 
 /* Per-kernel timing (no reference counterpart; the reference's HASCPP11TIMING timers are
  * the nearest, src/DE/mf2d.cpp:463-499): when enabled, every kernel launch of the plan is

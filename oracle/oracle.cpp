@@ -335,6 +335,7 @@ struct Front {
   // dirty-set trace of the last Factor/Update: per level, boxes factored and separators rebuilt
   std::vector<std::vector<int>> traceBoxes, traceSeps;
   int info = 0;
+  bool factored = false;  // the reference's Update on an unfactored object reads unset fronts
 
   virtual int nboxes(int ell) const = 0;
   virtual int nseps(int ell) const = 0;
@@ -914,10 +915,12 @@ int orc_factor(void* h) {
   // from the marks the previous call left behind, as the reference's Factor() does.
   Front* F = (Front*)h;
   F->factor();
+  F->factored = true;
   return F->info;
 }
 int orc_update(void* h, const double* val, int nmod, const int* mod) {
   Front* F = (Front*)h;
+  if (!F->factored) { g_err = "update before factor"; return -2; }
   std::vector<double> v(val, val + F->A.rp[F->A.n]);
   Idx m(mod, mod + nmod);
   int r = F->update(v, m);

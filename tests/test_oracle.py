@@ -287,3 +287,13 @@ def test_checks_reject_planted_mistakes():
     assert rel(A11, g[f"f_blk_{F.L - 1}_0_A11post"]) > 1e-3
     # a solve with one level's contribution dropped misses the fixture
     assert rel(g["b"], g["solve_b"]) > 1e-3
+
+
+def test_update_before_factor_rejected():
+    """The oracle refuses an update of an unfactored object (there are no fronts to update)."""
+    dims = (31, 31)
+    co = coeff_field(dims, "uniform", seed=1)
+    A = laplacian(dims, co)
+    F = Factorization(A, dims, 1, 3, "orc")
+    with pytest.raises(ValueError):
+        F.update(A, np.array([5], dtype=np.int32))

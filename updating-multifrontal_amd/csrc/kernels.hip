@@ -4,7 +4,7 @@
 // front per blockIdx.y (or blockIdx.x), so a whole level runs in one launch per stage.
 //   assembly   k_zero_lower, k_assemble, k_sep_add          (HBM-bound gathers / scatters)
 //   factor     k_chol_diag, k_chol_panel, k_gemm<0>, k_gemm<1>   (blocked Cholesky of the
-//              pivot columns, left- or right-looking, then the Schur update; FP64 MFMA)
+//              pivot columns, left- or right-looking, then the Schur update; MFMA)
 //   solve      k_solve_fwd, k_solve_bwd, k_sep_acc           (multi-RHS level sweeps)
 //   apply      k_apply_up, k_apply_down
 //   residual   k_spmv
@@ -12,35 +12,64 @@
 
 #include "kernels.h"
 
+// accumulator-row layout of v_mfma_f32_16x16x4 on gfx950: accumulator r of lane l holds row
+// 4 (l/16) + r (v_mfma_f64_16x16x4 holds row l/16 + 4r)
+#define MROW_F32(q, r) (4 * (q) + (r))
+
 namespace umf {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
+// Every factor / solve kernel is a template over the storage type T of the fronts: double (the
+// FP64 form) or float (the FP32-MFMA form, a preconditioner for FP64 iterative refinement).
+// The MFMA is v_mfma_f64_16x16x4 or v_mfma_f32_16x16x4 with the same operand layout: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator r of lane l holds D[mrow(l/16, r)][l%16],
+// where the two instructions differ in mrow.
+template <class T> struct Vec4;
+template <> struct Vec4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <class T> using V4 = typename Vec4<T>::type;
+template <class T> __device__ __forceinline__ V4<T> vzero() {
+  V4<T> v;
+  v[0] = v[1] = v[2] = v[3] = T(0);
+  return v;
+}
+__device__ __forceinline__ V4<double> mfma4(double a, double b, V4<double> c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ V4<float> mfma4(float a, float b, V4<float> c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// row within the 16 x 16 result of accumulator r of a lane in group q = lane / 16
+template <class T> __device__ __forceinline__ int mrow(int q, int r);
+template <> __device__ __forceinline__ int mrow<double>(int q, int r) { return q + 4 * r; }
+template <> __device__ __forceinline__ int mrow<float>(int q, int r) { return MROW_F32(q, r); }
 
 #define NB 64  // panel width of the blocked factorization and of the solve blocks
 
 // ----------------------------------------------------------------------------- assembly
 // Zero the lower triangle (row >= col) of each listed matrix.
+template <class T>
 __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids, const MatDev* __restrict__ mats,
-                                                    double* __restrict__ A) {
+                                                    T* __restrict__ A) {
   const MatDev M = mats[ids[blockIdx.y]];
-  double* a = A + M.off;
+  T* a = A + M.off;
   for (int c = blockIdx.x; c < M.cols; c += gridDim.x) {
-    double* col = a + (i64)c * M.ld;
-    for (int r = c + threadIdx.x; r < M.ld; r += 256) col[r] = 0.0;
+    T* col = a + (i64)c * M.ld;
+    for (int r = c + threadIdx.x; r < M.ld; r += 256) col[r] = T(0);
   }
 }
 
 // CSR scatter triples and 64x64 tiles of block copies into each listed matrix.  All writes of
 // one matrix are to distinct entries (the planner emits disjoint programs), so no ordering or
 // atomics are needed; the matrix must have been zeroed by k_zero_lower.
+template <class T>
 __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ ids, const MatDev* __restrict__ mats,
-                                                  double* __restrict__ A, const i64* __restrict__ tri_off,
+                                                  T* __restrict__ A, const i64* __restrict__ tri_off,
                                                   const Triple* __restrict__ tri, const i64* __restrict__ tile_off,
                                                   const Tile* __restrict__ tiles, const BlockOp* __restrict__ ops,
                                                   const double* __restrict__ csr_val) {
   const int mid = ids[blockIdx.y];
   const MatDev M = mats[mid];
-  double* a = A + M.off;
+  T* a = A + M.off;
   const i64 t0 = tri_off[mid], t1 = tri_off[mid + 1];
   for (i64 i = t0 + (i64)blockIdx.x * 256 + threadIdx.x; i < t1; i += (i64)gridDim.x * 256) {
     const Triple t = tri[i];
@@ -48,16 +77,16 @@ __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ ids, c
   }
   const i64 b0 = tile_off[mid], b1 = tile_off[mid + 1];
   for (i64 ti = b0 + blockIdx.x; ti < b1; ti += gridDim.x) {
-    const Tile T = tiles[ti];
-    const BlockOp op = ops[T.op];
+    const Tile tl = tiles[ti];
+    const BlockOp op = ops[tl.op];
     const MatDev S = mats[op.src];
-    const double* s = A + S.off;
+    const T* s = A + S.off;
     const bool trans = op.flags & 1, lower = op.flags & 2;
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-      int r = T.r0 + (e & 63), c = T.c0 + (e >> 6);
+      int r = tl.r0 + (e & 63), c = tl.c0 + (e >> 6);
       if (r >= op.nr || c >= op.nc) continue;
       if (lower && r < c) continue;
-      double v = trans ? s[(i64)(op.sc0 + r) * S.ld + (op.sr0 + c)] : s[(i64)(op.sc0 + c) * S.ld + (op.sr0 + r)];
+      T v = trans ? s[(i64)(op.sc0 + r) * S.ld + (op.sr0 + c)] : s[(i64)(op.sc0 + c) * S.ld + (op.sr0 + r)];
       a[(i64)(op.dc0 + c) * M.ld + (op.dr0 + r)] = v;
     }
   }
@@ -66,46 +95,49 @@ __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ ids, c
 // Separator Schur accumulation: A11 = (A11 + S_box0) + S_box1 on the lower triangle, the two
 // neighbour boxes in ascending index order as the reference's ScatterAdd_ calls
 // (src/DE/mf2d.cpp:599-606).  S is read from each box's trailing diagonal separator block.
+template <class T>
 __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, const SepDev* __restrict__ seps,
-                                                 const FrontDev* __restrict__ fr, double* __restrict__ A) {
+                                                 const FrontDev* __restrict__ fr, T* __restrict__ A) {
   const SepDev S = seps[ids[blockIdx.y]];
   const FrontDev B0 = fr[S.box[0]], B1 = fr[S.box[1]];
-  double* a = A + S.off;
-  const double* s0 = A + B0.off + (i64)(B0.n + S.pos[0]) * B0.N + B0.n + S.pos[0];
-  const double* s1 = A + B1.off + (i64)(B1.n + S.pos[1]) * B1.N + B1.n + S.pos[1];
+  T* a = A + S.off;
+  const T* s0 = A + B0.off + (i64)(B0.n + S.pos[0]) * B0.N + B0.n + S.pos[0];
+  const T* s1 = A + B1.off + (i64)(B1.n + S.pos[1]) * B1.N + B1.n + S.pos[1];
   for (int c = blockIdx.x; c < S.k; c += gridDim.x)
     for (int r = c + threadIdx.x; r < S.k; r += 256) {
-      double v = a[(i64)c * S.k + r] + s0[(i64)c * B0.N + r];
+      T v = a[(i64)c * S.k + r] + s0[(i64)c * B0.N + r];
       a[(i64)c * S.k + r] = v + s1[(i64)c * B1.N + r];
     }
 }
 
 // ----------------------------------------------------------------------------- MFMA tile GEMM
-// acc += P Q^T for one 64 x 64 tile with v_mfma_f64_16x16x4 (4 waves as 2 x 2, each 32 x 32 =
+// acc += P Q^T for one 64 x 64 tile with 16x16x4 MFMAs (4 waves as 2 x 2, each 32 x 32 =
 // 2 x 2 blocks of 16 x 16).  P(i,k) = P[k*ldp + i], Q(j,k) = Q[k*ldq + j]; rows i >= pn and
 // j >= qn read as zero; k in [0, K).  The MFMA A operand is Q and B is P, so accumulator r of a
-// lane holds C(i, j) with i = 16-block row + lane%16 and j = 16-block col + lane/16 + 4r
-// (v_mfma_f64_16x16x4 leaves D[(lane/16) + 4r][lane%16] in accumulator r), and the 16 lanes of a
+// lane holds C(i, j) with i = 16-block row + lane%16 and j = 16-block col + mrow(lane/16, r)
+// (the MFMA leaves D[mrow(lane/16, r)][lane%16] in accumulator r), and the 16 lanes of a
 // group cover 16 consecutive rows of C.
+template <class T>
 struct TileSmem {
-  double As[16][64];
-  double Bs[16][64];
+  T As[16][64];
+  T Bs[16][64];
 };
 
-__device__ __forceinline__ void tile_gemm_nt(d4 (&acc)[2][2], const double* __restrict__ P, int ldp, int pn,
-                                             const double* __restrict__ Q, int ldq, int qn, int K, TileSmem& sm) {
+template <class T>
+__device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __restrict__ P, int ldp, int pn,
+                                             const T* __restrict__ Q, int ldq, int qn, int K, TileSmem<T>& sm) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int li = t & 63, lk = t >> 6;
   const bool prow = li < pn, qrow = li < qn;
-  double pa[4], qb[4];
+  T pa[4], qb[4];
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int k = kc + lk + 4 * q;
       const bool kin = k < K;
-      pa[q] = (kin && prow) ? P[(i64)k * ldp + li] : 0.0;
-      qb[q] = (kin && qrow) ? Q[(i64)k * ldq + li] : 0.0;
+      pa[q] = (kin && prow) ? P[(i64)k * ldp + li] : T(0);
+      qb[q] = (kin && qrow) ? Q[(i64)k * ldq + li] : T(0);
     }
   };
   if (K <= 0) return;
@@ -124,11 +156,11 @@ __device__ __forceinline__ void tile_gemm_nt(d4 (&acc)[2][2], const double* __re
       const int kr = k4 * 4 + (lane >> 4);
 #pragma unroll
       for (int bi = 0; bi < 2; bi++) {
-        const double bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
+        const T bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) {
-          const double av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
-          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+          const T av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
+          acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
         }
       }
     }
@@ -138,8 +170,25 @@ __device__ __forceinline__ void tile_gemm_nt(d4 (&acc)[2][2], const double* __re
 
 // tile-local coordinates of accumulator (bi, bj, r) of this thread
 __device__ __forceinline__ int acc_row(int bi) { return ((threadIdx.x >> 6) >> 1) * 32 + bi * 16 + (threadIdx.x & 15); }
+template <class T>
 __device__ __forceinline__ int acc_col(int bj, int r) {
-  return ((threadIdx.x >> 6) & 1) * 32 + bj * 16 + ((threadIdx.x & 63) >> 4) + 4 * r;
+  return ((threadIdx.x >> 6) & 1) * 32 + bj * 16 + mrow<T>((threadIdx.x & 63) >> 4, r);
+}
+
+// Workgroup -> (tile, front id index).  xt == 0: 2-D grid (tile = blockIdx.x, front =
+// blockIdx.y).  xt > 0: 1-D grid of 8 * ceil(nf / 8) * xt workgroups laid out so that the xt
+// tiles of a front get workgroup ids congruent mod 8; ids are dealt to the 8 XCDs round-robin,
+// so a front's tiles run on one XCD and the operands they share are fetched into one L2.
+__device__ __forceinline__ bool wg_tile(int xt, int nf, int& tile, int& f) {
+  if (xt == 0) {
+    tile = blockIdx.x;
+    f = blockIdx.y;
+    return true;
+  }
+  const int id = blockIdx.x, x = id & 7, slot = id >> 3, fl = slot / xt;
+  tile = slot - fl * xt;
+  f = fl * 8 + x;
+  return f < nf;
 }
 
 // ----------------------------------------------------------------------------- factorization
@@ -153,25 +202,29 @@ __device__ __forceinline__ int acc_col(int bj, int r) {
 //                 future diagonal tile it owns (rows = cols = its row range).  Right-looking
 //                 (K = 0): the trailing region was updated by k_gemm<0>.
 // A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
-// acc += P Q^T for 16 x 16 blocks held in LDS, one wave, v_mfma_f64_16x16x4 (K = 16):
+// acc += P Q^T for 16 x 16 blocks held in LDS, one wave, one MFMA per 4 k (K = 16):
 // P(i,k) = P[i*psi + k*psk], Q(j,k) = Q[j*qsj + k*qsk]; accumulator r of a lane is
-// C(lane%16, lane/16 + 4r).  PM / QM mark a triangular operand whose other triangle is not
+// C(lane%16, mrow(lane/16, r)).  PM / QM mark a triangular operand whose other triangle is not
 // stored as zeros: 1 = entries with k > row index read as zero, 2 = entries with k < row index.
-template <int PM = 0, int QM = 0>
-__device__ __forceinline__ void mma16(d4& acc, const double* P, int psi, int psk, const double* Q, int qsj, int qsk) {
+template <int PM = 0, int QM = 0, class T>
+__device__ __forceinline__ void mma16(V4<T>& acc, const T* P, int psi, int psk, const T* Q, int qsj, int qsk) {
   const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int k4 = 0; k4 < 4; k4++) {
     const int k = 4 * k4 + kq;
-    double pv = P[i * psi + k * psk], qv = Q[i * qsj + k * qsk];
-    if ((PM == 1 && k > i) || (PM == 2 && k < i)) pv = 0.0;
-    if ((QM == 1 && k > i) || (QM == 2 && k < i)) qv = 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qv, pv, acc, 0, 0, 0);
+    T pv = P[i * psi + k * psk], qv = Q[i * qsj + k * qsk];
+    if ((PM == 1 && k > i) || (PM == 2 && k < i)) pv = T(0);
+    if ((QM == 1 && k > i) || (QM == 2 && k < i)) qv = T(0);
+    acc = mfma4(qv, pv, acc);
   }
 }
 
-// 1/sqrt(d) from the hardware estimate and two Newton steps, y <- y (3/2 - d y^2 / 2), which
-// carry the estimate to full double precision.
+// 1/sqrt(d) from the hardware estimate and Newton steps, y <- y (3/2 - d y^2 / 2): two in
+// double, one in float, which carry the estimate to full precision of the type.
+__device__ __forceinline__ float rsqrt_nr(float d) {
+  const float y = __builtin_amdgcn_rsqf(d);
+  return fmaf(y, fmaf(-0.5f * d * y, y, 0.5f), y);
+}
 __device__ __forceinline__ double rsqrt_nr(double d) {
   double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
@@ -190,47 +243,48 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 //   1/sqrt(pivot j); panel L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one
 //   16-block per wave; then the off-diagonal blocks of the inverse,
 //   Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance i - j.
-// T[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
-// triangle: Linv(r, c) at U[r*LD + c], U = T + LD, for r >= c.
+// Ts[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
+// triangle: Linv(r, c) at U[r*LD + c], U = Ts + LD, for r >= c.
 #define LD (NB + 1)
+template <class T>
 __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   double* __restrict__ A, int jb, int* __restrict__ err) {
+                                                   T* __restrict__ A, int jb, int* __restrict__ err) {
   const int fid = ids[blockIdx.x];
   const FrontDev F = fr[fid];
   if (jb >= F.n) return;
   const int kb = min(NB, F.n - jb);
-  __shared__ double T[LD * LD];
-  __shared__ double rs[NB];
-  double* U = T + LD;
+  __shared__ T Ts[LD * LD];
+  __shared__ T rs[NB];
+  T* U = Ts + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  double* blk = A + F.off + (i64)jb * F.N + jb;
+  T* blk = A + F.off + (i64)jb * F.N + jb;
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e & 63, j = e >> 6;
-    double v = 0.0;  // strict upper part: the inverse, initially zero
-    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? 1.0 : 0.0);  // identity padding
-    T[j * LD + i] = v;
+    T v = T(0);  // strict upper part: the inverse, initially zero
+    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
+    Ts[j * LD + i] = v;
   }
-  if (t < NB) T[NB * LD + t] = 0.0;
+  if (t < NB) Ts[NB * LD + t] = T(0);
   __syncthreads();
   if (t < NB) U[t * LD + t] = 1.0;
   const int er = t & 15, ec = t >> 4;  // element of a 16 x 16 block
   for (int k = 0; k < 4; k++) {
     const int K0 = 16 * k;
-    double* Dk = T + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
-    double* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
+    T* Dk = Ts + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
+    T* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
     for (int j = 0; j < 16; j++) {
       __syncthreads();
-      const double d = Dk[j * LD + j];
-      const double ar = Dk[j * LD + er], ac = Dk[j * LD + ec], vj = Vk[j * LD + ec];
-      const double r = rsqrt_nr(d);
+      const T d = Dk[j * LD + j];
+      const T ar = Dk[j * LD + er], ac = Dk[j * LD + ec], vj = Vk[j * LD + ec];
+      const T r = rsqrt_nr(d);
       if (t == 0) {
         rs[K0 + j] = r;
-        if (K0 + j < kb && !(d > 0.0) && atomicCAS(err, 0, 1) == 0) {
+        if (K0 + j < kb && !(d > T(0)) && atomicCAS(err, 0, 1) == 0) {
           err[1] = fid;
           err[2] = jb + K0 + j;
         }
       }
-      const double lr = ar * r;
+      const T lr = ar * r;
       if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= lr * (ac * r);
       if (er > j && ec <= j) Vk[er * LD + ec] -= (lr * r) * vj;
     }
@@ -243,12 +297,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
     // (b) panel blocks below: L_ik = A_ik Linv_kk^T, wave w -> block i = k+1+w
     {
       const int i = k + 1 + w;
-      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
-      if (i < 4) mma16<0, 1>(acc, T + K0 * LD + 16 * i, 1, LD, Vk, LD, 1);
+      V4<T> acc = vzero<T>();
+      if (i < 4) mma16<0, 1>(acc, Ts + K0 * LD + 16 * i, 1, LD, Vk, LD, 1);
       __syncthreads();
       if (i < 4)
 #pragma unroll
-        for (int q = 0; q < 4; q++) T[(K0 + (lane >> 4) + 4 * q) * LD + 16 * i + (lane & 15)] = acc[q];
+        for (int q = 0; q < 4; q++) Ts[(K0 + mrow<T>(lane >> 4, q)) * LD + 16 * i + (lane & 15)] = acc[q];
     }
     __syncthreads();
     // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
@@ -260,13 +314,13 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
           cnt++;
         }
       if (bi < 0) continue;
-      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
-      mma16(acc, T + K0 * LD + 16 * bi, 1, LD, T + K0 * LD + 16 * bj, 1, LD);
+      V4<T> acc = vzero<T>();
+      mma16(acc, Ts + K0 * LD + 16 * bi, 1, LD, Ts + K0 * LD + 16 * bj, 1, LD);
       // a diagonal block keeps its strict upper triangle, which holds the inverse
 #pragma unroll
       for (int q = 0; q < 4; q++)
-        if (bi != bj || (lane & 15) >= (lane >> 4) + 4 * q)
-          T[(16 * bj + (lane >> 4) + 4 * q) * LD + 16 * bi + (lane & 15)] -= acc[q];
+        if (bi != bj || (lane & 15) >= mrow<T>(lane >> 4, q))
+          Ts[(16 * bj + mrow<T>(lane >> 4, q)) * LD + 16 * bi + (lane & 15)] -= acc[q];
     }
   }
   __syncthreads();
@@ -274,86 +328,89 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
   // (not yet computed) destination block Linv_ij.
   for (int dist = 1; dist < 4; dist++) {
     const int i = w + dist, j = w;
-    double* Sij = U + 16 * i * LD + 16 * j;  // S(i', j') at Sij[i'*LD + j']
-    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+    T* Sij = U + 16 * i * LD + 16 * j;  // S(i', j') at Sij[i'*LD + j']
+    V4<T> acc = vzero<T>();
     if (i < 4) {  // P(i', k) = L(16i+i', 16kk+k), Q(j', k) = Linv(16kk+k, 16j+j'), lower for kk = j
-      mma16<0, 2>(acc, T + 16 * j * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * j, 1, LD);
-      for (int kk = j + 1; kk < i; kk++) mma16(acc, T + 16 * kk * LD + 16 * i, 1, LD, U + 16 * kk * LD + 16 * j, 1, LD);
+      mma16<0, 2>(acc, Ts + 16 * j * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * j, 1, LD);
+      for (int kk = j + 1; kk < i; kk++) mma16(acc, Ts + 16 * kk * LD + 16 * i, 1, LD, U + 16 * kk * LD + 16 * j, 1, LD);
     }
     if (i < 4)
 #pragma unroll
-      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + (lane >> 4) + 4 * q] = acc[q];
+      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = acc[q];
     __syncthreads();
-    d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+    V4<T> a2 = vzero<T>();
     if (i < 4)  // P(i', k) = Linv_ii(i', k), Q(j', k) = S(k, j')
       mma16<1, 0>(a2, U + 16 * i * LD + 16 * i, LD, 1, Sij, 1, LD);
     __syncthreads();
     if (i < 4)
 #pragma unroll
-      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + (lane >> 4) + 4 * q] = -a2[q];
+      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
     __syncthreads();
   }
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e & 63, j = e >> 6;
-    if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = T[j * LD + i];
+    if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = Ts[j * LD + i];
   }
-  double* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+  T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
   for (int e = t; e < NB * NB; e += 256) {
     const int r = e & 63, c = e >> 6;
-    dinv[e] = r >= c ? U[r * LD + c] : 0.0;  // column-major, zero above the diagonal
+    dinv[e] = r >= c ? U[r * LD + c] : T(0);  // column-major, zero above the diagonal
   }
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    double* __restrict__ A, int jb, int left) {
-  const FrontDev F = fr[ids[blockIdx.y]];
+                                                    T* __restrict__ A, int jb, int left, int xt, int nf) {
+  int tile, fi;
+  if (!wg_tile(xt, nf, tile, fi)) return;
+  const FrontDev F = fr[ids[fi]];
   if (jb >= F.n) return;
   const int kb = min(NB, F.n - jb);
-  const int row0 = jb + kb + NB * blockIdx.x;
+  const int row0 = jb + kb + NB * tile;
   if (row0 >= F.N) return;
   const int nrow = min(NB, F.N - row0);
   __shared__ union {
-    TileSmem g;
+    TileSmem<T> g;
     struct {
-      double X[NB * (NB + 1)];
+      T X[NB * (NB + 1)];
     } c;
   } sm;
   const int t = threadIdx.x, lane = t & 63;
   const int wr = (t >> 6) >> 1, wc = (t >> 6) & 1;
-  d4 acc[2][2];
+  V4<T> acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
   const int K = left ? jb : 0;
   tile_gemm_nt(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
-  double* pan = A + F.off + (i64)jb * F.N + row0;
+  T* pan = A + F.off + (i64)jb * F.N + row0;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col(bj, r);
-        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : 0.0;
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : T(0);
       }
-  const double* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
+  const T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
   __syncthreads();
   // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k)
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) {
     const int k = k4 * 4 + (lane >> 4);
 #pragma unroll
     for (int bi = 0; bi < 2; bi++) {
-      const double bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+      const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
 #pragma unroll
       for (int bj = 0; bj < 2; bj++) {
-        const double av = dinv[k * NB + wc * 32 + bj * 16 + (lane & 15)];
-        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+        const T av = dinv[k * NB + wc * 32 + bj * 16 + (lane & 15)];
+        acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
       }
     }
   }
@@ -363,7 +420,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col(bj, r);
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
         if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
       }
   if (!left || row0 >= F.n) return;
@@ -375,36 +432,36 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col(bj, r);
-        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : 0.0;
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
       }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) {
     const int k = k4 * 4 + (lane >> 4);
 #pragma unroll
     for (int bi = 0; bi < 2; bi++) {
-      const double bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+      const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
 #pragma unroll
       for (int bj = 0; bj < 2; bj++) {
-        const double av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + (lane & 15)];
-        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+        const T av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + (lane & 15)];
+        acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
       }
     }
   }
   const int dn = min(NB, F.n - row0);
-  double* D = A + F.off + (i64)row0 * F.N + row0;
+  T* D = A + F.off + (i64)row0 * F.N + row0;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col(bj, r);
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
         if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] -= acc[bi][bj][r];
       }
 }
@@ -436,10 +493,12 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         Q = F[cols, k0:k0+kb], C = F[s:N, s:n] lower, s = k0+kb.
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-template <int MODE>
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                              double* __restrict__ A, int k0) {
-  const FrontDev F = fr[ids[blockIdx.y]];
+                                              T* __restrict__ A, int k0, int xt, int nf) {
+  int tile, fi;
+  if (!wg_tile(xt, nf, tile, fi)) return;
+  const FrontDev F = fr[ids[fi]];
   int s, K, kbase, rend, cend;
   if (MODE == 0) {
     if (k0 >= F.n) return;
@@ -452,25 +511,25 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
     s = F.n; K = F.n; kbase = 0; rend = F.N; cend = F.N;
   }
   int row0, col0;
-  if (!tile_of(blockIdx.x, s, rend, cend, row0, col0)) return;
-  __shared__ TileSmem sm;
-  d4 acc[2][2];
+  if (!tile_of(tile, s, rend, cend, row0, col0)) return;
+  __shared__ TileSmem<T> sm;
+  V4<T> acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
-  const double* base = A + F.off + (i64)kbase * F.N;
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  const T* base = A + F.off + (i64)kbase * F.N;
   tile_gemm_nt(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
-  double* C = A + F.off;
+  T* C = A + F.off;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = row0 + acc_row(bi), j = col0 + acc_col(bj, r);
+        const int i = row0 + acc_row(bi), j = col0 + acc_col<T>(bj, r);
         if (i >= rend || j >= cend || i < j) continue;
-        double* c = C + (i64)j * F.N + i;
+        T* c = C + (i64)j * F.N + i;
         if (MODE == 0) {
           *c -= acc[bi][bj][r];
         } else {
@@ -483,61 +542,61 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
               if (tj >= F.sep_pos[q] && tj < F.sep_pos[q] + F.sep_len[q]) rj = q;
             }
           }
-          const double sv = -acc[bi][bj][r];
+          const T sv = -acc[bi][bj][r];
           *c = (ri == rj) ? sv : (*c + sv);
         }
       }
 }
 
-template __global__ void k_gemm<0>(const int*, const FrontDev*, double*, int);
-template __global__ void k_gemm<1>(const int*, const FrontDev*, double*, int);
 
 // ----------------------------------------------------------------------------- solve
 // The sweeps work on W, an n_dof x 16 row-major right-hand-side block (W[dof*16 + r]), and on a
-// per-front scratch X (N x 16, row-major).  All block products run on v_mfma_f64_16x16x4 with a
+// per-front scratch X (N x 16, row-major).  All block products run on 16x16x4 MFMAs with a
 // 16-row block of L (or Linv) as the A operand and a 4 x 16 slice of X as B, so accumulator r
-// of a lane is row (lane/16) + 4r, right-hand side lane%16.
+// of a lane is row mrow(lane/16, r), right-hand side lane%16.
 #define RB 16
 #define GU 4  // 16-row groups in flight per wave
 
-__device__ __forceinline__ void gather_rhs(const FrontDev& F, double* __restrict__ X, const double* __restrict__ W,
+template <class T>
+__device__ __forceinline__ void gather_rhs(const FrontDev& F, T* __restrict__ X, const T* __restrict__ W,
                                            const int* __restrict__ di, const int* __restrict__ db) {
   for (int e = threadIdx.x; e < F.N * RB; e += 256) {
     const int i = e >> 4, r = e & 15;
     const int d = i < F.n ? di[i] : (db ? db[i - F.n] : -1);
-    X[e] = d >= 0 ? W[(i64)d * RB + r] : 0.0;
+    X[e] = d >= 0 ? W[(i64)d * RB + r] : T(0);
   }
 }
 
 // Forward sweep of one front: X[0:n] <- L11^{-1} W[int] block by block (x_j = Linv_j b_j, then
 // b_below -= L(below, j) x_j), X[n:N] <- -L21 X[0:n]; X[0:n] goes back to W[int].
+template <class T>
 __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   const double* __restrict__ A, const int* __restrict__ dofs,
-                                                   double* __restrict__ W, double* __restrict__ xs) {
+                                                   const T* __restrict__ A, const int* __restrict__ dofs,
+                                                   T* __restrict__ W, T* __restrict__ xs) {
   const FrontDev F = fr[ids[blockIdx.x]];
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
   gather_rhs(F, X, W, di, nullptr);
-  __shared__ double xb[NB][RB];
+  __shared__ T xb[NB][RB];
   for (int jb = 0; jb < F.n; jb += NB) {
     const int kb = min(NB, F.n - jb);
     __syncthreads();
     {  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
-      const double* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
-      d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+      const T* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+      V4<T> acc = vzero<T>();
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) {
         const int k = 4 * k4 + q;
-        const double av = Li[k * NB + 16 * w + c16];
-        const double bv = k < kb ? X[(i64)(jb + k) * RB + c16] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        const T av = Li[k * NB + 16 * w + c16];
+        const T bv = k < kb ? X[(i64)(jb + k) * RB + c16] : T(0);
+        acc = mfma4(av, bv, acc);
       }
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = 16 * w + q + 4 * r;
+        const int i = 16 * w + mrow<T>(q, r);
         xb[i][c16] = acc[r];
         if (i < kb) X[(i64)(jb + i) * RB + c16] = acc[r];
       }
@@ -546,25 +605,25 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
     // rows below, GU 16-row groups per wave in flight; out-of-range rows read row N-1 and are
     // never written
     for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * GU) {
-      double av[GU][NB / 4];
+      T av[GU][NB / 4];
 #pragma unroll
       for (int u = 0; u < GU; u++) {
         const int row = min(g0 + 64 * u + c16, F.N - 1);
 #pragma unroll
         for (int k4 = 0; k4 < NB / 4; k4++) {
           const int k = 4 * k4 + q;
-          av[u][k4] = k < kb ? L[(i64)(jb + k) * F.N + row] : 0.0;
+          av[u][k4] = k < kb ? L[(i64)(jb + k) * F.N + row] : T(0);
         }
       }
 #pragma unroll
       for (int u = 0; u < GU; u++) {
-        d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+        V4<T> acc = vzero<T>();
 #pragma unroll
         for (int k4 = 0; k4 < NB / 4; k4++)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], xb[4 * k4 + q][c16], acc, 0, 0, 0);
+          acc = mfma4(av[u][k4], xb[4 * k4 + q][c16], acc);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = g0 + 64 * u + q + 4 * r;
+          const int i = g0 + 64 * u + mrow<T>(q, r);
           if (i < F.N) X[(i64)i * RB + c16] -= acc[r];
         }
       }
@@ -576,17 +635,18 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
 
 // Separator accumulation of per-box contributions X[n+pos : n+pos+k] of both neighbour boxes
 // into the work block, box 0 first: W = (W + X_b0) + X_b1.
+template <class T>
 __global__ __launch_bounds__(256) void k_sep_acc(const int* __restrict__ ids, const SepDev* __restrict__ seps,
                                                  const FrontDev* __restrict__ fr, const int* __restrict__ dofs,
-                                                 double* __restrict__ W, const double* __restrict__ xs) {
+                                                 T* __restrict__ W, const T* __restrict__ xs) {
   const SepDev S = seps[ids[blockIdx.x]];
   const FrontDev B0 = fr[S.box[0]], B1 = fr[S.box[1]];
   const int* d = dofs + S.dof_off;
-  const double* x0 = xs + B0.x_off + (i64)(B0.n + S.pos[0]) * RB;
-  const double* x1 = xs + B1.x_off + (i64)(B1.n + S.pos[1]) * RB;
+  const T* x0 = xs + B0.x_off + (i64)(B0.n + S.pos[0]) * RB;
+  const T* x1 = xs + B1.x_off + (i64)(B1.n + S.pos[1]) * RB;
   for (int e = threadIdx.x; e < S.k * RB; e += 256) {
-    double* wp = W + (i64)d[e >> 4] * RB + (e & 15);
-    const double v = *wp + x0[e];
+    T* wp = W + (i64)d[e >> 4] * RB + (e & 15);
+    const T v = *wp + x0[e];
     *wp = v + x1[e];
   }
 }
@@ -594,17 +654,18 @@ __global__ __launch_bounds__(256) void k_sep_acc(const int* __restrict__ ids, co
 // Backward sweep of one front: X[0:n] = W[int] (forward result y), X[n:N] = W[bd];
 // block by block from the last: z_j = y_j - L(below, j)^T X(below), x_j = Linv_j^T z_j
 // (= A22^{-1} b_rd - XR b_sk overall); X[0:n] goes back to W[int].
+template <class T>
 __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   const double* __restrict__ A, const int* __restrict__ dofs,
-                                                   double* __restrict__ W, double* __restrict__ xs) {
+                                                   const T* __restrict__ A, const int* __restrict__ dofs,
+                                                   T* __restrict__ W, T* __restrict__ xs) {
   const FrontDev F = fr[ids[blockIdx.x]];
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
   gather_rhs(F, X, W, di, db);
-  __shared__ double zb[NB][RB];
+  __shared__ T zb[NB][RB];
   const int nblk = (F.n + NB - 1) / NB;
   for (int bk = nblk - 1; bk >= 0; bk--) {
     const int jb = bk * NB, kb = min(NB, F.n - jb);
@@ -612,11 +673,11 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
     // T(c, r) = sum_{i >= jb+kb} L(i, jb+c) X(i, r); wave w owns columns c in [16w, 16w+16),
     // streaming 4-row slices straight from the column-major L (lanes: column c16, row q)
     const int col = 16 * w + c16;
-    const double* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
+    const T* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
     const bool cin = col < kb;
-    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+    V4<T> acc = vzero<T>();
     for (int i0 = jb + kb; i0 < F.N; i0 += 16 * GU) {
-      double av[GU][4], bv[GU][4];
+      T av[GU][4], bv[GU][4];
 #pragma unroll
       for (int u = 0; u < GU; u++)
 #pragma unroll
@@ -624,31 +685,31 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
           const int i = i0 + 16 * u + 4 * k4 + q;
           const bool ok = cin && i < F.N;
           const int ic = min(i, F.N - 1);
-          av[u][k4] = ok ? Lc[ic] : 0.0;
+          av[u][k4] = ok ? Lc[ic] : T(0);
           bv[u][k4] = X[(i64)ic * RB + c16];
         }
 #pragma unroll
       for (int u = 0; u < GU; u++)
 #pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], bv[u][k4], acc, 0, 0, 0);
+        for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int c = 16 * w + q + 4 * r;
-      zb[c][c16] = c < kb ? X[(i64)(jb + c) * RB + c16] - acc[r] : 0.0;
+      const int c = 16 * w + mrow<T>(q, r);
+      zb[c][c16] = c < kb ? X[(i64)(jb + c) * RB + c16] - acc[r] : T(0);
     }
     __syncthreads();
     {  // x_j = Linv_j^T z: x(i, r) = sum_c Linv(c, i) z(c, r); wave w -> rows 16w..16w+15
-      const double* Li = A + F.dinv_off + (i64)bk * NB * NB;
-      d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+      const T* Li = A + F.dinv_off + (i64)bk * NB * NB;
+      V4<T> a2 = vzero<T>();
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) {
         const int c = 4 * k4 + q;
-        a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[(16 * w + c16) * NB + c], zb[c][c16], a2, 0, 0, 0);
+        a2 = mfma4(Li[(16 * w + c16) * NB + c], zb[c][c16], a2);
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = 16 * w + q + 4 * r;
+        const int i = 16 * w + mrow<T>(q, r);
         if (i < kb) X[(i64)(jb + i) * RB + c16] = a2[r];
       }
     }
@@ -690,9 +751,10 @@ __device__ __forceinline__ void post_block(int* flag, int epoch) {
 // b waits for the blocks j < b, forms v = b_b - sum_j L_bj x_j and publishes x_b = Linv_b v
 // (to X and to W[int]); boundary block b accumulates over every interior block and leaves
 // X[rows] = -sum_j L_bj x_j, the contribution gathered by k_sep_acc.
+template <class T>
 __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                 const double* __restrict__ A, const int* __restrict__ dofs,
-                                                 double* __restrict__ W, double* __restrict__ xs, int* flags, int epoch,
+                                                 const T* __restrict__ A, const int* __restrict__ dofs,
+                                                 T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
                                                  int* err) {
   const FrontDev F = fr[ids[blockIdx.y]];
   const int nbi = (F.n + NB - 1) / NB, nbt = nbi + (F.m + NB - 1) / NB;
@@ -701,55 +763,55 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
   const bool interior = b < nbi;
   const int r0 = interior ? NB * b : F.n + NB * (b - nbi);
   const int rend = min(interior ? F.n : F.N, r0 + NB);
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   int* fl = flags + F.flag_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
-  __shared__ double vb[NB][RB];
+  __shared__ T vb[NB][RB];
   const int row = r0 + 16 * w + c16;  // L row supplied by this lane
   const bool rok = row < rend;
   const int rowc = rok ? row : r0;
-  d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+  V4<T> acc = vzero<T>();
   const int jmax = interior ? b : nbi;
   for (int j = 0; j < jmax; j++) {
     wait_block(fl + j, epoch, err);
     const int jb = NB * j, kb = min(NB, F.n - jb);
-    double av[NB / 4], bv[NB / 4];
+    T av[NB / 4], bv[NB / 4];
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
       const bool kin = k < kb;
-      av[k4] = (rok && kin) ? L[(i64)(jb + k) * F.N + rowc] : 0.0;
-      bv[k4] = kin ? X[(i64)(jb + k) * RB + c16] : 0.0;
+      av[k4] = (rok && kin) ? L[(i64)(jb + k) * F.N + rowc] : T(0);
+      bv[k4] = kin ? X[(i64)(jb + k) * RB + c16] : T(0);
     }
 #pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], bv[k4], acc, 0, 0, 0);
+    for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
   }
   if (!interior) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int i = r0 + 16 * w + q + 4 * r;
+      const int i = r0 + 16 * w + mrow<T>(q, r);
       if (i < rend) X[(i64)i * RB + c16] = -acc[r];
     }
     return;
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int i = 16 * w + q + 4 * r;
-    vb[i][c16] = r0 + i < rend ? W[(i64)di[r0 + i] * RB + c16] - acc[r] : 0.0;
+    const int i = 16 * w + mrow<T>(q, r);
+    vb[i][c16] = r0 + i < rend ? W[(i64)di[r0 + i] * RB + c16] - acc[r] : T(0);
   }
   __syncthreads();
-  const double* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
-  d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+  const T* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
+  V4<T> a2 = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) {
     const int c = 4 * k4 + q;
-    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[c * NB + 16 * w + c16], vb[c][c16], a2, 0, 0, 0);
+    a2 = mfma4(Li[c * NB + 16 * w + c16], vb[c][c16], a2);
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int i = r0 + 16 * w + q + 4 * r;
+    const int i = r0 + 16 * w + mrow<T>(q, r);
     if (i < rend) {
       X[(i64)i * RB + c16] = a2[r];
       W[(i64)di[i] * RB + c16] = a2[r];
@@ -761,72 +823,73 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
 // Backward sweep of one front by blocks, last block first (workgroup b owns block
 // nbi-1-b): z = y_i - L(rows below, i)^T x(rows below), boundary rows (read from W[bd]) first,
 // then the interior blocks below as they are published; x_i = Linv_i^T z goes to X and W[int].
+template <class T>
 __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                 const double* __restrict__ A, const int* __restrict__ dofs,
-                                                 double* __restrict__ W, double* __restrict__ xs, int* flags, int epoch,
+                                                 const T* __restrict__ A, const int* __restrict__ dofs,
+                                                 T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
                                                  int* err) {
   const FrontDev F = fr[ids[blockIdx.y]];
   const int nbi = (F.n + NB - 1) / NB;
   if ((int)blockIdx.x >= nbi) return;
   const int ib = nbi - 1 - blockIdx.x;
   const int jb = NB * ib, kb = min(NB, F.n - jb);
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   int* fl = flags + F.flag_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
-  __shared__ double zb[NB][RB];
+  __shared__ T zb[NB][RB];
   const int col = 16 * w + c16;  // column of the block supplied by this lane
   const bool cin = col < kb;
-  const double* Lc = L + (i64)(jb + (cin ? col : 0)) * F.N;
-  d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+  const T* Lc = L + (i64)(jb + (cin ? col : 0)) * F.N;
+  V4<T> acc = vzero<T>();
   for (int i0 = F.n; i0 < F.N; i0 += 16 * GU) {  // boundary rows: x = W[bd]
-    double av[GU][4], bv[GU][4];
+    T av[GU][4], bv[GU][4];
 #pragma unroll
     for (int u = 0; u < GU; u++)
 #pragma unroll
       for (int k4 = 0; k4 < 4; k4++) {
         const int i = i0 + 16 * u + 4 * k4 + q;
         const bool ok = i < F.N;
-        av[u][k4] = (ok && cin) ? Lc[i] : 0.0;
-        bv[u][k4] = ok ? W[(i64)db[i - F.n] * RB + c16] : 0.0;
+        av[u][k4] = (ok && cin) ? Lc[i] : T(0);
+        bv[u][k4] = ok ? W[(i64)db[i - F.n] * RB + c16] : T(0);
       }
 #pragma unroll
     for (int u = 0; u < GU; u++)
 #pragma unroll
-      for (int k4 = 0; k4 < 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][k4], bv[u][k4], acc, 0, 0, 0);
+      for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
   }
   for (int kbk = nbi - 1; kbk > ib; kbk--) {  // interior blocks below, in publication order
     wait_block(fl + kbk, epoch, err);
     const int k0 = NB * kbk, kn = min(NB, F.n - k0);
-    double av[NB / 4], bv[NB / 4];
+    T av[NB / 4], bv[NB / 4];
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
       const bool kin = k < kn;
-      av[k4] = (kin && cin) ? Lc[k0 + k] : 0.0;
-      bv[k4] = kin ? X[(i64)(k0 + k) * RB + c16] : 0.0;
+      av[k4] = (kin && cin) ? Lc[k0 + k] : T(0);
+      bv[k4] = kin ? X[(i64)(k0 + k) * RB + c16] : T(0);
     }
 #pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], bv[k4], acc, 0, 0, 0);
+    for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int c = 16 * w + q + 4 * r;
-    zb[c][c16] = c < kb ? W[(i64)di[jb + c] * RB + c16] - acc[r] : 0.0;
+    const int c = 16 * w + mrow<T>(q, r);
+    zb[c][c16] = c < kb ? W[(i64)di[jb + c] * RB + c16] - acc[r] : T(0);
   }
   __syncthreads();
-  const double* Li = A + F.dinv_off + (i64)ib * NB * NB;
-  d4 a2 = (d4){0.0, 0.0, 0.0, 0.0};
+  const T* Li = A + F.dinv_off + (i64)ib * NB * NB;
+  V4<T> a2 = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) {
     const int c = 4 * k4 + q;
-    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[(16 * w + c16) * NB + c], zb[c][c16], a2, 0, 0, 0);
+    a2 = mfma4(Li[(16 * w + c16) * NB + c], zb[c][c16], a2);
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int i = 16 * w + q + 4 * r;
+    const int i = 16 * w + mrow<T>(q, r);
     if (i < kb) {
       X[(i64)(jb + i) * RB + c16] = a2[r];
       W[(i64)di[jb + i] * RB + c16] = a2[r];
@@ -836,40 +899,49 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 }
 
 // Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.
+template <class T>
 __global__ __launch_bounds__(256) void k_to_rows(int n, const double* __restrict__ B, int ld, int nr,
-                                                 double* __restrict__ W) {
+                                                 T* __restrict__ W) {
   const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
   if (e >= (i64)n * RB) return;
   const int i = (int)(e >> 4), r = (int)(e & 15);
-  W[e] = r < nr ? B[(i64)r * ld + i] : 0.0;
+  W[e] = r < nr ? (T)B[(i64)r * ld + i] : T(0);
 }
-__global__ __launch_bounds__(256) void k_from_rows(int n, const double* __restrict__ W, double* __restrict__ B,
+template <class T>
+__global__ __launch_bounds__(256) void k_from_rows(int n, const T* __restrict__ W, double* __restrict__ B,
                                                    int ld, int nr) {
   const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
   if (e >= (i64)n * nr) return;
   const int r = (int)(e / n), i = (int)(e % n);
-  B[(i64)r * ld + i] = W[(i64)i * RB + r];
+  B[(i64)r * ld + i] = (double)W[(i64)i * RB + r];
+}
+// Element-wise conversion between the FP64 and the storage-type work blocks.
+template <class To, class From>
+__global__ __launch_bounds__(256) void k_cvt(i64 n, const From* __restrict__ a, To* __restrict__ b) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) b[e] = (To)a[e];
 }
 
 // ----------------------------------------------------------------------------- apply
 // Upward step of Apply for one front: u = [L11; L21]^T [x_rd; x_sk] (= L11^T b_rd + W b_sk),
 // b_rd <- L11 u.  u is kept in X[0:n] for the downward step.
+template <class T>
 __global__ __launch_bounds__(256) void k_apply_up(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                  const double* __restrict__ A, const int* __restrict__ dofs,
-                                                  double* __restrict__ W, double* __restrict__ xs, int nr) {
+                                                  const T* __restrict__ A, const int* __restrict__ dofs,
+                                                  T* __restrict__ W, T* __restrict__ xs, int nr) {
   const FrontDev F = fr[ids[blockIdx.x]];
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   const int t = threadIdx.x;
   gather_rhs(F, X, W, di, db);
   __syncthreads();
   // u_c = sum_{i >= c} L[i][c] X[i]: one column at a time, block reduction
-  __shared__ double red[256];
+  __shared__ T red[256];
   for (int c = 0; c < F.n; c++) {
     for (int r = 0; r < nr; r++) {
-      double s = 0.0;
+      T s = T(0);
       for (int i = c + t; i < F.N; i += 256) s += L[(i64)c * F.N + i] * X[(i64)i * RB + r];
       red[t] = s;
       __syncthreads();
@@ -884,21 +956,22 @@ __global__ __launch_bounds__(256) void k_apply_up(const int* __restrict__ ids, c
   // b_rd = L11 u
   for (int i = t; i < F.n; i += 256)
     for (int r = 0; r < nr; r++) {
-      double s = 0.0;
+      T s = T(0);
       for (int c = 0; c <= i; c++) s += L[(i64)c * F.N + i] * X[(i64)c * RB + r];
       W[(i64)di[i] * RB + r] = s;
     }
 }
 
 // Downward step of Apply: X[n:N] = L21 u (the reference's XL b_rd), accumulated by k_sep_acc.
+template <class T>
 __global__ __launch_bounds__(256) void k_apply_down(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    const double* __restrict__ A, double* __restrict__ xs, int nr) {
+                                                    const T* __restrict__ A, T* __restrict__ xs, int nr) {
   const FrontDev F = fr[ids[blockIdx.x]];
-  const double* L = A + F.off;
-  double* X = xs + F.x_off;
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
   for (int i = F.n + threadIdx.x; i < F.N; i += 256)
     for (int r = 0; r < RB; r++) {
-      double s = 0.0;
+      T s = T(0);
       if (r < nr)
         for (int c = 0; c < F.n; c++) s += L[(i64)c * F.N + i] * X[(i64)c * RB + r];
       X[(i64)i * RB + r] = s;
@@ -917,6 +990,102 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
     for (int k = k0; k < k1; k++) s += v[k] * x[(i64)r * ldx + ci[k]];
     y[(i64)r * ldy + i] = s;
   }
+}
+
+// ----------------------------------------------------------------------------- refinement
+// FP64 kernels of the preconditioned conjugate-gradient refinement that turns the FP32 factor
+// into an FP64-accurate solve.  Vectors are row-major n x 16 blocks (v[i*16 + r]).
+// y = A x, 16 lanes per row (one right-hand side each): the 16 x reads of a nonzero are one
+// contiguous 128-byte segment.
+__global__ __launch_bounds__(256) void k_spmv_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const double* __restrict__ v, const double* __restrict__ x,
+                                                   double* __restrict__ y) {
+  const int i = blockIdx.x * 16 + (threadIdx.x >> 4), r = threadIdx.x & 15;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * x[(i64)ci[k] * RB + r];
+  y[(i64)i * RB + r] = s;
+}
+
+// Per-column dot products of one or two pairs, first pass: block g sums rows
+// [g*chunk, (g+1)*chunk) in a fixed order into part[g*32 + 16*pair + r].
+__global__ __launch_bounds__(256) void k_dots(int n, int npair, const double* __restrict__ a0,
+                                              const double* __restrict__ b0, const double* __restrict__ a1,
+                                              const double* __restrict__ b1, double* __restrict__ part) {
+  const int r = threadIdx.x & 15, rl = threadIdx.x >> 4;
+  const int chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = i0 + rl; i < i1; i += 16) {
+    const i64 e = (i64)i * RB + r;
+    s0 += a0[e] * b0[e];
+    if (npair > 1) s1 += a1[e] * b1[e];
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int pr = threadIdx.x >> 4, c = threadIdx.x & 15;
+    double t = 0.0;
+    for (int q = 0; q < 16; q++) t += red[pr][q * 16 + c];
+    part[(i64)blockIdx.x * 32 + threadIdx.x] = t;
+  }
+}
+// Sum of squares of a plain vector, first pass (part[g], fixed order).
+__global__ __launch_bounds__(256) void k_sumsq(i64 n, const double* __restrict__ v, double* __restrict__ part) {
+  const i64 chunk = (n + gridDim.x - 1) / gridDim.x;
+  const i64 i0 = (i64)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double s = 0.0;
+  for (i64 i = i0 + threadIdx.x; i < i1; i += 256) s += v[i] * v[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x * 32] = red[0];
+}
+// Second pass: out[c] = sum_g part[g*32 + c], c < 32, in a fixed order (8 interleaved partial
+// sums per column, then combined).
+__global__ __launch_bounds__(256) void k_dots_final(int nblk, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+  const int c = threadIdx.x & 31, s8 = threadIdx.x >> 5;
+  double s = 0.0;
+  for (int g = s8; g < nblk; g += 8) s += part[(i64)g * 32 + c];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    double t = 0.0;
+    for (int q = 0; q < 8; q++) t += red[q * 32 + c];
+    out[c] = t;
+  }
+}
+// x += a p, r -= a q with a = rz / pq per column; columns outside `active` (bit mask) frozen.
+__global__ __launch_bounds__(256) void k_cg_xr(i64 n16, const double* __restrict__ rz, const double* __restrict__ pq,
+                                               int active, const double* __restrict__ p,
+                                               const double* __restrict__ q, double* __restrict__ x,
+                                               double* __restrict__ r) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n16) return;
+  const int c = (int)(e & 15);
+  if (!((active >> c) & 1)) return;
+  const double a = pq[c] != 0.0 ? rz[c] / pq[c] : 0.0;
+  x[e] += a * p[e];
+  r[e] -= a * q[e];
+}
+// p = z + b p with b = rz_new / rz per column.
+__global__ __launch_bounds__(256) void k_cg_p(i64 n16, const double* __restrict__ rz_new,
+                                              const double* __restrict__ rz, int active,
+                                              const double* __restrict__ z, double* __restrict__ p) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n16) return;
+  const int c = (int)(e & 15);
+  if (!((active >> c) & 1)) return;
+  const double b = rz[c] != 0.0 ? rz_new[c] / rz[c] : 0.0;
+  p[e] = z[e] + b * p[e];
 }
 
 }  // namespace umf

@@ -96,6 +96,18 @@ struct umf_plan {
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
   double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
   double* d_w = nullptr;   // solve work block n x RB, row-major
+  // FP32 form: its own front pool and solve buffers, the FP64 refinement vectors
+  int prec = UMF_PREC_FP64;       // precision of the next factorization
+  int fact_prec = UMF_PREC_FP64;  // precision the current factors are held in
+  double ir_tol = 1e-10;
+  int ir_max = 50;
+  float* arena32 = nullptr;
+  float* d_xs32 = nullptr;
+  float* d_w32 = nullptr;
+  double* d_cg = nullptr;    // 5 row-major n x RB blocks: B, X, R, P, Q/Z
+  double* d_part = nullptr;  // reduction partials (DOT_BLOCKS x 32)
+  double* d_dots = nullptr;  // 4 x 32 reduction results
+  double norm_a = 0;         // ||A||_F of the current operator
   double* d_stage = nullptr;  // device copy of host right-hand sides
   int stage_cols = 0;
   i64 xs_elems = 0;
@@ -236,8 +248,9 @@ inline int tiles_of(int s, int rend, int cend) {
   return T * (T + 1) / 2 + (RT - T) * T;
 }
 
-// Run the factorization over the dirty schedule on the plan stream.
-int run_factor(umf_plan* p, const Plan::Dirty& D) {
+// Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
+template <class T>
+int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
   const Plan& P = p->P;
   const Grid& g = P.g;
   Sched S = make_sched(P, D);
@@ -275,10 +288,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
     if (L.sm_cnt > 0) {
       double zb, ab;
       asm_work(L.sm_off, L.sm_cnt, zb, ab);
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower, dim3(std::min(std::max(L.max_k, 1), 256), L.sm_cnt),
-             ids + L.sm_off, (const MatDev*)p->d_mats, p->arena);
-      launch(p, "k_assemble", 0, ab, k_assemble, dim3(L.s_grid, L.sm_cnt), ids + L.sm_off, (const MatDev*)p->d_mats,
-             p->arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(std::max(L.max_k, 1), 256), L.sm_cnt),
+             ids + L.sm_off, (const MatDev*)p->d_mats, arena);
+      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.s_grid, L.sm_cnt), ids + L.sm_off, (const MatDev*)p->d_mats,
+             arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
              (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
     if (L.f_cnt == 0) continue;
@@ -286,10 +299,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
     {
       double zb, ab;
       asm_work(L.f_off, L.f_cnt, zb, ab);
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
-             (const MatDev*)p->d_mats, p->arena);
-      launch(p, "k_assemble", 0, ab, k_assemble, dim3(L.f_grid, L.f_cnt), ids + L.f_off, (const MatDev*)p->d_mats,
-             p->arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
+             (const MatDev*)p->d_mats, arena);
+      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off, (const MatDev*)p->d_mats,
+             arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
              (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
     // 3. blocked partial Cholesky of the pivot columns, 64 at a time.  Levels with many fronts
@@ -297,6 +310,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
     //    columns, then factored; no trailing read-modify-write); levels with few large fronts go
     //    right-looking (factor the block, then one wide trailing update) for parallelism.
     const bool left = L.f_cnt >= 32;
+    // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
+    // diagonal-block columns); levels of few fronts spread each front over the whole chip
+    const bool xcd = L.f_cnt >= 64;
+    auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((L.f_cnt + 7) / 8) * per_front); };
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       int rows = 0, tiles = 0;
       bool partial = false;
@@ -320,14 +337,16 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
           bgem += 16.0 * elems + 8.0 * kb * (nr + nc);
         }
       }
-      launch(p, "k_chol_diag", fdiag, 0, k_chol_diag, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
-             p->arena, k0, p->d_err);
-      if (rows > 0)
-        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel, dim3((rows + NB - 1) / NB, L.f_cnt), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, p->arena, k0, left ? 1 : 0);
+      launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+             arena, k0, p->d_err);
+      if (rows > 0) {
+        const int pt = (rows + NB - 1) / NB;
+        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt);
+      }
       if (!left && tiles > 0)
-        launch(p, "k_gemm_panel", fgem, bgem, k_gemm<0>, dim3(tiles, L.f_cnt), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, p->arena, k0);
+        launch(p, "k_gemm_panel", fgem, bgem, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? tiles : 0, L.f_cnt);
     }
     // 4. Schur complement into the trailing block
     if (L.maxm > 0) {
@@ -340,20 +359,45 @@ int run_factor(umf_plan* p, const Plan::Dirty& D) {
         by += 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
       }
       if (tiles > 0)
-        launch(p, "k_gemm_schur", fl, by, k_gemm<1>, dim3(tiles, L.f_cnt), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, p->arena, 0);
+        launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, xcd ? tiles : 0, L.f_cnt);
     }
     // 5. separators of this level receive both neighbours' Schur blocks
     if (L.s_cnt > 0) {
       double by = 0;
       for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
-      launch(p, "k_sep_add", 0, by, k_sep_add, dim3(std::min(std::max(L.max_k, 1), 256), L.s_cnt), ids + L.s_off,
-             (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, p->arena);
+      launch(p, "k_sep_add", 0, by, k_sep_add<T>, dim3(std::min(std::max(L.max_k, 1), 256), L.s_cnt), ids + L.s_off,
+             (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
     }
   }
   (void)NF;
   g_level = -1;
   HIPCHK(hipGetLastError());
+  return UMF_OK;
+}
+
+// Factor over a dirty schedule in the precision the plan is set to (the FP32 front pool is
+// allocated on first use).
+int factor_any(umf_plan* p, const Plan::Dirty& D) {
+  if (p->prec == UMF_PREC_FP32) {
+    if (!p->arena32)
+      HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
+    return run_factor<float>(p, D, p->arena32);
+  }
+  return run_factor<double>(p, D, p->arena);
+}
+
+// Copy cnt front-pool entries at offset off of the current factors to the host as double.
+int read_pool(umf_plan* p, i64 off, size_t cnt, std::vector<double>& h) {
+  h.resize(cnt);
+  if (!cnt) return UMF_OK;
+  if (p->fact_prec == UMF_PREC_FP64) {
+    HIPCHK(hipMemcpy(h.data(), p->arena + off, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    return UMF_OK;
+  }
+  std::vector<float> f(cnt);
+  HIPCHK(hipMemcpy(f.data(), p->arena32 + off, sizeof(float) * cnt, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < cnt; i++) h[i] = f[i];
   return UMF_OK;
 }
 
@@ -372,10 +416,20 @@ int check_err(umf_plan* p) {
   return UMF_OK;
 }
 
+#define DOT_BLOCKS 512
+
 int ensure_solve_scratch(umf_plan* p) {
-  if (p->d_xs) return UMF_OK;
-  HIPCHK(hipMalloc((void**)&p->d_xs, sizeof(double) * std::max<i64>(p->xs_elems, 1)));
-  HIPCHK(hipMalloc((void**)&p->d_w, sizeof(double) * (size_t)p->P.n * RB));
+  const size_t nw = (size_t)p->P.n * RB, nx = (size_t)std::max<i64>(p->xs_elems, 1);
+  if (p->fact_prec == UMF_PREC_FP64) {
+    if (!p->d_xs) HIPCHK(hipMalloc((void**)&p->d_xs, sizeof(double) * nx));
+    if (!p->d_w) HIPCHK(hipMalloc((void**)&p->d_w, sizeof(double) * nw));
+    return UMF_OK;
+  }
+  if (!p->d_xs32) HIPCHK(hipMalloc((void**)&p->d_xs32, sizeof(float) * nx));
+  if (!p->d_w32) HIPCHK(hipMalloc((void**)&p->d_w32, sizeof(float) * nw));
+  if (!p->d_cg) HIPCHK(hipMalloc((void**)&p->d_cg, sizeof(double) * 5 * nw));
+  if (!p->d_part) HIPCHK(hipMalloc((void**)&p->d_part, sizeof(double) * DOT_BLOCKS * 32));
+  if (!p->d_dots) HIPCHK(hipMalloc((void**)&p->d_dots, sizeof(double) * 5 * 32));
   return UMF_OK;
 }
 
@@ -406,8 +460,10 @@ int next_epoch(umf_plan* p) {
   return p->epoch;
 }
 
-// Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the work vector d_w (ld = n).
-int sweep(umf_plan* p, int mode, int nr) {
+// Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the row-major work block w
+// (n x RB), fronts of type T in `arena`, per-front scratch xs.
+template <class T>
+int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
   const Plan& P = p->P;
   int L = P.g.L;
   AllLists A = all_lists(P);
@@ -445,44 +501,44 @@ int sweep(umf_plan* p, int mode, int nr) {
     work(ell, fl, by);
     g_level = ell;
     if (!blocked(ell)) {
-      launch(p, "k_solve_fwd", fl, by, k_solve_fwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
+      launch(p, "k_solve_fwd", fl, by, k_solve_fwd<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const T*)arena, (const int*)p->d_dofs, w, xs);
     } else {
       int nbi, nbt;
       blocks(ell, nbi, nbt);
-      launch(p, "k_fwd_blk", fl, by, k_fwd_blk, dim3(nbt, A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs, p->d_flags, next_epoch(p), p->d_err);
+      launch(p, "k_fwd_blk", fl, by, k_fwd_blk<T>, dim3(nbt, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
     }
   };
   if (mode == 0) {
     for (int ell = L - 1; ell >= 1; ell--) {
       fwd(ell);
-      launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
-             (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
+      launch(p, "k_sep_acc", 0, 0, k_sep_acc<T>, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
+             (const int*)p->d_dofs, w, (const T*)xs);
     }
     fwd(0);
     for (int ell = 0; ell < L; ell++) {
       work(ell, fl, by);
       g_level = ell;
       if (!blocked(ell)) {
-        launch(p, "k_solve_bwd", fl, by, k_solve_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-               (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs);
+        launch(p, "k_solve_bwd", fl, by, k_solve_bwd<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+               (const T*)arena, (const int*)p->d_dofs, w, xs);
       } else {
         int nbi, nbt;
         blocks(ell, nbi, nbt);
-        launch(p, "k_bwd_blk", fl, by, k_bwd_blk, dim3(nbi, A.f_cnt[ell]), ids + A.f_off[ell], fr,
-               (const double*)p->arena, (const int*)p->d_dofs, p->d_w, p->d_xs, p->d_flags, next_epoch(p), p->d_err);
+        launch(p, "k_bwd_blk", fl, by, k_bwd_blk<T>, dim3(nbi, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+               (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
       }
     }
   } else {
     for (int ell = L - 1; ell >= 0; ell--)
-      launch(p, "k_apply_up", 0, 0, k_apply_up, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const double*)p->arena,
-             (const int*)p->d_dofs, p->d_w, p->d_xs, nr);
+      launch(p, "k_apply_up", 0, 0, k_apply_up<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const T*)arena,
+             (const int*)p->d_dofs, w, xs, nr);
     for (int ell = 1; ell < L; ell++) {
-      launch(p, "k_apply_down", 0, 0, k_apply_down, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const double*)p->arena, p->d_xs, nr);
-      launch(p, "k_sep_acc", 0, 0, k_sep_acc, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
-             (const int*)p->d_dofs, p->d_w, (const double*)p->d_xs);
+      launch(p, "k_apply_down", 0, 0, k_apply_down<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+             (const T*)arena, xs, nr);
+      launch(p, "k_sep_acc", 0, 0, k_sep_acc<T>, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
+             (const int*)p->d_dofs, w, (const T*)xs);
     }
   }
   g_level = -1;
@@ -490,7 +546,98 @@ int sweep(umf_plan* p, int mode, int nr) {
   return UMF_OK;
 }
 
-// Shared body of the solve/apply entry points: columns in chunks of RB through d_w.
+// Per-column dot products of one or two pairs of row-major n x RB blocks into out[0:32]
+// (pair 0 in out[0:16], pair 1 in out[16:32]); deterministic two-pass reduction.
+void dots(umf_plan* p, const double* a0, const double* b0, const double* a1, const double* b1, double* out) {
+  const int n = p->P.n;
+  launch(p, "k_dots", 0, 16.0 * RB * n * (a1 ? 2 : 1), k_dots, dim3(DOT_BLOCKS), n, a1 ? 2 : 1, a0, b0,
+         a1 ? a1 : a0, b1 ? b1 : b0, p->d_part);
+  launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, out);
+}
+
+// FP32 form: z = M^{-1} r with M the FP32 multifrontal factorization (r, z FP64 row-major).
+int precondition(umf_plan* p, const double* r, double* z) {
+  const i64 n16 = (i64)p->P.n * RB;
+  const unsigned g = (unsigned)((n16 + 255) / 256);
+  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<float, double>, dim3(g), n16, r, p->d_w32);
+  int rc = sweep<float>(p, 0, RB, p->arena32, p->d_w32, p->d_xs32);
+  if (rc) return rc;
+  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<double, float>, dim3(g), n16, (const float*)p->d_w32, z);
+  return UMF_OK;
+}
+
+// FP32 form of Solve for nr <= RB columns: preconditioned conjugate gradients in FP64 (FP64
+// operator and residuals, FP32 factor as preconditioner), x0 = 0, each column stopped once its
+// normwise backward error ||r|| / (||A||_F ||x|| + ||b||) <= ir_tol.  Returns the iteration count
+// and the largest final backward error through it / berr.
+int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out, double& berr_out) {
+  const int n = p->P.n;
+  const i64 n16 = (i64)n * RB;
+  const unsigned g16 = (unsigned)((n16 + 255) / 256), gsp = (unsigned)((n + 15) / 16);
+  double* Bv = p->d_cg;
+  double* X = Bv + n16;
+  double* R = X + n16;
+  double* Pd = R + n16;
+  double* QZ = Pd + n16;
+  double* S1 = p->d_dots + 32;  // p.q
+  double* S2 = p->d_dots + 64;  // r.r | x.x
+  double* rz = p->d_dots;       // r.z, current
+  double* rzn = p->d_dots + 96; // r.z, next
+  // ||A||_F of the operator the plan holds
+  const i64 nnz = (i64)p->P.ci.size();
+  launch(p, "k_sumsq", 0, 8.0 * nnz, k_sumsq, dim3(DOT_BLOCKS), nnz, (const double*)p->d_val, p->d_part);
+  launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
+  launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3(g16), n, in, ld, nr, Bv);
+  HIPCHK(hipMemsetAsync(X, 0, sizeof(double) * n16, p->stream));
+  HIPCHK(hipMemcpyAsync(R, Bv, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
+  dots(p, Bv, Bv, nullptr, nullptr, S2);
+  double h[32], na;
+  HIPCHK(hipMemcpyAsync(&na, p->d_dots + 128, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  int rc = precondition(p, R, QZ);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(Pd, QZ, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
+  dots(p, R, QZ, nullptr, nullptr, rz);
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->norm_a = std::sqrt(na);
+  double nb[RB];
+  for (int c = 0; c < RB; c++) nb[c] = std::sqrt(h[c]);
+  int active = (1 << nr) - 1;
+  double berr[RB] = {0};
+  int it = 0;
+  while (active && it < p->ir_max) {
+    it++;
+    launch(p, "k_spmv_rows", 0, 0, k_spmv_rows, dim3(gsp), n, (const int*)p->d_rp, (const int*)p->d_ci,
+           (const double*)p->d_val, (const double*)Pd, QZ);
+    dots(p, Pd, QZ, nullptr, nullptr, S1);
+    launch(p, "k_cg_xr", 0, 0, k_cg_xr, dim3(g16), n16, (const double*)rz, (const double*)S1, active,
+           (const double*)Pd, (const double*)QZ, X, R);
+    dots(p, R, R, X, X, S2);
+    HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (int c = 0; c < nr; c++) {
+      if (!((active >> c) & 1)) continue;
+      const double den = p->norm_a * std::sqrt(h[16 + c]) + nb[c];
+      berr[c] = den > 0 ? std::sqrt(h[c]) / den : 0.0;
+      if (berr[c] <= p->ir_tol) active &= ~(1 << c);
+    }
+    if (!active) break;
+    rc = precondition(p, R, QZ);
+    if (rc) return rc;
+    dots(p, R, QZ, nullptr, nullptr, rzn);
+    launch(p, "k_cg_p", 0, 0, k_cg_p, dim3(g16), n16, (const double*)rzn, (const double*)rz, active,
+           (const double*)QZ, Pd);
+    std::swap(rz, rzn);
+  }
+  launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3((unsigned)(((i64)n * nr + 255) / 256)), n,
+         (const double*)X, out, ld, nr);
+  it_out = it;
+  berr_out = 0;
+  for (int c = 0; c < nr; c++) berr_out = std::max(berr_out, berr[c]);
+  return active ? UMF_ENOCONV : UMF_OK;
+}
+
+// Shared body of the solve/apply entry points: columns in chunks of RB through the work block.
 int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, int ld, bool device) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (!p->factored) return fail(UMF_ESTATE, "factor before solve/apply");
@@ -499,9 +646,10 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   int rc = ensure_solve_scratch(p);
   if (rc) return rc;
   int n = p->P.n;
+  const bool f32 = p->fact_prec == UMF_PREC_FP32;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
   // host operands are staged through a device column block first
-  double* d_in = const_cast<double*>(in);
+  const double* d_in = in;
   double* d_out = out;
   if (!device) {
     if (!p->d_stage || p->stage_cols < nrhs) {
@@ -515,16 +663,34 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
     d_in = d_out = p->d_stage;
   }
   const int ldd = device ? ld : n;
+  int iters = 0, conv = UMF_OK;
+  double berr = 0;
   for (int c0 = 0; c0 < nrhs; c0 += RB) {
     int nr = std::min(RB, nrhs - c0);
-    unsigned gb = (unsigned)(((i64)n * RB + 255) / 256);
-    launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows, dim3(gb), n, (const double*)(d_in + (size_t)c0 * ldd), ldd,
-           nr, p->d_w);
-    rc = sweep(p, mode, nr);
-    if (rc) return rc;
-    unsigned go = (unsigned)(((i64)n * nr + 255) / 256);
-    launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows, dim3(go), n, (const double*)p->d_w,
-           d_out + (size_t)c0 * ldd, ldd, nr);
+    unsigned gb = (unsigned)(((i64)n * RB + 255) / 256), go = (unsigned)(((i64)n * nr + 255) / 256);
+    const double* ci = d_in + (size_t)c0 * ldd;
+    double* co = d_out + (size_t)c0 * ldd;
+    if (!f32) {
+      launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3(gb), n, ci, ldd, nr, p->d_w);
+      rc = sweep<double>(p, mode, nr, p->arena, p->d_w, p->d_xs);
+      if (rc) return rc;
+      launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3(go), n, (const double*)p->d_w, co, ldd,
+             nr);
+    } else if (mode == 1) {
+      launch(p, "k_to_rows", 0, 12.0 * n * nr, k_to_rows<float>, dim3(gb), n, ci, ldd, nr, p->d_w32);
+      rc = sweep<float>(p, mode, nr, p->arena32, p->d_w32, p->d_xs32);
+      if (rc) return rc;
+      launch(p, "k_from_rows", 0, 12.0 * n * nr, k_from_rows<float>, dim3(go), n, (const float*)p->d_w32, co, ldd,
+             nr);
+    } else {
+      int it = 0;
+      double be = 0;
+      rc = pcg(p, ci, co, ldd, nr, it, be);
+      if (rc && rc != UMF_ENOCONV) return rc;
+      if (rc == UMF_ENOCONV) conv = rc;
+      iters = std::max(iters, it);
+      berr = std::max(berr, be);
+    }
   }
   if (!device)
     HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * ld, p->d_stage, sizeof(double) * n, sizeof(double) * n, nrhs,
@@ -534,24 +700,33 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   prof_collect(p);
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  int serr = 0;
+  HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
+  if (serr) {
+    HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
+    return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
+  }
   if (mode == 0) {
-    int serr = 0;
-    HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
-    if (serr) {
-      HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
-      return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
-    }
     p->stats.t_solve_ms = ms;
     double fl = 0;
     for (const FrontInfo& F : p->P.fronts) fl += 2.0 * F.n * F.n + 4.0 * F.n * F.m;
     p->stats.flops_solve = fl * nrhs;
+    p->stats.ir_iters = f32 ? iters : 0;
+    p->stats.ir_berr = f32 ? berr : 0.0;
+    if (conv) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "FP32 refinement: backward error %.3e after %d iterations (tolerance %.3e)", berr,
+               iters, p->ir_tol);
+      return fail(UMF_ENOCONV, buf);
+    }
   }
   return UMF_OK;
 }
 
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
-                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage};
+                  p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
+                  p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -760,8 +935,10 @@ int umf_factor(umf_plan* p) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
   Plan::Dirty D = p->P.all_dirty();
+  p->factored = false;
+  p->fact_prec = p->prec;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
-  int rc = run_factor(p, D);
+  int rc = factor_any(p, D);
   if (rc) return rc;
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -792,7 +969,7 @@ int do_update(umf_plan* p, const double* src_val, bool src_device, int n_mod, co
   double fl = 0;
   int nf = 0;
   if (D.any) {
-    int rc = run_factor(p, D);
+    int rc = factor_any(p, D);
     if (rc) return rc;
     for (int ell = 0; ell < P.g.L; ell++)
       for (int b : D.boxes[ell]) {
@@ -916,8 +1093,9 @@ int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out,
   int f = p->P.box_gid[level][box];
   const FrontDev& F = p->fronts_h[f];
   int n = F.n, m = F.m, N = F.N;
-  std::vector<double> h((size_t)N * N);
-  HIPCHK(hipMemcpy(h.data(), p->arena + F.off, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  std::vector<double> h;
+  int rc = read_pool(p, F.off, (size_t)N * N, h);
+  if (rc) return rc;
   auto at = [&](int i, int j) { return h[(size_t)j * N + i]; };
   std::vector<double> r;
   int R = 0, Cc = 0;
@@ -993,8 +1171,9 @@ int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
   const SepDev& S = p->seps_h[p->P.sep_gid[level][sep]];
   *k = S.k;
   if (!out) return UMF_OK;
-  std::vector<double> h((size_t)S.k * S.k);
-  if (!h.empty()) HIPCHK(hipMemcpy(h.data(), p->arena + S.off, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  std::vector<double> h;
+  int rc = read_pool(p, S.off, (size_t)S.k * S.k, h);
+  if (rc) return rc;
   for (int j = 0; j < S.k; j++)
     for (int i = j; i < S.k; i++) {
       out[(size_t)j * S.k + i] = h[(size_t)j * S.k + i];
@@ -1006,6 +1185,18 @@ int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
 int umf_get_stats(const umf_plan* p, umf_stats* out) {
   if (!p || !out) return fail(UMF_EINVAL, "null argument");
   *out = p->stats;
+  out->bytes_front_pool = (p->prec == UMF_PREC_FP32 ? 4.0 : 8.0) * p->P.arena_elems;
+  return UMF_OK;
+}
+
+int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32) return fail(UMF_EINVAL, "unknown precision");
+  if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
+  if (precision != p->prec) p->factored = false;
+  p->prec = precision;
+  p->ir_tol = tol;
+  p->ir_max = max_iter;
   return UMF_OK;
 }
 

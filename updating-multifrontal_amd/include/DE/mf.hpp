@@ -49,6 +49,14 @@ class Front {
 
   void Factor() { check(umf_factor(plan_)); }
 
+  // Precision form (no reference counterpart): Precision::FP32 keeps and factors the fronts in
+  // FP32 on FP32 MFMA, and Solve refines in FP64 to backward error <= tol.  Factor() again after
+  // switching.
+  enum class Precision { FP64 = UMF_PREC_FP64, FP32 = UMF_PREC_FP32 };
+  void SetPrecision(Precision p, double tol = 1e-10, int maxIter = 50) {
+    check(umf_set_precision(plan_, (int)p, tol, maxIter));
+  }
+
   void Update(const LinAlg::Sparse<Scalar>& A, std::set<int> modifiedDOFs) {
     std::vector<int> mod(modifiedDOFs.begin(), modifiedDOFs.end());
     check(umf_update(plan_, A.row_ptr_data(), A.col_ind_data(), A.val_data(), (int)mod.size(), mod.data()));

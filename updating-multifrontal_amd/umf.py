@@ -23,7 +23,13 @@ if not os.path.exists(LIB_PATH):
 _lib = C.CDLL(LIB_PATH)
 
 UMF_OK = 0
-UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE = range(-1, -8, -1)
+UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE, UMF_ENOCONV = range(-1, -9, -1)
+PRECISIONS = {"fp64": 0, "fp32": 1}
+
+
+class RefinementError(ArithmeticError):
+    """FP32 form: the refinement stopped above the requested backward error (the returned
+    solution is the last iterate)."""
 BLOCKS = {"A22": 0, "XL": 1, "XR": 2, "S": 3, "A11post": 4, "L": 5}
 
 _ip = C.POINTER(C.c_int)
@@ -34,7 +40,8 @@ _vp = C.c_void_p
 class Stats(C.Structure):
     _fields_ = [("t_factor_ms", C.c_double), ("t_update_ms", C.c_double), ("t_solve_ms", C.c_double),
                 ("flops_factor", C.c_double), ("flops_update", C.c_double), ("flops_solve", C.c_double),
-                ("bytes_front_pool", C.c_double), ("n_fronts", C.c_int), ("n_fronts_update", C.c_int)]
+                ("bytes_front_pool", C.c_double), ("n_fronts", C.c_int), ("n_fronts_update", C.c_int),
+                ("ir_iters", C.c_int), ("ir_berr", C.c_double)]
 
 
 def _sig(name, res, args):
@@ -71,12 +78,13 @@ _dirty = _sig("umf_dirty", C.c_int, [_vp, C.c_int, C.c_int, _ip])
 _front_block = _sig("umf_get_front_block", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip])
 _sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip])
 _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
+_set_prec = _sig("umf_set_precision", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
 
 EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
             "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
-            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream"]
+            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision"]
 
 
 def _i(a):
@@ -93,6 +101,8 @@ def _check(rc):
     msg = _last_error().decode()
     if rc > 0:
         raise ArithmeticError(msg)
+    if rc == UMF_ENOCONV:
+        raise RefinementError(msg)
     if rc == UMF_EGEOM:
         raise RuntimeError(msg)
     if rc in (UMF_EINVAL, UMF_EPATTERN, UMF_EUPDATEMAP):
@@ -137,6 +147,12 @@ class _MF:
         if h:
             _destroy(h)
             self._h = None
+
+    # ---- precision form (no reference counterpart)
+    def SetPrecision(self, precision="fp64", tol=1e-10, max_iter=50):
+        """'fp64' (default, the reference's arithmetic) or 'fp32': FP32 fronts on FP32 MFMA, Solve
+        refined in FP64 (preconditioned CG) to backward error <= tol.  Factor() again afterwards."""
+        _check(_set_prec(self._h, PRECISIONS[precision], float(tol), int(max_iter)))
 
     # ---- the reference's public API
     def Factor(self):
