@@ -225,6 +225,17 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
   const float y = __builtin_amdgcn_rsqf(d);
   return fmaf(y, fmaf(-0.5f * d * y, y, 0.5f), y);
 }
+// 1/d from the hardware estimate and Newton steps y <- y (2 - d y): two in double, one in float.
+__device__ __forceinline__ float rcp_nr(float d) {
+  const float y = __builtin_amdgcn_rcpf(d);
+  return fmaf(y, fmaf(-d, y, 1.0f), y);
+}
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+  for (int it = 0; it < 2; it++) y = fma(y, fma(-d, y, 1.0), y);
+  return y;
+}
 __device__ __forceinline__ double rsqrt_nr(double d) {
   double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
@@ -258,6 +269,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
   T* U = Ts + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   T* blk = A + F.off + (i64)jb * F.N + jb;
+#pragma unroll
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e & 63, j = e >> 6;
     T v = T(0);  // strict upper part: the inverse, initially zero
@@ -272,22 +284,25 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
     const int K0 = 16 * k;
     T* Dk = Ts + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
     T* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
+    // (a) eliminate D_kk on [D_kk | I]: step j reads column j of D and row j of V and writes only
+    //     later columns / rows; the multiplier D(er, j) / D(j, j) uses 1/d (reciprocal + Newton)
     for (int j = 0; j < 16; j++) {
       __syncthreads();
       const T d = Dk[j * LD + j];
       const T ar = Dk[j * LD + er], ac = Dk[j * LD + ec], vj = Vk[j * LD + ec];
-      const T r = rsqrt_nr(d);
+      const T m = ar * rcp_nr(d);
+      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= m * ac;
+      if (er > j && ec <= j) Vk[er * LD + ec] -= m * vj;
       if (t == 0) {
-        rs[K0 + j] = r;
+        rs[K0 + j] = d;  // pivot; turned into 1/sqrt(pivot) after the block
         if (K0 + j < kb && !(d > T(0)) && atomicCAS(err, 0, 1) == 0) {
           err[1] = fid;
           err[2] = jb + K0 + j;
         }
       }
-      const T lr = ar * r;
-      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= lr * (ac * r);
-      if (er > j && ec <= j) Vk[er * LD + ec] -= (lr * r) * vj;
     }
+    __syncthreads();
+    if (t < 16) rs[K0 + t] = rsqrt_nr(rs[K0 + t]);
     __syncthreads();
     if (er >= ec) {
       Dk[ec * LD + er] *= rs[K0 + ec];
@@ -305,6 +320,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
         for (int q = 0; q < 4; q++) Ts[(K0 + mrow<T>(lane >> 4, q)) * LD + 16 * i + (lane & 15)] = acc[q];
     }
     __syncthreads();
+    // columns K0..K0+15 of L are final: store them now, overlapping the remaining work
+#pragma unroll
+    for (int e = t; e < 16 * NB; e += 256) {
+      const int i = e & 63, j = K0 + (e >> 6);
+      if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = Ts[j * LD + i];
+    }
     // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
     for (int p = w; p < 6; p += 4) {
       int cnt = 0, bi = -1, bj = -1;
@@ -347,11 +368,8 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
       for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
     __syncthreads();
   }
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e & 63, j = e >> 6;
-    if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = Ts[j * LD + i];
-  }
   T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+#pragma unroll
   for (int e = t; e < NB * NB; e += 256) {
     const int r = e & 63, c = e >> 6;
     dinv[e] = r >= c ? U[r * LD + c] : T(0);  // column-major, zero above the diagonal
