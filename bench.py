@@ -30,19 +30,39 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def make_problem(N, seed, box=32, scale=100.0, nrhs=16):
+# workloads: name -> (dims, maxSideLength, update box side, update scale or None for one LN(0,2)
+# factor, seed); the first is the headline, the 3D ones are scale runs of the same step
+WORKLOADS = {
+    "2d1023": ((1023, 1023), 16, 32, 100.0, 2),
+    "3d95": ((95, 95, 95), 8, 8, None, 3),
+    "3d127": ((127, 127, 127), 8, 8, None, 3),
+}
+
+
+def make_problem(N, seed, box=32, scale=100.0, nrhs=16, dims=None):
+    """Lognormal(0,1) nodal coefficients on the padded grid, the update multiplies a random
+    box of nodes by `scale` (or by one LN(0,2) draw when scale is None)."""
     import umf
+    dims = dims or (N, N)
+    d = len(dims)
     rng = np.random.default_rng(seed)
-    co = np.exp(rng.standard_normal((N + 2, N + 2)))
+    co = np.exp(rng.standard_normal(tuple(n + 2 for n in dims)))
     co2 = co.copy()
-    i0, j0 = (int(v) for v in rng.integers(0, N + 2 - box, size=2))
-    co2[i0:i0 + box, j0:j0 + box] *= scale
-    h = 1.0 / (N + 1)
-    rp, ci, v1 = umf.build_laplacian((N, N), co, 1, h)
-    _, _, v2 = umf.build_laplacian((N, N), co2, 1, h)
+    lo = [int(v) for v in rng.integers(0, [n + 2 - box for n in dims])]
+    if scale is None:
+        scale = float(np.exp(2.0 * rng.standard_normal()))
+    co2[tuple(slice(l, l + box) for l in lo)] *= scale
+    h = 1.0 / (dims[0] + 1)
+    rp, ci, v1 = umf.build_laplacian(dims, co, 1, h)
+    _, _, v2 = umf.build_laplacian(dims, co2, 1, h)
     mod = np.unique(ci[v1 != v2]).astype(np.int32)  # modDOF rule (testMF2D.cpp:143-150)
-    B = rng.standard_normal((nrhs, N * N))  # row r = right-hand side r: column-major n x nrhs
-    return rp, ci, v1, v2, mod, B, (i0, j0)
+    B = rng.standard_normal((nrhs, int(np.prod(dims))))  # row r = right-hand side r: column-major n x nrhs
+    return rp, ci, v1, v2, mod, B, tuple(lo)
+
+
+def progress(msg):
+    """One line per finished step on stderr (long 3D runs stay visibly alive)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 class Csr:
@@ -122,9 +142,11 @@ def fp32_form(plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
     iters = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step()
         st = plan.stats()
+        progress(f"step {i}: factor {st['t_factor_ms']:.1f} ms, update {st['t_update_ms']:.1f} ms, "
+                 f"solve {st['t_solve_ms']:.1f} ms")
         tf += st["t_factor_ms"]
         tu += st["t_update_ms"]
         ts += st["t_solve_ms"]
@@ -148,7 +170,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1023)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d1023")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
@@ -163,10 +185,11 @@ def main():
     import torch
     import umf
     torch.cuda.set_device(0)
-    N, nrhs, seed = args.n, args.nrhs, 2
-    rp, ci, v1, v2, mod, B, corner = make_problem(N, seed, nrhs=nrhs)
-    n = N * N
-    plan = umf.MF2D(Csr(rp, ci, v1), N, N, 1, 16)
+    dims, max_side, box, scale, seed = WORKLOADS[args.workload]
+    nrhs = args.nrhs
+    rp, ci, v1, v2, mod, B, corner = make_problem(dims[0], seed, box=box, scale=scale, nrhs=nrhs, dims=dims)
+    n = int(np.prod(dims))
+    plan = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, max_side)
     d_v = [torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()]
     d_B = torch.from_numpy(B).cuda()
     d_X = torch.empty_like(d_B)
@@ -181,8 +204,11 @@ def main():
         state["cur"] = nxt
         plan.solve_device(d_B.data_ptr(), d_X.data_ptr(), nrhs)
 
-    for _ in range(args.warmup):
+    progress(f"workload {args.workload}: n={n}, {plan.levels} levels, front pool "
+             f"{plan.stats()['bytes_front_pool'] / 1e9:.1f} GB")
+    for i in range(args.warmup):
         step()
+        progress(f"warmup step {i} done")
     # untimed profiling step: every kernel bracketed by events -> per-kernel table, dominant kernel
     prof_lv = {}
     if not args.no_profile:
@@ -205,9 +231,11 @@ def main():
     flops = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step()
         st = plan.stats()
+        progress(f"step {i}: factor {st['t_factor_ms']:.1f} ms, update {st['t_update_ms']:.1f} ms, "
+                 f"solve {st['t_solve_ms']:.1f} ms")
         tf += st["t_factor_ms"]
         tu += st["t_update_ms"]
         ts += st["t_solve_ms"]
@@ -255,9 +283,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"2D 5-point -div(a grad u), {N}x{N} grid (n={n}), a~lognormal(0,1) per node, "
-                        f"maxSideLength 16 ({plan.levels} levels, {plan.leaf_width(0)}^2 leaves), "
-                        f"Factor + Update (a x100 on 32x32 nodes at {corner}) + {nrhs}-RHS Solve, FP64",
+            "workload": f"{len(dims)}D {'5' if len(dims) == 2 else '7'}-point -div(a grad u), "
+                        f"{'x'.join(map(str, dims))} grid (n={n}), a~lognormal(0,1) per node, "
+                        f"maxSideLength {max_side} ({plan.levels} levels, {plan.leaf_width(0)}^{len(dims)} leaves), "
+                        f"Factor + Update (a x{'100' if scale else 'LN(0,2)'} on {box}^{len(dims)} nodes at {corner}) "
+                        f"+ {nrhs}-RHS Solve, FP64",
             "nrhs": nrhs,
             "precision": "fp64",
         },
@@ -279,7 +309,7 @@ def main():
     }
     if not args.no_fp32:
         out["fp32_ir"] = fp32_form(plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "2d1023":
         out["cpu_baseline"] = cpu_baseline(nrhs=nrhs)
     print(json.dumps(out), flush=True)
 

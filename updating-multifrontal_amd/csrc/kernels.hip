@@ -507,23 +507,26 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 }
 
 // C -= P Q^T on the lower part of a region, one 64x64 tile per workgroup.
-// MODE 0: right-looking trailing update after panel k0: P = F[rows, k0:k0+kb],
-//         Q = F[cols, k0:k0+kb], C = F[s:N, s:n] lower, s = k0+kb.
+// MODE 0: right-looking update by the finished columns [k0, k0+K), K = min(kw, n - k0):
+//         P = F[rows, k0:k0+K], Q = F[cols, k0:k0+K], C = F[s:N, s:min(n, ccap)] lower,
+//         s = k0+K (kw = 64, ccap = the group end: the update inside a column group;
+//         kw = group width, ccap = n: the group's trailing update).
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                              T* __restrict__ A, int k0, int xt, int nf) {
+                                              T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
   const FrontDev F = fr[ids[fi]];
   int s, K, kbase, rend, cend;
-  if (MODE == 0) {
+  if (MODE == 0) {  // columns [k0, k0 + kw) applied to rows [s, N) x columns [s, min(n, ccap))
     if (k0 >= F.n) return;
-    const int kb = min(NB, F.n - k0);
-    s = k0 + kb;
-    if (s >= F.n) return;
-    K = kb; kbase = k0; rend = F.N; cend = F.n;
+    K = min(kw, F.n - k0);
+    s = k0 + K;
+    cend = min(F.n, ccap);
+    if (s >= cend) return;
+    kbase = k0; rend = F.N;
   } else {
     if (F.m == 0) return;
     s = F.n; K = F.n; kbase = 0; rend = F.N; cend = F.N;

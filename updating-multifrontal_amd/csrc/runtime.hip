@@ -307,34 +307,52 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
     }
     // 3. blocked partial Cholesky of the pivot columns, 64 at a time.  Levels with many fronts
     //    go left-looking (each block column is formed once by an MFMA GEMM over the finished
-    //    columns, then factored; no trailing read-modify-write); levels with few large fronts go
-    //    right-looking (factor the block, then one wide trailing update) for parallelism.
+    //    columns, then factored; no trailing read-modify-write).  Levels with few large fronts go
+    //    right-looking in column groups of GW: each block updates only the rest of its group
+    //    (K = 64), and the group then updates the trailing matrix once (K = GW), so the trailing
+    //    read-modify-write happens once per GW columns.
     const bool left = L.f_cnt >= 32;
+    const int GW = 256;
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     const bool xcd = L.f_cnt >= 64;
     auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((L.f_cnt + 7) / 8) * per_front); };
+    auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
+      const double nc = cend - s, nr = rend - s;
+      return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
+    };
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
-      int rows = 0, tiles = 0;
-      bool partial = false;
-      double fdiag = 0, fpan = 0, fgem = 0, bgem = 0, bpan = 0;
+      const int g0 = k0 / GW * GW, gend = g0 + GW;
+      const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
+      int rows = 0, tin = 0, ttr = 0;
+      double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0;
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
         int kb = std::min(NB, F.n - k0), N = F.n + F.m;
-        partial |= kb < NB;
         double K = left ? k0 : 0;
         rows = std::max(rows, N - k0 - kb);
-        tiles = std::max(tiles, tiles_of(k0 + kb, N, F.n));
-        double s0 = k0 + kb, nc = F.n - s0, nr = N - s0;
+        double s0 = k0 + kb, nr = N - s0;
         fdiag += (double)kb * kb * kb / 3.0;
         // TRSM (substitution count) + left-looking GEMM + the future diagonal tile's update
         fpan += nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
         bpan += 16.0 * nr * kb + 8.0 * nr * K;
-        if (!left && nc > 0) {
-          double elems = nc * nr - nc * (nc - 1) / 2;  // lower part of the trailing region
-          fgem += 2.0 * kb * elems;
-          bgem += 16.0 * elems + 8.0 * kb * (nr + nc);
+        if (left) continue;
+        const int ce = std::min(F.n, gend);
+        if (k0 + kb < ce) {
+          tin = std::max(tin, tiles_of(k0 + kb, N, ce));
+          const double e = lower_elems(s0, N, ce);
+          fin += 2.0 * kb * e;
+          bin += 16.0 * e + 8.0 * kb * (nr + ce - s0);
+        }
+        if (group_done) {
+          const int kg = std::min(GW, F.n - g0), sg = g0 + kg;
+          if (sg < F.n) {
+            ttr = std::max(ttr, tiles_of(sg, N, F.n));
+            const double e = lower_elems(sg, N, F.n);
+            ftr += 2.0 * kg * e;
+            btr += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
+          }
         }
       }
       launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
@@ -344,9 +362,12 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
         launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, L.f_cnt),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt);
       }
-      if (!left && tiles > 0)
-        launch(p, "k_gemm_panel", fgem, bgem, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? tiles : 0, L.f_cnt);
+      if (tin > 0)
+        launch(p, "k_gemm_group", fin, bin, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, L.f_cnt);
+      if (ttr > 0)
+        launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0>, xcd ? dim3(xcd_grid(ttr)) : dim3(ttr, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, xcd ? ttr : 0, L.f_cnt);
     }
     // 4. Schur complement into the trailing block
     if (L.maxm > 0) {
@@ -360,7 +381,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
       }
       if (tiles > 0)
         launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, xcd ? tiles : 0, L.f_cnt);
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles : 0, L.f_cnt);
     }
     // 5. separators of this level receive both neighbours' Schur blocks
     if (L.s_cnt > 0) {
