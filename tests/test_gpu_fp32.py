@@ -151,3 +151,45 @@ def test_precision_switch_needs_factor():
     O.factor()
     b = np.ones(A.n)
     assert relerr(G.Solve(b), O.solve(b)) <= 1e-10
+
+
+def test_fp32_large_fronts():
+    """Fronts of 1500+ rows: the FP32 form's 128 x 128-tile trailing and Schur updates."""
+    dims, maxs = (23, 23, 23), 5
+    _, A, G, O = _mk(dims, maxs, "lognormal", 1, seed=4)
+    G.SetPrecision("fp32", tol=IR_TOL)
+    G.Factor()
+    O.factor()
+    for ell, b, blk in ((0, 0, "A22"), (1, 0, "S"), (1, 5, "XR")):
+        e = relerr(G.front_block(ell, b, blk), O.box_block(ell, b, blk))
+        print(ell, b, blk, e)
+        assert 1e-12 < e <= BLOCK_TOL
+    B = np.random.default_rng(8).standard_normal((A.n, 3))
+    X = G.Solve(B)
+    assert berr(A, X, B) <= 4 * IR_TOL
+    for j in range(3):
+        assert relerr(X[:, j], O.solve(B[:, j])) <= FWD_TOL
+
+
+def test_fp32_matches_fp64_at_scale():
+    """3D 63^3 (root front ~12k rows): the FP32 form's 128 x 128-tile trailing and Schur
+    updates.  No oracle at this size: FP64 Solve(Ax) = x, the FP32 form reaches the requested
+    backward error and agrees with the FP64 solve."""
+    import umf
+    dims, maxs = (63, 63, 63), 8
+    co = coeff_field(dims, "lognormal", seed=6)
+    A = laplacian(dims, co)
+    G = umf.MF3D(A, *dims, width=1, maxSideLength=maxs)
+    G.Factor()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((A.n, 2))
+    B = np.stack([A.matvec(x[:, j]) for j in range(2)], axis=1)
+    X64 = G.Solve(B)
+    assert relerr(X64, x) <= 1e-7
+    G.SetPrecision("fp32", tol=IR_TOL)
+    G.Factor()
+    X32 = G.Solve(B)
+    st = G.stats()
+    print("iterations", st["ir_iters"], "berr", st["ir_berr"])
+    assert berr(A, X32, B) <= 4 * IR_TOL
+    assert relerr(X32, X64) <= 1e-7

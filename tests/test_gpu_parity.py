@@ -197,3 +197,24 @@ def test_unsymmetric_operator_rejected():
     assert abs(D - D.T).max() > 0
     with pytest.raises(ValueError, match="not symmetric"):
         umf.MF2D(A, *dims, width=2, maxSideLength=5)
+
+
+def test_large_fronts_match_oracle():
+    """Fronts of 1500+ rows and 363 boundary rows: the right-looking column groups, the
+    128 x 128-tile trailing and Schur updates and the block-parallel sweeps all run here."""
+    dims, maxs = (23, 23, 23), 5
+    _, A, G, O = _mk(dims, maxs, "lognormal", 1, seed=4)
+    G.Factor()
+    O.factor()
+    for ell, b, blk in ((0, 0, "A22"), (1, 0, "S"), (1, 5, "XR"), (1, 7, "A11post")):
+        g = G.front_block(ell, b, blk)
+        o = O.box_block(ell, b, blk)
+        mask = ~np.isnan(g)
+        e = relerr(g[mask], o[mask])
+        print(ell, b, blk, e)
+        assert e <= FACTOR_TOL, (ell, b, blk, e)
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((A.n, 3))
+    X = G.Solve(B)
+    for j in range(3):
+        assert relerr(X[:, j], O.solve(B[:, j])) <= SOLVE_TOL

@@ -168,6 +168,68 @@ __device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __rest
   __syncthreads();
 }
 
+// The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
+// (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
+// from LDS for (TS/32)^2 MFMAs.
+template <class T, int TS>
+struct TileSmemT {
+  T As[16][TS];
+  T Bs[16][TS];
+};
+
+template <class T, int TS>
+__device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], const T* __restrict__ P, int ldp,
+                                               int pn, const T* __restrict__ Q, int ldq, int qn, int K,
+                                               TileSmemT<T, TS>& sm) {
+  constexpr int NL = TS / 64, NBK = TS / 32;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int li = t & 63, lk = t >> 6;
+  T pa[4][NL], qb[4][NL];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = kc + lk + 4 * q;
+      const bool kin = k < K;
+#pragma unroll
+      for (int u = 0; u < NL; u++) {
+        const int row = li + 64 * u;
+        pa[q][u] = (kin && row < pn) ? P[(i64)k * ldp + row] : T(0);
+        qb[q][u] = (kin && row < qn) ? Q[(i64)k * ldq + row] : T(0);
+      }
+    }
+  };
+  if (K <= 0) return;
+  fetch(0);
+  for (int kc = 0; kc < K; kc += 16) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int u = 0; u < NL; u++) {
+        sm.As[lk + 4 * q][li + 64 * u] = pa[q][u];
+        sm.Bs[lk + 4 * q][li + 64 * u] = qb[q][u];
+      }
+    __syncthreads();
+    if (kc + 16 < K) fetch(kc + 16);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+      const int kr = k4 * 4 + (lane >> 4);
+      T av[NBK], bv[NBK];
+#pragma unroll
+      for (int b = 0; b < NBK; b++) {
+        bv[b] = sm.As[kr][wr * (TS / 2) + b * 16 + (lane & 15)];
+        av[b] = sm.Bs[kr][wc * (TS / 2) + b * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int bi = 0; bi < NBK; bi++)
+#pragma unroll
+        for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
+    }
+  }
+  __syncthreads();
+}
+
 // tile-local coordinates of accumulator (bi, bj, r) of this thread
 __device__ __forceinline__ int acc_row(int bi) { return ((threadIdx.x >> 6) >> 1) * 32 + bi * 16 + (threadIdx.x & 15); }
 template <class T>
@@ -484,10 +546,11 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
       }
 }
 
-// Lower-triangular tile enumeration of the region rows [s, rend) x cols [s, cend), 64x64
+// Lower-triangular tile enumeration of the region rows [s, rend) x cols [s, cend), TS x TS
 // tiles: first the T(T+1)/2 tiles of the square part, then the full rows below it.
+template <int TS = 64>
 __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& row0, int& col0) {
-  const int T = (cend - s + 63) >> 6, RT = (rend - s + 63) >> 6;
+  const int T = (cend - s + TS - 1) / TS, RT = (rend - s + TS - 1) / TS;
   const int tri = T * (T + 1) / 2;
   if (x >= tri + (RT - T) * T) return false;
   int rt, ct;
@@ -501,8 +564,8 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
     rt = T + x / T;
     ct = x % T;
   }
-  row0 = s + 64 * rt;
-  col0 = s + 64 * ct;
+  row0 = s + TS * rt;
+  col0 = s + TS * ct;
   return true;
 }
 
@@ -513,8 +576,8 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         kw = group width, ccap = n: the group's trailing update).
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+template <class T, int MODE, int TS = 64>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                               T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
@@ -532,23 +595,26 @@ __global__ __launch_bounds__(256) void k_gemm(const int* __restrict__ ids, const
     s = F.n; K = F.n; kbase = 0; rend = F.N; cend = F.N;
   }
   int row0, col0;
-  if (!tile_of(tile, s, rend, cend, row0, col0)) return;
-  __shared__ TileSmem<T> sm;
-  V4<T> acc[2][2];
+  if (!tile_of<TS>(tile, s, rend, cend, row0, col0)) return;
+  constexpr int NBK = TS / 32;
+  __shared__ TileSmemT<T, TS> sm;
+  V4<T> acc[NBK][NBK];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NBK; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+    for (int j = 0; j < NBK; j++) acc[i][j] = vzero<T>();
   const T* base = A + F.off + (i64)kbase * F.N;
-  tile_gemm_nt(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+  tile_gemm_nt_t<T, TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   T* C = A + F.off;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
+  for (int bi = 0; bi < NBK; bi++)
 #pragma unroll
-    for (int bj = 0; bj < 2; bj++)
+    for (int bj = 0; bj < NBK; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = row0 + acc_row(bi), j = col0 + acc_col<T>(bj, r);
+        const int i = row0 + (w >> 1) * (TS / 2) + bi * 16 + (lane & 15);
+        const int j = col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r);
         if (i >= rend || j >= cend || i < j) continue;
         T* c = C + (i64)j * F.N + i;
         if (MODE == 0) {

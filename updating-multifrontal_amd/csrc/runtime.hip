@@ -241,10 +241,10 @@ void prof_collect(umf_plan* p) {
   pr.used = 0;
 }
 
-// max 64x64 tiles of the lower region rows [s, rend) x cols [s, cend)
-inline int tiles_of(int s, int rend, int cend) {
+// ts x ts tiles of the lower region rows [s, rend) x cols [s, cend)
+inline int tiles_of(int s, int rend, int cend, int ts = 64) {
   if (s >= cend) return 0;
-  int T = (cend - s + 63) / 64, RT = (rend - s + 63) / 64;
+  int T = (cend - s + ts - 1) / ts, RT = (rend - s + ts - 1) / ts;
   return T * (T + 1) / 2 + (RT - T) * T;
 }
 
@@ -313,6 +313,9 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
     //    read-modify-write happens once per GW columns.
     const bool left = L.f_cnt >= 32;
     const int GW = 256;
+    // FP32: 128 x 128 tiles for a wide update (half the operand traffic) when it still yields
+    // 2048+ tiles; in FP64 the 128-tile kernel holds 2 waves per SIMD and measured slower
+    const bool f32 = sizeof(T) == 4;
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     const bool xcd = L.f_cnt >= 64;
@@ -324,7 +327,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       const int g0 = k0 / GW * GW, gend = g0 + GW;
       const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
-      int rows = 0, tin = 0, ttr = 0;
+      int rows = 0, tin = 0, ttr = 0, ttr128 = 0;
       double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0;
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
@@ -349,6 +352,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
           const int kg = std::min(GW, F.n - g0), sg = g0 + kg;
           if (sg < F.n) {
             ttr = std::max(ttr, tiles_of(sg, N, F.n));
+            ttr128 = std::max(ttr128, tiles_of(sg, N, F.n, 128));
             const double e = lower_elems(sg, N, F.n);
             ftr += 2.0 * kg * e;
             btr += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
@@ -365,23 +369,36 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
       if (tin > 0)
         launch(p, "k_gemm_group", fin, bin, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, L.f_cnt);
-      if (ttr > 0)
-        launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0>, xcd ? dim3(xcd_grid(ttr)) : dim3(ttr, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, xcd ? ttr : 0, L.f_cnt);
+      if (ttr > 0) {
+        if (f32 && (i64)ttr128 * L.f_cnt >= 2048)
+          launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0, 128>, dim3(ttr128, L.f_cnt), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, 0, L.f_cnt);
+        else
+          launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0>, xcd ? dim3(xcd_grid(ttr)) : dim3(ttr, L.f_cnt),
+                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, xcd ? ttr : 0, L.f_cnt);
+      }
     }
     // 4. Schur complement into the trailing block
     if (L.maxm > 0) {
       int tiles = 0;
       double fl = 0, by = 0;
+      int tiles128 = 0;
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         tiles = std::max(tiles, tiles_of(F.n, F.n + F.m, F.n + F.m));
+        tiles128 = std::max(tiles128, tiles_of(F.n, F.n + F.m, F.n + F.m, 128));
         fl += (double)F.n * F.m * (F.m + 1);
         by += 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
       }
-      if (tiles > 0)
-        launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles : 0, L.f_cnt);
+      if (tiles > 0) {
+        if (sizeof(T) == 4 && (i64)tiles128 * L.f_cnt >= 2048)
+          launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1, 128>,
+                 xcd ? dim3(xcd_grid(tiles128)) : dim3(tiles128, L.f_cnt), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles128 : 0, L.f_cnt);
+        else
+          launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
+                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles : 0, L.f_cnt);
+      }
     }
     // 5. separators of this level receive both neighbours' Schur blocks
     if (L.s_cnt > 0) {
