@@ -36,6 +36,7 @@ WORKLOADS = {
     "2d1023": ((1023, 1023), 16, 32, 100.0, 2),
     "3d95": ((95, 95, 95), 8, 8, None, 3),
     "3d127": ((127, 127, 127), 8, 8, None, 3),
+    "batch256": None,  # make_batch
 }
 
 
@@ -58,6 +59,40 @@ def make_problem(N, seed, box=32, scale=100.0, nrhs=16, dims=None):
     mod = np.unique(ci[v1 != v2]).astype(np.int32)  # modDOF rule (testMF2D.cpp:143-150)
     B = rng.standard_normal((nrhs, int(np.prod(dims))))  # row r = right-hand side r: column-major n x nrhs
     return rp, ci, v1, v2, mod, B, tuple(lo)
+
+
+def make_batch(seed=4, nprob=256, nrhs=4, box=16, scale=100.0):
+    """The batched workload: nprob 2D problems, N ~ U{63..511}, a ~ lognormal(0,1), maxSideLength 8,
+    planned as one batch; the update multiplies a random box of one random problem by `scale`.
+    Returns merged CSR arrays, both value arrays, the merged modified DOFs, right-hand sides."""
+    import umf
+    rng = np.random.default_rng(seed)
+    Ns = rng.integers(63, 512, size=nprob)
+    q_upd = int(rng.integers(0, nprob))
+    problems, v2s = [], []
+    mod = None
+    for q, N in enumerate(Ns):
+        N = int(N)
+        co = np.exp(rng.standard_normal((N + 2, N + 2)))
+        h = 1.0 / (N + 1)
+        rp, ci, v = umf.build_laplacian((N, N), co, 1, h)
+        v2 = v
+        if q == q_upd:
+            i0, j0 = (int(x) for x in rng.integers(0, N + 2 - box, size=2))
+            co[i0:i0 + box, j0:j0 + box] *= scale
+            _, _, v2 = umf.build_laplacian((N, N), co, 1, h)
+            mod = np.unique(ci[v != v2]).astype(np.int64)
+        problems.append((Csr(rp, ci, v), (N, N), 1, 8))
+        v2s.append(v2)
+    plan = umf.Batch(problems)
+    mod = (mod + plan.offsets[q_upd]).astype(np.int32)
+    v1 = plan._val.copy()
+    v2 = np.concatenate(v2s)
+    B = rng.standard_normal((nrhs, plan.n))
+    desc = (f"batch of {nprob} 2D 5-point -div(a grad u) problems, N ~ U{{63..511}} (n={plan.n} in total), "
+            f"a~lognormal(0,1), maxSideLength 8, one plan ({plan.levels} root-aligned levels); "
+            f"Factor + Update (a x{scale:g} on a {box}^2 node box of problem {q_upd}) + {nrhs}-RHS Solve, FP64")
+    return plan, plan._rp, plan._ci, v1, v2, mod, B, desc
 
 
 def progress(msg):
@@ -171,7 +206,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d1023")
-    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (16; 4 for the batch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA + FP64-refinement form")
@@ -185,11 +220,21 @@ def main():
     import torch
     import umf
     torch.cuda.set_device(0)
-    dims, max_side, box, scale, seed = WORKLOADS[args.workload]
-    nrhs = args.nrhs
-    rp, ci, v1, v2, mod, B, corner = make_problem(dims[0], seed, box=box, scale=scale, nrhs=nrhs, dims=dims)
-    n = int(np.prod(dims))
-    plan = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, max_side)
+    if args.workload == "batch256":
+        nrhs = args.nrhs or 4
+        plan, rp, ci, v1, v2, mod, B, wdesc = make_batch(nrhs=nrhs)
+        n = plan.n
+    else:
+        dims, max_side, box, scale, seed = WORKLOADS[args.workload]
+        nrhs = args.nrhs or 16
+        rp, ci, v1, v2, mod, B, corner = make_problem(dims[0], seed, box=box, scale=scale, nrhs=nrhs, dims=dims)
+        n = int(np.prod(dims))
+        plan = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, max_side)
+        wdesc = (f"{len(dims)}D {'5' if len(dims) == 2 else '7'}-point -div(a grad u), "
+                 f"{'x'.join(map(str, dims))} grid (n={n}), a~lognormal(0,1) per node, "
+                 f"maxSideLength {max_side} ({plan.levels} levels, {plan.leaf_width(0)}^{len(dims)} leaves), "
+                 f"Factor + Update (a x{'100' if scale else 'LN(0,2)'} on {box}^{len(dims)} nodes at {corner}) "
+                 f"+ {nrhs}-RHS Solve, FP64")
     d_v = [torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()]
     d_B = torch.from_numpy(B).cuda()
     d_X = torch.empty_like(d_B)
@@ -283,11 +328,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{len(dims)}D {'5' if len(dims) == 2 else '7'}-point -div(a grad u), "
-                        f"{'x'.join(map(str, dims))} grid (n={n}), a~lognormal(0,1) per node, "
-                        f"maxSideLength {max_side} ({plan.levels} levels, {plan.leaf_width(0)}^{len(dims)} leaves), "
-                        f"Factor + Update (a x{'100' if scale else 'LN(0,2)'} on {box}^{len(dims)} nodes at {corner}) "
-                        f"+ {nrhs}-RHS Solve, FP64",
+            "workload": wdesc,
             "nrhs": nrhs,
             "precision": "fp64",
         },

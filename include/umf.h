@@ -52,6 +52,22 @@ typedef struct umf_plan umf_plan;
 int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_length, int n,
                     const int* row_ptr, const int* col_ind, const double* val, umf_plan** plan);
 
+/* Batch of independent problems (no reference counterpart: the reference factors one problem
+ * per object; this is one MF2D/MF3D per problem merged): problem q is described by
+ * desc[7q..7q+6] = {dim, nx, ny, nz, width, maxSideLength, n} and its CSR arrays, as for
+ * umf_plan_create.  The batch is one plan of the block-diagonal operator diag(A_0, A_1, ...):
+ * DOF d of problem q is merged DOF dof0(q) + d, so umf_factor / umf_update / umf_solve and their
+ * device variants work on the whole batch (a right-hand-side column stacks every problem's
+ * right-hand side; an update's modified DOFs may touch any problems, each gets its own dirty
+ * sets).  Every level of every tree runs in the same batched launches: the trees are aligned at
+ * the root.  flags bit 0: plan only (no device memory, val may be NULL).  Per-tree structure
+ * and block queries take single-problem plans. */
+int umf_batch_create(int nprob, const int* desc, const int* const* row_ptr, const int* const* col_ind,
+                     const double* const* val, int flags, umf_plan** plan);
+/* Problem `prob` of a batch: first merged DOF, size, tree levels, level shift (merged level =
+ * own level + shift).  Returns the number of problems (1 for a single-problem plan). */
+int umf_batch_info(const umf_plan* plan, int prob, int* dof0, int* n, int* levels, int* shift);
+
 /* Symbolic analysis only (no device memory, no values): the elimination tree, DOF lists and
  * assembly programs of umf_plan_create, for structure queries on hosts without a GPU.
  * umf_factor / umf_update / umf_solve on such a plan return UMF_ESTATE. */

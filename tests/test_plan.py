@@ -150,3 +150,65 @@ def test_flop_counts():
             m = len(P.box_dofs(ell, b, 1))
             want += n ** 3 / 3 + n * n * m + n * m * m
     assert abs(P.stats()["flops_factor"] - want) <= 1e-9 * want
+
+
+BATCH = [((31, 31), 3, 1), ((40, 33), 5, 1), ((9, 9, 9), 3, 1), ((63, 61), 4, 1), ((11, 14, 9), 2, 1)]
+
+
+def _batch(cases, seed=1):
+    import umf
+    As = [laplacian(d, coeff_field(d, "lognormal", seed=seed + i)) for i, (d, m, w) in enumerate(cases)]
+    B = umf.Batch([(A, d, w, m) for A, (d, m, w) in zip(As, cases)], analyze_only=True)
+    singles = [analyzed(A, d, w, m) for A, (d, m, w) in zip(As, cases)]
+    return As, B, singles
+
+
+def test_batch_layout_matches_single_plans():
+    As, B, singles = _batch(BATCH)
+    assert B.nprob == len(BATCH)
+    assert B.levels == max(S.levels for S in singles)
+    off = 0
+    for q, S in enumerate(singles):
+        o, n, lv, sh = B.problem(q)
+        assert (o, n, lv, sh) == (off, As[q].n, S.levels, B.levels - S.levels)
+        off += n
+    assert B.n == off
+    st, st1 = B.stats(), [S.stats() for S in singles]
+    assert st["n_fronts"] == sum(s["n_fronts"] for s in st1)
+    assert st["flops_factor"] == pytest.approx(sum(s["flops_factor"] for s in st1), rel=1e-12)
+    assert st["bytes_front_pool"] == sum(s["bytes_front_pool"] for s in st1)
+
+
+def test_batch_dirty_sets_are_the_problems_own():
+    """An update touching problems 1 and 3 marks exactly their own dirty sets, shifted to the
+    merged (root-aligned) levels and offset by the boxes of the problems before them."""
+    As, B, singles = _batch(BATCH)
+    rng = np.random.default_rng(3)
+    mods = {1: rng.choice(As[1].n, 5, replace=False), 3: rng.choice(As[3].n, 7, replace=False)}
+    for q, m in mods.items():
+        singles[q].analyze_update(m)
+    B.analyze_update(np.concatenate([m + B.offsets[q] for q, m in mods.items()]))
+    L = B.levels
+    for lam in range(L):
+        for which in (0, 1):
+            want = []
+            for q, S in enumerate(singles):
+                ell = lam - (L - S.levels)
+                if ell < 0:
+                    continue
+                base = sum((S2.nboxes(lam - (L - S2.levels)) if which == 0 else
+                            (S2.nseps(lam - (L - S2.levels)) if lam - (L - S2.levels) >= 1 else 0))
+                           for S2 in singles[:q] if lam - (L - S2.levels) >= 0)
+                if q in mods and (which == 0 or ell >= 1):
+                    want += [base + b for b in S.dirty(ell, which)]
+            assert np.array_equal(np.sort(B.dirty(lam, which)), np.sort(want)), (lam, which)
+
+
+def test_batch_errors_name_the_problem():
+    import umf
+    d = (20, 20)
+    A = laplacian(d, coeff_field(d, "uniform", seed=1))
+    with pytest.raises(RuntimeError, match="problem 1"):
+        umf.Batch([(A, d, 1, 3), (A, d, 1, 40)], analyze_only=True)  # one level only
+    with pytest.raises(ValueError, match="tree queries"):
+        umf.Batch([(A, d, 1, 3)], analyze_only=True).box_dofs(0, 0, 0)

@@ -171,10 +171,15 @@ __device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __rest
 // The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
 // (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
 // from LDS for (TS/32)^2 MFMAs.
+// The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
+// (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
+// from LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
+// c+1 (loaded into registers one chunk earlier) is written to the other stage and chunk c+2 is
+// fetched, one barrier per 16-deep chunk.
 template <class T, int TS>
 struct TileSmemT {
-  T As[16][TS];
-  T Bs[16][TS];
+  T As[2][16][TS];
+  T Bs[2][16][TS];
 };
 
 template <class T, int TS>
@@ -199,35 +204,43 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
       }
     }
   };
-  if (K <= 0) return;
-  fetch(0);
-  for (int kc = 0; kc < K; kc += 16) {
-    __syncthreads();
+  auto stage = [&](int b) {
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int u = 0; u < NL; u++) {
-        sm.As[lk + 4 * q][li + 64 * u] = pa[q][u];
-        sm.Bs[lk + 4 * q][li + 64 * u] = qb[q][u];
+        sm.As[b][lk + 4 * q][li + 64 * u] = pa[q][u];
+        sm.Bs[b][lk + 4 * q][li + 64 * u] = qb[q][u];
       }
-    __syncthreads();
-    if (kc + 16 < K) fetch(kc + 16);
+  };
+  if (K <= 0) return;
+  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
+  fetch(0);
+  stage(0);
+  if (16 < K) fetch(16);
+  __syncthreads();
+  int b = 0;
+  for (int kc = 0; kc < K; kc += 16, b ^= 1) {
 #pragma unroll
     for (int k4 = 0; k4 < 4; k4++) {
       const int kr = k4 * 4 + (lane >> 4);
       T av[NBK], bv[NBK];
 #pragma unroll
-      for (int b = 0; b < NBK; b++) {
-        bv[b] = sm.As[kr][wr * (TS / 2) + b * 16 + (lane & 15)];
-        av[b] = sm.Bs[kr][wc * (TS / 2) + b * 16 + (lane & 15)];
+      for (int x = 0; x < NBK; x++) {
+        bv[x] = sm.As[b][kr][wr * (TS / 2) + x * 16 + (lane & 15)];
+        av[x] = sm.Bs[b][kr][wc * (TS / 2) + x * 16 + (lane & 15)];
       }
 #pragma unroll
       for (int bi = 0; bi < NBK; bi++)
 #pragma unroll
         for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
     }
+    if (kc + 16 < K) {
+      stage(b ^ 1);
+      if (kc + 32 < K) fetch(kc + 32);
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // tile-local coordinates of accumulator (bi, bj, r) of this thread

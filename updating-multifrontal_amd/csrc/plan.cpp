@@ -242,8 +242,27 @@ int Grid::leaf_of_dof(int d) const {
 }
 
 // ------------------------------------------------------------------------------ dirty sets
+namespace {
+// Appends problem p's own dirty sets to the merged ones.
+void map_dirty(const Forest& Fo, int p, const Plan::Dirty& d, Plan::Dirty& D) {
+  const Plan& S = Fo.sub[p];
+  for (int ell = 0; ell < S.L; ell++) {
+    const int lam = ell + Fo.shift[p];
+    for (int b : d.boxes[ell]) D.boxes[lam].push_back(Fo.box_base[p][lam] + b);
+    for (int e : d.seps[ell]) D.seps[lam].push_back(Fo.sep_base[p][lam] + e);
+  }
+  D.any = D.any || d.any;
+}
+}  // namespace
+
 Plan::Dirty Plan::all_dirty() const {
   Dirty D;
+  if (forest) {
+    D.boxes.assign(L, {});
+    D.seps.assign(L, {});
+    for (int p = 0; p < (int)forest->sub.size(); p++) map_dirty(*forest, p, forest->sub[p].all_dirty(), D);
+    return D;
+  }
   D.boxes.assign(g.L, {});
   D.seps.assign(g.L, {});
   for (int b = 0; b < g.nboxes(g.L - 1); b++) D.boxes[g.L - 1].push_back(b);
@@ -269,6 +288,29 @@ Plan::Dirty Plan::all_dirty() const {
 }
 
 bool Plan::dirty_from_dofs(const std::vector<int>& mod, Dirty& D, std::string& err) const {
+  if (forest) {  // split the DOFs by problem, each problem's own recurrence, merged
+    const Forest& Fo = *forest;
+    const int np = (int)Fo.sub.size();
+    std::vector<std::vector<int>> per(np);
+    for (int d : mod) {
+      if (d < 0 || d >= n) { err = "modified DOF out of range"; return false; }
+      const int p = (int)(std::upper_bound(Fo.dof0.begin(), Fo.dof0.end(), d) - Fo.dof0.begin()) - 1;
+      per[p].push_back(d - Fo.dof0[p]);
+    }
+    D.boxes.assign(L, {});
+    D.seps.assign(L, {});
+    D.any = false;
+    for (int p = 0; p < np; p++) {
+      if (per[p].empty()) continue;
+      Dirty d;
+      if (!Fo.sub[p].dirty_from_dofs(per[p], d, err)) {
+        err = "problem " + std::to_string(p) + ": " + err;
+        return false;
+      }
+      map_dirty(Fo, p, d, D);
+    }
+    return true;
+  }
   D.boxes.assign(g.L, {});
   D.seps.assign(g.L, {});
   std::set<int> leaves;
@@ -509,6 +551,7 @@ std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int 
            "increasing the problem size or decreasing maxSideLength.";
   }
   if (g.L > 14) return "tree too deep";
+  P.L = g.L;
   int b = g.side(g.L - 1);
   for (int a = 0; a < dim; a++) {
     g.bw[a] = (int)std::floor((double)(g.N[a] - (b - 1) * width) / (double)b);
@@ -700,6 +743,140 @@ std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int 
   P.tri_off[P.mats.size()] = (i64)P.triples.size();
   P.tile_off[P.mats.size()] = (i64)P.tiles.size();
   return "";
+}
+
+// ------------------------------------------------------------------------------ batches
+void merge_plans(Plan& M, std::vector<Plan>&& subs) {
+  M = Plan();
+  auto Fo = std::make_shared<Forest>();
+  Fo->sub = std::move(subs);
+  std::vector<Plan>& S = Fo->sub;
+  const int np = (int)S.size();
+  int Lmax = 0;
+  for (const Plan& P : S) Lmax = std::max(Lmax, P.L);
+  M.g = S[0].g;
+  M.L = Lmax;
+  int nf = 0, ns = 0, nd = 0;
+  i64 nnz = 0, dsz = 0, ar = 0;
+  int nops = 0;
+  std::vector<i64> dofsz0(np), arena0(np);
+  std::vector<int> ops0(np);
+  Fo->shift.resize(np);
+  Fo->front0.resize(np);
+  Fo->sep0.resize(np);
+  Fo->dof0.resize(np);
+  Fo->nnz0.resize(np);
+  for (int p = 0; p < np; p++) {
+    Fo->shift[p] = Lmax - S[p].L;
+    Fo->front0[p] = nf;
+    Fo->sep0[p] = ns;
+    Fo->dof0[p] = nd;
+    Fo->nnz0[p] = nnz;
+    dofsz0[p] = dsz;
+    arena0[p] = ar;
+    ops0[p] = nops;
+    nf += S[p].nfronts();
+    ns += S[p].nseps_total();
+    nd += S[p].n;
+    nnz += (i64)S[p].ci.size();
+    dsz += (i64)S[p].dofs.size();
+    ar += S[p].arena_elems;
+    nops += (int)S[p].ops.size();
+  }
+  M.n = nd;
+  M.arena_elems = ar;
+  // block-diagonal CSR pattern
+  M.rp.reserve(nd + 1);
+  M.ci.reserve(nnz);
+  for (int p = 0; p < np; p++) {
+    for (int i = 0; i < S[p].n; i++) M.rp.push_back((int)(Fo->nnz0[p] + S[p].rp[i]));
+    for (int c : S[p].ci) M.ci.push_back(c + Fo->dof0[p]);
+  }
+  M.rp.push_back((int)nnz);
+  // level lists, aligned at the root depth
+  M.box_gid.assign(Lmax, {});
+  M.sep_gid.assign(Lmax, {});
+  Fo->box_base.assign(np, std::vector<int>(Lmax, 0));
+  Fo->sep_base.assign(np, std::vector<int>(Lmax, 0));
+  for (int lam = 0; lam < Lmax; lam++)
+    for (int p = 0; p < np; p++) {
+      const int ell = lam - Fo->shift[p];
+      Fo->box_base[p][lam] = (int)M.box_gid[lam].size();
+      Fo->sep_base[p][lam] = (int)M.sep_gid[lam].size();
+      if (ell >= 0)
+        for (int f : S[p].box_gid[ell]) M.box_gid[lam].push_back(Fo->front0[p] + f);
+      if (ell >= 1)
+        for (int e : S[p].sep_gid[ell]) M.sep_gid[lam].push_back(Fo->sep0[p] + e);
+    }
+  // fronts, separators, DOF lists
+  for (int p = 0; p < np; p++)
+    for (FrontInfo F : S[p].fronts) {
+      F.prob = p;
+      F.int_off += dofsz0[p];
+      F.bd_off += dofsz0[p];
+      for (int q = 0; q < F.nsep; q++) F.sep[q] += Fo->sep0[p];
+      F.dinv_off += arena0[p];
+      M.fronts.push_back(F);
+    }
+  for (int p = 0; p < np; p++)
+    for (SepInfo E : S[p].seps) {
+      E.dof_off += dofsz0[p];
+      E.box[0] += Fo->front0[p];
+      E.box[1] += Fo->front0[p];
+      M.seps.push_back(E);
+    }
+  M.dofs.reserve(dsz);
+  for (int p = 0; p < np; p++)
+    for (int d : S[p].dofs) M.dofs.push_back(d + Fo->dof0[p]);
+  // matrices (all fronts, then all separators) and their assembly programs
+  auto merged_mid = [&](int p, int mid) {
+    const int NFp = S[p].nfronts();
+    return mid < NFp ? Fo->front0[p] + mid : nf + Fo->sep0[p] + (mid - NFp);
+  };
+  std::vector<std::pair<int, int>> order;  // merged mid -> (problem, own mid)
+  for (int p = 0; p < np; p++)
+    for (int f = 0; f < S[p].nfronts(); f++) order.push_back({p, f});
+  for (int p = 0; p < np; p++)
+    for (int e = 0; e < S[p].nseps_total(); e++) order.push_back({p, S[p].nfronts() + e});
+  for (int p = 0; p < np; p++)
+    for (BlockOp op : S[p].ops) {
+      op.src = merged_mid(p, op.src);
+      M.ops.push_back(op);
+    }
+  M.tri_off.assign(order.size() + 1, 0);
+  M.tile_off.assign(order.size() + 1, 0);
+  for (size_t mid = 0; mid < order.size(); mid++) {
+    const int p = order[mid].first, sm = order[mid].second;
+    const Plan& P = S[p];
+    MatDesc md = P.mats[sm];
+    md.off += arena0[p];
+    M.mats.push_back(md);
+    M.tri_off[mid] = (i64)M.triples.size();
+    for (i64 t = P.tri_off[sm]; t < P.tri_off[sm + 1]; t++) {
+      Triple tr = P.triples[t];
+      tr.k = (int)(tr.k + Fo->nnz0[p]);
+      M.triples.push_back(tr);
+    }
+    M.tile_off[mid] = (i64)M.tiles.size();
+    for (i64 t = P.tile_off[sm]; t < P.tile_off[sm + 1]; t++) {
+      Tile tl = P.tiles[t];
+      tl.op += ops0[p];
+      M.tiles.push_back(tl);
+    }
+  }
+  M.tri_off[order.size()] = (i64)M.triples.size();
+  M.tile_off[order.size()] = (i64)M.tiles.size();
+  // the problems' own plans keep only what their dirty sets need (grid, sizes, level lists)
+  for (Plan& P : S) {
+    std::vector<int>().swap(P.rp);
+    std::vector<int>().swap(P.ci);
+    std::vector<int>().swap(P.dofs);
+    std::vector<Triple>().swap(P.triples);
+    std::vector<BlockOp>().swap(P.ops);
+    std::vector<Tile>().swap(P.tiles);
+    std::vector<MatDesc>().swap(P.mats);
+  }
+  M.forest = Fo;
 }
 
 }  // namespace umf

@@ -6,6 +6,7 @@
 // batching) is this library's own layout.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -62,7 +63,8 @@ struct Triple {
 };
 
 struct FrontInfo {
-  int level, box;
+  int level, box;     // level and box index in its own problem's tree
+  int prob;           // problem of a batch (0 for a single problem)
   int n, m;           // pivot size, boundary size
   i64 int_off, bd_off;  // offsets into dofs[]
   int nsep;           // number of existing neighbour separators
@@ -79,8 +81,11 @@ struct SepInfo {
   int pos[2];   // offset of this separator inside each neighbour's bdDOFs
 };
 
+struct Forest;
+
 struct Plan {
   Grid g;
+  int L = 0;  // levels (of the deepest tree for a batch)
   int n = 0;  // DOFs
   std::vector<int> rp, ci;  // CSR pattern
 
@@ -113,11 +118,31 @@ struct Plan {
   // returns false (with err) if a DOF maps outside the leaf grid
   bool dirty_from_dofs(const std::vector<int>& mod, Dirty& out, std::string& err) const;
 
+  // A batch of independent problems merged into one block-diagonal operator: the problems'
+  // own plans, and where each one sits in the merged numbering.  Null for a single problem.
+  std::shared_ptr<Forest> forest;
+
   double front_flops(int f) const {
     double nn = fronts[f].n, mm = fronts[f].m;
     return nn * nn * nn / 3.0 + nn * nn * mm + nn * mm * mm;
   }
 };
+
+// A batch: tree levels are aligned at the leaves-to-root depth of the deepest tree, problem p's
+// level ell being merged level ell + shift[p] (shift = L_max - L_p), so one batched launch per
+// stage covers every problem's fronts at that depth.
+struct Forest {
+  std::vector<Plan> sub;
+  std::vector<int> shift;           // merged level = own level + shift
+  std::vector<int> front0, sep0;    // first merged front / separator id
+  std::vector<int> dof0;            // first merged DOF
+  std::vector<i64> nnz0;            // first merged CSR entry
+  // merged box / separator index of the problem's first box / separator at each merged level
+  std::vector<std::vector<int>> box_base, sep_base;  // [problem][merged level]
+};
+
+// Merges the problems' plans (moved into the forest) into one plan of the block-diagonal batch.
+void merge_plans(Plan& M, std::vector<Plan>&& subs);
 
 // Builds the plan; returns an empty string on success or the error message.  `geom_error`
 // is set when the failure is the reference's geometry exception.

@@ -140,15 +140,14 @@ struct Sched {
 
 Sched make_sched(const Plan& P, const Plan::Dirty& D) {
   Sched S;
-  const Grid& g = P.g;
-  S.lv.assign(g.L, {});
+  S.lv.assign(P.L, {});
   int NF = P.nfronts();
   auto grid_of = [&](int mid) {
     i64 tiles = P.tile_off[mid + 1] - P.tile_off[mid];
     i64 tri = P.tri_off[mid + 1] - P.tri_off[mid];
     return (int)std::min<i64>(std::max<i64>(std::max(tiles, (tri + 1023) / 1024), 1), 1024);
   };
-  for (int ell = g.L - 1; ell >= 0; ell--) {
+  for (int ell = P.L - 1; ell >= 0; ell--) {
     Sched::Lvl& L = S.lv[ell];
     L.f_off = (int)S.ids.size();
     L.maxN = L.maxn = L.maxm = 0;
@@ -252,7 +251,6 @@ inline int tiles_of(int s, int rend, int cend, int ts = 64) {
 template <class T>
 int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
   const Plan& P = p->P;
-  const Grid& g = P.g;
   Sched S = make_sched(P, D);
   int rc = ensure_ids(p, S.ids.size());
   if (rc) return rc;
@@ -265,7 +263,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
   const int* ids = p->d_ids;
   const int NF = P.nfronts();
   auto lowerN = [](double N) { return N * (N + 1) / 2; };
-  for (int ell = g.L - 1; ell >= 0; ell--) {
+  for (int ell = P.L - 1; ell >= 0; ell--) {
     const Sched::Lvl& L = S.lv[ell];
     g_level = ell;
     // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
@@ -445,9 +443,14 @@ int check_err(umf_plan* p) {
   if (h[0]) {
     const FrontInfo& F = p->P.fronts[h[1]];
     char buf[256];
-    snprintf(buf, sizeof buf,
-             "non-positive pivot: level %d box %d pivot column %d (front of n=%d): operator not SPD", F.level, F.box,
-             h[2], F.n);
+    if (p->P.forest)
+      snprintf(buf, sizeof buf,
+               "non-positive pivot: problem %d level %d box %d pivot column %d (front of n=%d): operator not SPD",
+               F.prob, F.level, F.box, h[2], F.n);
+    else
+      snprintf(buf, sizeof buf,
+               "non-positive pivot: level %d box %d pivot column %d (front of n=%d): operator not SPD", F.level, F.box,
+               h[2], F.n);
     g_err = buf;
     return 1 + h[2];
   }
@@ -478,7 +481,7 @@ struct AllLists {
 };
 AllLists all_lists(const Plan& P) {
   AllLists A;
-  int L = P.g.L;
+  int L = P.L;
   A.f_off.assign(L, 0); A.f_cnt.assign(L, 0); A.s_off.assign(L, 0); A.s_cnt.assign(L, 0);
   for (int ell = 0; ell < L; ell++) {
     A.f_off[ell] = (int)A.ids.size();
@@ -503,7 +506,7 @@ int next_epoch(umf_plan* p) {
 template <class T>
 int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
   const Plan& P = p->P;
-  int L = P.g.L;
+  int L = P.L;
   AllLists A = all_lists(P);
   int rc = ensure_ids(p, A.ids.size());
   if (rc) return rc;
@@ -818,6 +821,10 @@ std::string check_operator(int n, const int* rp, const int* ci, const double* v)
 }  // namespace
 
 // ============================================================================== C ABI
+namespace {
+int setup_device(umf_plan* p, umf_plan** out);
+}
+
 extern "C" {
 
 const char* umf_last_error(void) { return g_err.c_str(); }
@@ -836,8 +843,17 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
     delete p;
     return fail(geom ? UMF_EGEOM : UMF_EINVAL, e);
   }
-  Plan& P = p->P;
   p->val_h.assign(val, val + row_ptr[n]);
+  return setup_device(p, out);
+}
+
+}  // extern "C"
+
+namespace {
+// Device side of a planned (single or batched) plan whose values are in val_h: descriptors,
+// front pool, assembly programs, CSR operator, stream.  Frees p on failure.
+int setup_device(umf_plan* p, umf_plan** out) {
+  Plan& P = p->P;
   // device descriptors
   std::vector<MatDev> mats;
   for (const MatDesc& M : P.mats) mats.push_back(MatDev{M.off, M.rows, M.cols});
@@ -917,6 +933,62 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
   *out = p;
   return UMF_OK;
 }
+}  // namespace
+
+extern "C" {
+
+int umf_batch_create(int nprob, const int* desc, const int* const* row_ptr, const int* const* col_ind,
+                     const double* const* val, int flags, umf_plan** out) {
+  if (!out || !desc || !row_ptr || !col_ind || (!val && !(flags & 1))) return fail(UMF_EINVAL, "null argument");
+  *out = nullptr;
+  if (nprob < 1) return fail(UMF_EINVAL, "a batch needs at least one problem");
+  std::vector<Plan> subs(nprob);
+  i64 nd = 0, nnz = 0;
+  for (int q = 0; q < nprob; q++) {
+    const int* d = desc + 7 * q;  // dim, nx, ny, nz, width, maxSideLength, n
+    const int n = d[6];
+    if (!row_ptr[q] || !col_ind[q] || (!(flags & 1) && !val[q])) return fail(UMF_EINVAL, "null argument");
+    if (n <= 0) return fail(UMF_EINVAL, "problem " + std::to_string(q) + ": n must be positive");
+    std::string e = check_operator(n, row_ptr[q], col_ind[q], (flags & 1) ? nullptr : val[q]);
+    if (!e.empty()) return fail(UMF_EINVAL, "problem " + std::to_string(q) + ": " + e);
+    bool geom = false;
+    e = build_plan(subs[q], d[0], d[1], d[2], d[3], d[4], d[5], n, row_ptr[q], col_ind[q], geom);
+    if (!e.empty()) return fail(geom ? UMF_EGEOM : UMF_EINVAL, "problem " + std::to_string(q) + ": " + e);
+    nd += n;
+    nnz += row_ptr[q][n];
+  }
+  if (nd > 0x7fffffff || nnz > 0x7fffffff) return fail(UMF_EINVAL, "batch exceeds 2^31 unknowns or nonzeros");
+  umf_plan* p = new umf_plan();
+  std::vector<std::pair<const double*, int>> vals;
+  for (int q = 0; q < nprob; q++) vals.push_back({(flags & 1) ? nullptr : val[q], row_ptr[q][desc[7 * q + 6]]});
+  merge_plans(p->P, std::move(subs));
+  p->last = p->P.all_dirty();
+  if (flags & 1) {
+    p->host_only = true;
+    double fl = 0;
+    for (int f = 0; f < p->P.nfronts(); f++) fl += p->P.front_flops(f);
+    p->stats.flops_factor = fl;
+    p->stats.bytes_front_pool = 8.0 * p->P.arena_elems;
+    p->stats.n_fronts = p->P.nfronts();
+    *out = p;
+    return UMF_OK;
+  }
+  p->val_h.reserve(nnz);
+  for (auto& v : vals) p->val_h.insert(p->val_h.end(), v.first, v.first + v.second);
+  return setup_device(p, out);
+}
+
+int umf_batch_info(const umf_plan* p, int prob, int* dof0, int* n, int* levels, int* shift) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  const Forest* Fo = p->P.forest.get();
+  const int np = Fo ? (int)Fo->sub.size() : 1;
+  if (prob < 0 || prob >= np) return fail(UMF_EINVAL, "bad problem index");
+  if (dof0) *dof0 = Fo ? Fo->dof0[prob] : 0;
+  if (n) *n = Fo ? Fo->sub[prob].n : p->P.n;
+  if (levels) *levels = Fo ? Fo->sub[prob].L : p->P.L;
+  if (shift) *shift = Fo ? Fo->shift[prob] : 0;
+  return np;
+}
 
 int umf_plan_analyze(int dim, int nx, int ny, int nz, int width, int max_side_length, int n, const int* row_ptr,
                      const int* col_ind, umf_plan** out) {
@@ -952,7 +1024,7 @@ int umf_analyze_update(umf_plan* p, int n_mod, const int* mod_dofs) {
   p->last = D;
   double fl = 0;
   int nf = 0;
-  for (int ell = 0; ell < p->P.g.L; ell++)
+  for (int ell = 0; ell < p->P.L; ell++)
     for (int b : D.boxes[ell]) {
       fl += p->P.front_flops(p->P.box_gid[ell][b]);
       nf++;
@@ -1009,7 +1081,7 @@ int do_update(umf_plan* p, const double* src_val, bool src_device, int n_mod, co
   if (D.any) {
     int rc = factor_any(p, D);
     if (rc) return rc;
-    for (int ell = 0; ell < P.g.L; ell++)
+    for (int ell = 0; ell < P.L; ell++)
       for (int b : D.boxes[ell]) {
         fl += P.front_flops(P.box_gid[ell][b]);
         nf++;
@@ -1083,20 +1155,24 @@ int umf_spmv_device(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
   return UMF_OK;
 }
 
-int umf_levels(const umf_plan* p) { return p ? p->P.g.L : UMF_EINVAL; }
+int umf_levels(const umf_plan* p) { return p ? p->P.L : UMF_EINVAL; }
 int umf_leaf_width(const umf_plan* p, int a) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || a < 0 || a >= p->P.g.dim) return fail(UMF_EINVAL, "bad axis");
   return p->P.g.bw[a];
 }
 int umf_num_boxes(const umf_plan* p, int level) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || level < 0 || level >= p->P.g.L) return fail(UMF_EINVAL, "bad level");
   return p->P.g.nboxes(level);
 }
 int umf_num_seps(const umf_plan* p, int level) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || level < 1 || level >= p->P.g.L) return fail(UMF_EINVAL, "bad level");
   return p->P.g.nseps(level);
 }
 int umf_box_dofs(const umf_plan* p, int level, int box, int which, int* out) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || level < 0 || level >= p->P.g.L || box < 0 || box >= p->P.g.nboxes(level))
     return fail(UMF_EINVAL, "bad box");
   const FrontInfo& F = p->P.fronts[p->P.box_gid[level][box]];
@@ -1106,6 +1182,7 @@ int umf_box_dofs(const umf_plan* p, int level, int box, int which, int* out) {
   return cnt;
 }
 int umf_sep_dofs(const umf_plan* p, int level, int sep, int* out) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || level < 1 || level >= p->P.g.L || sep < 0 || sep >= p->P.g.nseps(level))
     return fail(UMF_EINVAL, "bad separator");
   const SepInfo& S = p->P.seps[p->P.sep_gid[level][sep]];
@@ -1113,18 +1190,19 @@ int umf_sep_dofs(const umf_plan* p, int level, int sep, int* out) {
   return S.k;
 }
 int umf_dirty(const umf_plan* p, int level, int which, int* out) {
-  if (!p || level < 0 || level >= p->P.g.L || (int)p->last.boxes.size() != p->P.g.L)
+  if (!p || level < 0 || level >= p->P.L || (int)p->last.boxes.size() != p->P.L)
     return fail(UMF_EINVAL, "bad level or nothing factored");
   const Plan& P = p->P;
   std::vector<int> v;
   if (which == 0) v = p->last.boxes[level];
   else if (which == 1) v = p->last.seps[level];
-  else if (level < P.g.L - 1) v = p->last.seps[level];  // rebuilt from children == receive adds
+  else if (level < P.L - 1) v = p->last.seps[level];  // rebuilt from children == receive adds
   if (out) std::copy(v.begin(), v.end(), out);
   return (int)v.size();
 }
 
 int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out, int* rows, int* cols) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || !rows || !cols || level < 0 || level >= p->P.g.L || box < 0 || box >= p->P.g.nboxes(level))
     return fail(UMF_EINVAL, "bad block query");
   if (!p->factored) return fail(UMF_ESTATE, "factor before querying blocks");
@@ -1204,6 +1282,7 @@ int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out,
 }
 
 int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
+  if (p && p->P.forest) return fail(UMF_EINVAL, "tree queries take a single-problem plan");
   if (!p || !k || level < 1 || level >= p->P.g.L || sep < 0 || sep >= p->P.g.nseps(level))
     return fail(UMF_EINVAL, "bad separator");
   const SepDev& S = p->seps_h[p->P.sep_gid[level][sep]];

@@ -79,12 +79,16 @@ _front_block = _sig("umf_get_front_block", C.c_int, [_vp, C.c_int, C.c_int, C.c_
 _sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip])
 _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 _set_prec = _sig("umf_set_precision", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
+_batch_create = _sig("umf_batch_create", C.c_int, [C.c_int, _ip, C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp),
+                                                   C.c_int, C.POINTER(_vp)])
+_batch_info = _sig("umf_batch_info", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip])
 
 EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
             "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
-            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision"]
+            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision",
+            "umf_batch_create", "umf_batch_info"]
 
 
 def _i(a):
@@ -342,3 +346,60 @@ def build_laplacian(dims, coeff, width=1, h=1.0):
         val[fill[rows]] = v_[rows]
         fill[rows] += 1
     return rp, ci, val
+
+
+class Batch(_MF):
+    """Batch(problems) -- many independent problems as one plan of the block-diagonal operator
+    diag(A_0, A_1, ...); `problems` is a list of (A, dims, width, maxSideLength).  Every tree
+    level of every problem runs in the same batched launches.  Solve / Apply take the stacked
+    n_total x nrhs block (rows offsets[q] : offsets[q] + sizes[q] belong to problem q);
+    Update(q, A_q, modifiedDOFs) changes one problem.  Per-tree structure queries take
+    single-problem plans (MF2D / MF3D)."""
+
+    def __init__(self, problems, analyze_only=False):
+        desc, rps, cis, vals = [], [], [], []
+        for A, dims, width, max_side in problems:
+            rp, ci, val = _csr(A)
+            nz = dims[2] if len(dims) == 3 else 1
+            desc += [len(dims), dims[0], dims[1], nz, width, max_side, len(rp) - 1]
+            rps.append(rp)
+            cis.append(ci)
+            vals.append(val)
+        q = len(problems)
+        d = np.ascontiguousarray(desc, dtype=np.int32)
+        rpa = (_ip * q)(*[_i(a) for a in rps])
+        cia = (_ip * q)(*[_i(a) for a in cis])
+        vla = (_dp * q)(*[_d(a) for a in vals])
+        h = _vp()
+        _check(_batch_create(q, _i(d), rpa, cia, None if analyze_only else vla, 1 if analyze_only else 0, C.byref(h)))
+        self._h = h
+        self.nprob = q
+        info = [self.problem(i) for i in range(q)]
+        self.offsets = np.array([o for o, _, _, _ in info])
+        self.sizes = np.array([n for _, n, _, _ in info])
+        self.n = int(self.sizes.sum())
+        self.dims = None
+        # host copy of the merged block-diagonal operator (Update replaces one problem's slice)
+        self._nnz0 = np.cumsum([0] + [len(c) for c in cis])
+        self._rp = np.concatenate([r[:-1] + self._nnz0[i] for i, r in enumerate(rps)] + [self._nnz0[-1:]]).astype(np.int32)
+        self._ci = np.concatenate([c + self.offsets[i] for i, c in enumerate(cis)]).astype(np.int32)
+        self._val = np.concatenate(vals)
+
+    def problem(self, q):
+        """(first merged DOF, size, levels, level shift) of problem q."""
+        o, n, l, sh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        k = _batch_info(self._h, q, C.byref(o), C.byref(n), C.byref(l), C.byref(sh))
+        if k < 0:
+            _check(k)
+        return o.value, n.value, l.value, sh.value
+
+    def Update(self, q, A, modifiedDOFs):
+        """New values for problem q (same pattern); modifiedDOFs are problem-local."""
+        _, _, val = _csr(A)
+        self._val[self._nnz0[q]:self._nnz0[q + 1]] = val
+        mod = np.ascontiguousarray(sorted(set(int(d) + int(self.offsets[q]) for d in modifiedDOFs)), dtype=np.int32)
+        _check(_update(self._h, _i(self._rp), _i(self._ci), _d(self._val), len(mod), _i(mod) if len(mod) else None))
+
+    def split(self, X):
+        """The per-problem row blocks of a stacked n_total (x nrhs) array."""
+        return [X[o:o + n] for o, n in zip(self.offsets, self.sizes)]
