@@ -37,6 +37,7 @@ WORKLOADS = {
     "3d95": ((95, 95, 95), 8, 8, None, 3),
     "3d127": ((127, 127, 127), 8, 8, None, 3),
     "batch256": None,  # make_batch
+    "serve95": None,   # run_serve
 }
 
 
@@ -200,6 +201,77 @@ def fp32_form(plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
             "dtype": "f32 fronts (FP32 MFMA) + f64 refinement"}
 
 
+def run_serve(args):
+    """Serving: Poisson arrivals of coefficient-change + solve requests on one 3D 95^3
+    factorization (a ~ lognormal(0,1), maxSideLength 8, seed 5); a request multiplies a box of
+    side U{4..16} at a uniform position by one LN(0,2) factor and brings U{1..32} N(0,1)
+    right-hand sides.  The server batches every waiting request into one Update + one Solve
+    (updating-multifrontal_amd/serve.py); latency = completion - arrival on a virtual clock
+    advanced by the measured GPU service time."""
+    import torch
+    import umf
+    from serve import Request, UpdateServer, box_change
+    torch.cuda.set_device(0)
+    N, ms, seed = 95, 8, 5
+    dims = (N, N, N)
+    n = N ** 3
+    h = 1.0 / (N + 1)
+    rng = np.random.default_rng(seed)
+    coeff = np.exp(rng.standard_normal((N + 2,) * 3))
+    rp, ci, v = umf.build_laplacian(dims, coeff, 1, h)
+    plan = umf.MF3D(Csr(rp, ci, v), *dims, 1, ms)
+    out = {"metric": "update+solve request latency, continuous batching", "unit": "ms", "n_gpus": 1,
+           "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": f"3D 7-point -div(a grad u), 95^3 grid (n={n}), a~lognormal(0,1), maxSideLength {ms}; "
+                                  f"requests: a x LN(0,2) on a U{{4..16}}-side node box + U{{1..32}} N(0,1) RHS, "
+                                  f"Poisson arrivals", "requests_per_rate": args.requests},
+           "forms": []}
+    # one operator state throughout: the FP32 pass continues from the FP64 pass's last round
+    d_val = torch.from_numpy(v).cuda()
+    vals = v.copy()
+    cf = coeff.copy()
+    for prec in ("fp64", "fp32"):
+        plan.SetPrecision(prec, tol=args.ir_tol)
+        plan.Factor()
+        srv = UpdateServer(plan, d_val, torch)
+        form = {"precision": prec, "rates": []}
+        for rate in args.rates:
+            reqs = []
+            t = 0.0
+            for _ in range(args.requests):
+                t += float(rng.exponential(1.0 / rate))
+                side = int(rng.integers(4, 17))
+                lo = [int(x) for x in rng.integers(0, N + 2 - side, size=3)]
+                fac = float(np.exp(2.0 * rng.standard_normal()))
+                pos, val, mod = box_change(dims, cf, vals, lo, side, fac, rp, ci, h)
+                k = int(rng.integers(1, 33))
+                reqs.append(Request(pos, val, mod, rng.standard_normal((k, n)), t))
+            srv.rounds = []
+            lat = srv.run(reqs) * 1e3
+            # accuracy of the last round: every right-hand side of its last request
+            r = reqs[-1]
+            X = r.x.contiguous()
+            Bd = r.rhs_d
+            R = torch.empty_like(X)
+            plan.spmv_device(X.data_ptr(), R.data_ptr(), X.shape[0])
+            na = float(torch.linalg.norm(d_val))
+            be = max(float(torch.linalg.norm(Bd[j] - R[j]) / (na * torch.linalg.norm(X[j]) + torch.linalg.norm(Bd[j])))
+                     for j in range(X.shape[0]))
+            rounds = srv.rounds
+            span = max(q.done for q in reqs) - min(q.arrival for q in reqs)
+            ent = {"rate_per_s": rate, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "mean_ms": float(lat.mean()), "rounds": len(rounds),
+                   "requests_per_round": float(np.mean([x[0] for x in rounds])),
+                   "rhs_per_round": float(np.mean([x[1] for x in rounds])),
+                   "round_ms_mean": float(np.mean([x[2] for x in rounds]) * 1e3),
+                   "throughput_per_s": len(reqs) / span, "backward_error_last": be}
+            progress(f"{prec} rate {rate}/s: p50 {ent['p50_ms']:.0f} ms, p99 {ent['p99_ms']:.0f} ms, "
+                     f"{len(rounds)} rounds, berr {be:.1e}")
+            form["rates"].append(ent)
+        out["forms"].append(form)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,12 +283,17 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA + FP64-refinement form")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
+                    help="serve95: request arrival rates (per second)")
+    ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the sharded path is design-only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve95":
+        return run_serve(args)
     import torch
     import umf
     torch.cuda.set_device(0)

@@ -212,3 +212,24 @@ def test_batch_errors_name_the_problem():
         umf.Batch([(A, d, 1, 3), (A, d, 1, 40)], analyze_only=True)  # one level only
     with pytest.raises(ValueError, match="tree queries"):
         umf.Batch([(A, d, 1, 3)], analyze_only=True).box_dofs(0, 0, 0)
+
+
+@pytest.mark.parametrize("dims", [(21, 18, 19), (40, 37)])
+def test_serve_box_change_matches_rebuild(dims):
+    """A request's windowed operator change equals a full rebuild of the changed field (bit
+    for bit) and its modified DOFs follow the reference's modDOF rule."""
+    import umf
+    from serve import box_change
+    rng = np.random.default_rng(7)
+    h = 1.0 / (dims[0] + 1)
+    co = np.exp(rng.standard_normal(tuple(n + 2 for n in dims)))
+    rp, ci, v = umf.build_laplacian(dims, co, 1, h)
+    vals = v.copy()
+    cf = co.copy()
+    for side, lo in ((4, [0] * len(dims)), (6, [d + 2 - 6 for d in dims]), (5, [3] * len(dims))):
+        before = vals.copy()
+        pos, val, mod = box_change(dims, cf, vals, lo, side, 7.5, rp, ci, h)
+        _, _, full = umf.build_laplacian(dims, cf, 1, h)
+        assert np.array_equal(vals, full)
+        assert np.array_equal(np.sort(mod), np.unique(ci[before != full]))
+        assert np.array_equal(np.sort(pos), np.nonzero(before != full)[0])
