@@ -818,6 +818,80 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
 
+// ---- forward sweep with the right-hand-side rows in registers (fronts of at most 64 G rows):
+// wave w owns rows 16w..16w+15 of every 64-row chunk of the front; chunk c of those rows lives
+// in a lane's accumulator c (row 64c + 16w + mrow(lane/16, r), right-hand side lane%16), so no
+// row is written back before it is final.
+// S(rows) = sum_j L(rows, block j) x_j accumulates over the blocks; block j's own rows
+// go through LDS once (v = b_j - S_j, x_j = Linv_j v); at the end X[n:N] = -S (the boundary
+// contributions k_sep_acc gathers) and W[int] holds y.
+template <class T, int G>
+__global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                     const T* __restrict__ A, const int* __restrict__ dofs,
+                                                     T* __restrict__ W, T* __restrict__ xs) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  const T* L = A + F.off;
+  T* X = xs + F.x_off;
+  const int* di = dofs + F.int_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  __shared__ T sb[NB][RB];  // v = b_j - S_j
+  __shared__ T xb[NB][RB];  // x_j
+  V4<T> S[G];
+#pragma unroll
+  for (int c = 0; c < G; c++) S[c] = vzero<T>();
+  const int nch = (F.N + NB - 1) / NB;
+  for (int j = 0, jb = 0; jb < F.n; j++, jb += NB) {
+    const int kb = min(NB, F.n - jb);
+#pragma unroll
+    for (int c = 0; c < G; c++)
+      if (c == j)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * w + mrow<T>(q, r), gi = jb + i;
+          sb[i][c16] = i < kb ? W[(i64)di[gi] * RB + c16] - S[c][r] : T(0);
+        }
+    __syncthreads();
+    {  // x_j = Linv_j v: wave w -> rows 16w..16w+15
+      const T* Li = A + F.dinv_off + (i64)j * NB * NB;  // Linv(i, c) at Li[c*NB + i]
+      V4<T> a = vzero<T>();
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        const int c = 4 * k4 + q;
+        a = mfma4(Li[c * NB + 16 * w + c16], sb[c][c16], a);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + mrow<T>(q, r);
+        xb[i][c16] = a[r];
+        if (i < kb) W[(i64)di[jb + i] * RB + c16] = a[r];
+      }
+    }
+    __syncthreads();
+    // S(rows below the block) += L(rows, block j) x_j
+#pragma unroll
+    for (int c = 0; c < G; c++) {
+      if (c < j || c >= nch) continue;
+      const int row = NB * c + 16 * w + c16;
+      const bool rok = row < F.N && row >= jb + kb;
+      T av[NB / 4];
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        const int k = 4 * k4 + q;
+        av[k4] = (rok && k < kb) ? L[(i64)(jb + k) * F.N + row] : T(0);
+      }
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) S[c] = mfma4(av[k4], xb[4 * k4 + q][c16], S[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < G; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = NB * c + 16 * w + mrow<T>(q, r);
+      if (i >= F.n && i < F.N) X[(i64)i * RB + c16] = -S[c][r];
+    }
+}
+
 // ---- blocked sweeps for large fronts: one workgroup per 64-row block, blocks of a front
 // hand their results to one another through per-block flags (set to the launch's epoch).
 // A workgroup only waits for blocks owned by workgroups with smaller ids, which the hardware

@@ -538,10 +538,28 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
       nbt = std::max(nbt, (F.n + NB - 1) / NB + (F.m + NB - 1) / NB);
     }
   };
+  // FP64 levels of many fronts of at most 64 G rows: the forward sweep keeps the right-hand-side
+  // rows in registers (G = 6, 12 or 18 chunks of 64 rows)
+  auto chunks = [&](int ell) {
+    int mx = 0;
+    for (int f : P.box_gid[ell]) mx = std::max(mx, P.fronts[f].n + P.fronts[f].m);
+    const int c = (mx + NB - 1) / NB;
+    return c <= 6 ? 6 : c <= 12 ? 12 : c <= 18 ? 18 : 0;
+  };
   auto fwd = [&](int ell) {
     work(ell, fl, by);
     g_level = ell;
-    if (!blocked(ell)) {
+    // (FP32 keeps the streaming kernel: the register form measured slower there)
+    const int G = (sizeof(T) == 8 && !blocked(ell)) ? chunks(ell) : 0;
+    const dim3 gf(A.f_cnt[ell]);
+    const int* li = ids + A.f_off[ell];
+    if (G == 6)
+      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 6>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
+    else if (G == 12)
+      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 12>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
+    else if (G == 18)
+      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 18>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
+    else if (!blocked(ell)) {
       launch(p, "k_solve_fwd", fl, by, k_solve_fwd<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
              (const T*)arena, (const int*)p->d_dofs, w, xs);
     } else {
