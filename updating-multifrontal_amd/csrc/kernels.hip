@@ -116,18 +116,31 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
 // j >= qn read as zero; k in [0, K).  The MFMA A operand is Q and B is P, so accumulator r of a
 // lane holds C(i, j) with i = 16-block row + lane%16 and j = 16-block col + mrow(lane/16, r)
 // (the MFMA leaves D[mrow(lane/16, r)][lane%16] in accumulator r), and the 16 lanes of a
-// group cover 16 consecutive rows of C.
+// group cover 16 consecutive rows of C.  A wave skips the MFMAs of its 16 x 16 blocks that lie
+// wholly in rows >= pn or columns >= qn, and with LOWER those wholly above the diagonal
+// (P = Q, only C's lower triangle wanted); those accumulators stay zero.
 template <class T>
 struct TileSmem {
   T As[16][64];
   T Bs[16][64];
 };
 
-template <class T>
+// the wave index as a scalar, so that branches on it are uniform
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+template <class T, bool LOWER = false>
 __device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __restrict__ P, int ldp, int pn,
                                              const T* __restrict__ Q, int ldq, int qn, int K, TileSmem<T>& sm) {
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = wave_id();
   const int wr = w >> 1, wc = w & 1;
+  bool blk_on[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) {
+      const int r0 = wr * 32 + bi * 16, c0 = wc * 32 + bj * 16;
+      blk_on[bi][bj] = r0 < pn && c0 < qn && (!LOWER || r0 + 15 >= c0);
+    }
   const int li = t & 63, lk = t >> 6;
   const bool prow = li < pn, qrow = li < qn;
   T pa[4], qb[4];
@@ -141,36 +154,41 @@ __device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __rest
     }
   };
   if (K <= 0) return;
-  fetch(0);
-  for (int kc = 0; kc < K; kc += 16) {
-    __syncthreads();
+  // the common case (every block of the wave on) runs a loop without per-MFMA branches
+  const bool all_on = blk_on[0][0] && blk_on[0][1] && blk_on[1][0] && blk_on[1][1];
+  auto run = [&](auto guard) {
+    fetch(0);
+    for (int kc = 0; kc < K; kc += 16) {
+      __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sm.As[lk + 4 * q][li] = pa[q];
-      sm.Bs[lk + 4 * q][li] = qb[q];
-    }
-    __syncthreads();
-    if (kc + 16 < K) fetch(kc + 16);
+      for (int q = 0; q < 4; q++) {
+        sm.As[lk + 4 * q][li] = pa[q];
+        sm.Bs[lk + 4 * q][li] = qb[q];
+      }
+      __syncthreads();
+      if (kc + 16 < K) fetch(kc + 16);
 #pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) {
-      const int kr = k4 * 4 + (lane >> 4);
+      for (int k4 = 0; k4 < 4; k4++) {
+        const int kr = k4 * 4 + (lane >> 4);
 #pragma unroll
-      for (int bi = 0; bi < 2; bi++) {
-        const T bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
+        for (int bi = 0; bi < 2; bi++) {
+          const T bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
-          const T av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
-          acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+          for (int bj = 0; bj < 2; bj++) {
+            const T av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
+            if (!decltype(guard)::value || blk_on[bi][bj]) acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+          }
         }
       }
     }
-  }
+  };
+  if (all_on)
+    run(std::false_type{});
+  else
+    run(std::true_type{});
   __syncthreads();
 }
 
-// The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
-// (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
-// from LDS for (TS/32)^2 MFMAs.
 // The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
 // (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
 // from LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
@@ -333,24 +351,21 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 // triangle: Linv(r, c) at U[r*LD + c], U = Ts + LD, for r >= c.
 #define LD (NB + 1)
 template <class T>
-__global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   T* __restrict__ A, int jb, int* __restrict__ err) {
-  const int fid = ids[blockIdx.x];
-  const FrontDev F = fr[fid];
-  if (jb >= F.n) return;
-  const int kb = min(NB, F.n - jb);
-  __shared__ T Ts[LD * LD];
-  __shared__ T rs[NB];
+struct DiagSmem {
+  T Ts[LD * LD];
+  T rs[NB];
+};
+
+// Factor the block held in Ts (lower triangle D(i, j) at Ts[j*LD + i], kb x kb, identity
+// padding beyond kb, strict upper triangle zero): L to blk (column stride ld), Linv to dinv.
+// fid / jb name the front and the block's first column in an error report.
+template <class T>
+__device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, i64 ld, int kb, T* __restrict__ dinv,
+                          int* __restrict__ err, int fid, int jb) {
+  T* Ts = sm.Ts;
+  T* rs = sm.rs;
   T* U = Ts + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  T* blk = A + F.off + (i64)jb * F.N + jb;
-#pragma unroll
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e & 63, j = e >> 6;
-    T v = T(0);  // strict upper part: the inverse, initially zero
-    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
-    Ts[j * LD + i] = v;
-  }
   if (t < NB) Ts[NB * LD + t] = T(0);
   __syncthreads();
   if (t < NB) U[t * LD + t] = 1.0;
@@ -399,7 +414,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
 #pragma unroll
     for (int e = t; e < 16 * NB; e += 256) {
       const int i = e & 63, j = K0 + (e >> 6);
-      if (i < kb && j < kb && i >= j) blk[(i64)j * F.N + i] = Ts[j * LD + i];
+      if (i < kb && j < kb && i >= j) blk[(i64)j * ld + i] = Ts[j * LD + i];
     }
     // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
     for (int p = w; p < 6; p += 4) {
@@ -443,12 +458,31 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
       for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
     __syncthreads();
   }
-  T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
 #pragma unroll
   for (int e = t; e < NB * NB; e += 256) {
     const int r = e & 63, c = e >> 6;
     dinv[e] = r >= c ? U[r * LD + c] : T(0);  // column-major, zero above the diagonal
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                   T* __restrict__ A, int jb, int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  if (jb >= F.n) return;
+  const int kb = min(NB, F.n - jb);
+  __shared__ DiagSmem<T> sm;
+  const int t = threadIdx.x;
+  T* blk = A + F.off + (i64)jb * F.N + jb;
+#pragma unroll
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    T v = T(0);  // strict upper part: the inverse, initially zero
+    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
+    sm.Ts[j * LD + i] = v;
+  }
+  diag_core(sm, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
 }
 
 template <class T>
@@ -459,22 +493,23 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
   const FrontDev F = fr[ids[fi]];
   if (jb >= F.n) return;
   const int kb = min(NB, F.n - jb);
-  const int row0 = jb + kb + NB * tile;
-  if (row0 >= F.N) return;
-  const int nrow = min(NB, F.N - row0);
+  const int s0 = jb + kb;  // first row below the block
+  if (s0 + NB * tile >= F.N) return;
   __shared__ union {
     TileSmem<T> g;
     struct {
       T X[NB * (NB + 1)];
     } c;
   } sm;
-  const int t = threadIdx.x, lane = t & 63;
-  const int wr = (t >> 6) >> 1, wc = (t >> 6) & 1;
+  const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
+  const int w = wave_id(), wr = w >> 1, wc = w & 1;
   V4<T> acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  const int row0 = s0 + NB * tile;
+  const int nrow = min(NB, F.N - row0);
   const int K = left ? jb : 0;
   tile_gemm_nt(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
   T* pan = A + F.off + (i64)jb * F.N + row0;
@@ -489,23 +524,42 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
       }
   const T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
   __syncthreads();
-  // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k)
+  // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k), where Linv(j, k) = 0 for k > j and X(i, k) = 0
+  // for k >= kb: block column bj of the wave needs k < min(32 wc + 16 bj + 16, kb)
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  auto trsm = [&](auto half, auto guard) {
+    constexpr int WC = decltype(half)::value;
+    const int k4end = (kb + 3) / 4;
 #pragma unroll
-  for (int k4 = 0; k4 < NB / 4; k4++) {
-    const int k = k4 * 4 + (lane >> 4);
+    for (int k4 = 0; k4 < 8 * WC + 8; k4++) {
+      const int k = k4 * 4 + q;
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++) {
-      const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+      for (int bi = 0; bi < 2; bi++) {
+        const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
 #pragma unroll
-      for (int bj = 0; bj < 2; bj++) {
-        const T av = dinv[k * NB + wc * 32 + bj * 16 + (lane & 15)];
-        acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+        for (int bj = 0; bj < 2; bj++) {
+          if (k4 >= 8 * WC + 4 * bj + 4) continue;
+          const T av = dinv[k * NB + WC * 32 + bj * 16 + c16];
+          if (!decltype(guard)::value || (k4 < k4end && wr * 32 + bi * 16 < nrow))
+            acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+        }
       }
     }
+  };
+  const bool full = kb == NB && nrow == NB;
+  if (wc == 0) {
+    if (full)
+      trsm(std::integral_constant<int, 0>{}, std::false_type{});
+    else
+      trsm(std::integral_constant<int, 0>{}, std::true_type{});
+  } else {
+    if (full)
+      trsm(std::integral_constant<int, 1>{}, std::false_type{});
+    else
+      trsm(std::integral_constant<int, 1>{}, std::true_type{});
   }
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -517,7 +571,8 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
         if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
       }
   if (!left || row0 >= F.n) return;
-  // rank-64 update of the future diagonal tile D = A(row0.., row0..): D -= Y Y^T (lower)
+  // rank-64 update of the future diagonal tile D = A(row0.., row0..): D -= Y Y^T, lower
+  // triangle only (the wave above the diagonal and the upper block of the diagonal waves skip)
   __syncthreads();
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -529,23 +584,31 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
         sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
       }
   __syncthreads();
+  if (wr < wc) return;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  auto syrk = [&](auto diag_wave) {
 #pragma unroll
-  for (int k4 = 0; k4 < NB / 4; k4++) {
-    const int k = k4 * 4 + (lane >> 4);
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = k4 * 4 + q;
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++) {
-      const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + (lane & 15)];
+      for (int bi = 0; bi < 2; bi++) {
+        const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
 #pragma unroll
-      for (int bj = 0; bj < 2; bj++) {
-        const T av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + (lane & 15)];
-        acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+        for (int bj = 0; bj < 2; bj++) {
+          if (decltype(diag_wave)::value && bj > bi) continue;
+          const T av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + c16];
+          acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+        }
       }
     }
-  }
+  };
+  if (wr == wc)
+    syrk(std::true_type{});
+  else
+    syrk(std::false_type{});
   const int dn = min(NB, F.n - row0);
   T* D = A + F.off + (i64)row0 * F.N + row0;
 #pragma unroll
