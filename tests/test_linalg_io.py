@@ -1,0 +1,101 @@
+"""CPU: the reference's CSR text format (src/LinAlg/sparse.cpp:5-44), read and written by the
+drop-in C++ header (LinAlg::Sparse(filename), Sparse::write) and by umf.read_csr / write_csr.
+
+The reference's own tests read operators from such files (test_src/DE/test.cpp:28, :109, :135;
+the .dat files are not in the reference repository), so the check is a round trip of a harness
+operator through both implementations: same m, n, nnz, row pointers, column indices and
+bit-identical values, plus the reference's error for a missing file.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import umf
+from oracle_py import coeff_field, laplacian
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INC = os.path.join(HERE, "..", "updating-multifrontal_amd", "include")
+
+PROG = r"""
+#include <cstdio>
+#include <stdexcept>
+#include "LinAlg/linalg.hpp"
+int main(int argc, char** argv) {
+  try {
+    LinAlg::Sparse<double> A{std::string(argv[1])};
+    A.write(argv[2]);
+    double s = 0;
+    for (int k = 0; k < A.nnz(); k++) s += A.val(k) * (1 + k % 7);
+    std::printf("%d %d %d %.17g\n", A.m(), A.n(), A.nnz(), s);
+  } catch (const std::runtime_error& e) {
+    std::printf("runtime_error: %s\n", e.what());
+    return 3;
+  }
+  return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def prog(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("csrio")
+    src = d / "csrio.cpp"
+    src.write_text(PROG)
+    exe = d / "csrio"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", INC, str(src), "-o", str(exe)], check=True,
+                   capture_output=True, timeout=120)
+    return str(exe)
+
+
+@pytest.mark.parametrize("dims,kind", [((23, 17), "lognormal"), ((5, 6, 7), "rand")])
+def test_csr_text_round_trip(prog, tmp_path, dims, kind):
+    A = laplacian(dims, coeff_field(dims, kind, seed=3))
+    p1, p2 = str(tmp_path / "a.dat"), str(tmp_path / "b.dat")
+    umf.write_csr(p1, A)
+    r = subprocess.run([prog, p1, p2], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m, n, nnz, s = r.stdout.split()
+    assert (int(m), int(n), int(nnz)) == (A.n, A.n, len(A.val))
+    ref = 0.0
+    for k, v in enumerate(A.val.tolist()):  # the program's sequential sum, bit for bit
+        ref += v * (1 + k % 7)
+    assert float(s) == ref
+    for p in (p1, p2):  # the Python writer's file and the C++ writer's file read back identically
+        B = umf.read_csr(p)
+        assert (B.m, B.n) == (A.n, A.n)
+        assert np.array_equal(B.rp, A.rp) and np.array_equal(B.ci, A.ci)
+        assert np.array_equal(B.val, A.val)  # 17 significant digits: exact
+
+
+def test_csr_text_errors(prog, tmp_path):
+    r = subprocess.run([prog, str(tmp_path / "missing.dat"), str(tmp_path / "x.dat")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 3 and "failed to open file" in r.stdout
+    bad = tmp_path / "trunc.dat"
+    bad.write_text("3 3 4\n1 2 3 4\n0 1\n")
+    r = subprocess.run([prog, str(bad), str(tmp_path / "x.dat")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "truncated" in r.stdout
+    with pytest.raises(ValueError):
+        umf.read_csr(str(bad))
+
+
+def test_csr_file_operator_factors_like_the_array_one(tmp_path):
+    """A plan built from the file's operator has the same structure as one built from the
+    arrays (host-side analysis; no device work)."""
+    dims = (31, 29)
+    A = laplacian(dims, coeff_field(dims, "lognormal", seed=4))
+    p = str(tmp_path / "op.dat")
+    umf.write_csr(p, A)
+    B = umf.read_csr(p)
+    G1 = umf.MF2D(A, *dims, width=1, maxSideLength=5, analyze_only=True)
+    G2 = umf.MF2D(B, *dims, width=1, maxSideLength=5, analyze_only=True)
+    assert G1.levels == G2.levels
+    for ell in range(G1.levels):
+        for b in range(G1.nboxes(ell)):
+            for w in (0, 1):
+                assert np.array_equal(G1.box_dofs(ell, b, w), G2.box_dofs(ell, b, w))

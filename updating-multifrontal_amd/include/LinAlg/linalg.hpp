@@ -11,8 +11,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -127,6 +129,38 @@ class Sparse {
   Sparse() : m_(0), n_(0), nnz_(0) {}
   Sparse(const std::vector<Scalar>& val, const IdxVec& col_ind, const IdxVec& row_ptr, int m, int n)
       : m_(m), n_(n), nnz_(row_ptr.empty() ? 0 : row_ptr[m]), val_(val), col_ind_(col_ind), row_ptr_(row_ptr) {}
+  // CSR text file (sparse.cpp:5-44): whitespace-separated "m n nnz", then nnz values, nnz column
+  // indices and m+1 row pointers; nnz() is taken from the last row pointer.
+  explicit Sparse(const std::string& filename) : m_(0), n_(0), nnz_(0) {
+    std::ifstream file(filename);
+    if (!file.is_open()) throw std::runtime_error("failed to open file");
+    int nnz = 0;
+    file >> m_ >> n_ >> nnz;
+    if (!file || m_ < 0 || n_ < 0 || nnz < 0) throw std::runtime_error("malformed CSR file header");
+    val_.resize(nnz);
+    col_ind_.resize(nnz);
+    row_ptr_.resize((size_t)m_ + 1);
+    for (int i = 0; i < nnz; i++) {
+      double a;
+      file >> a;
+      val_[i] = (Scalar)a;
+    }
+    for (int i = 0; i < nnz; i++) file >> col_ind_[i];
+    for (int i = 0; i <= m_; i++) file >> row_ptr_[i];
+    if (!file) throw std::runtime_error("truncated CSR file");
+    nnz_ = row_ptr_[m_];
+  }
+  // The same format written back (full double precision), for on-disk operator exchange.
+  void write(const std::string& filename) const {
+    std::ofstream file(filename);
+    if (!file.is_open()) throw std::runtime_error("failed to open file");
+    file.precision(17);
+    file << m_ << " " << n_ << " " << nnz_ << "\n";
+    for (int i = 0; i < nnz_; i++) file << val_[i] << (i + 1 < nnz_ ? ' ' : '\n');
+    for (int i = 0; i < nnz_; i++) file << col_ind_[i] << (i + 1 < nnz_ ? ' ' : '\n');
+    for (int i = 0; i <= m_; i++) file << row_ptr_[i] << (i < m_ ? ' ' : '\n');
+    if (!file) throw std::runtime_error("failed to write file");
+  }
   int m() const { return m_; }
   int n() const { return n_; }
   int nnz() const { return nnz_; }
@@ -161,6 +195,27 @@ class Sparse {
   const int* row_ptr_data() const { return row_ptr_.data(); }
   const int* col_ind_data() const { return col_ind_.data(); }
   const Scalar* val_data() const { return val_.data(); }
+  // Printing (sparse.cpp:180-215): triples, the dense matrix, and 1-based "i,j,a," triples to
+  // data.csv for MATLAB.
+  void print() const {
+    std::cout << m_ << " " << n_ << std::endl;
+    for (int i = 0; i < m_; i++)
+      for (int k = row_ptr_[i]; k < row_ptr_[i + 1]; k++)
+        std::cout << "(i,j,a) = (" << i << "," << col_ind_[k] << "," << val_[k] << ")" << std::endl;
+    std::cout << std::flush;
+  }
+  void printDense() const {
+    std::cout << m_ << " " << n_ << std::endl;
+    for (int i = 0; i < m_; i++) {
+      for (int j = 0; j < n_; j++) std::cout << (*this)(i, j) << ' ';
+      std::cout << std::endl;
+    }
+  }
+  void printMatlab() const {
+    std::ofstream f("data.csv");
+    for (int i = 0; i < m_; i++)
+      for (int k = row_ptr_[i]; k < row_ptr_[i + 1]; k++) f << i + 1 << "," << col_ind_[k] + 1 << "," << val_[k] << ",";
+  }
 
  private:
   int m_, n_, nnz_;

@@ -283,6 +283,46 @@ class MF3D(_MF):
         super().__init__(A, (Nx, Ny, Nz), width, maxSideLength, analyze_only)
 
 
+class CSR:
+    """Host CSR operator (int32 row pointers / column indices, float64 values), the form every
+    constructor here accepts."""
+
+    def __init__(self, rp, ci, val, m, n):
+        self.rp = np.ascontiguousarray(rp, dtype=np.int32)
+        self.ci = np.ascontiguousarray(ci, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.m, self.n = int(m), int(n)
+
+
+def read_csr(path):
+    """The reference's CSR text file (src/LinAlg/sparse.cpp:5-44): whitespace-separated
+    "m n nnz", nnz values, nnz column indices, m+1 row pointers."""
+    with open(path) as f:
+        tok = f.read().split()
+    if len(tok) < 3:
+        raise ValueError("malformed CSR file header")
+    m, n, nnz = int(tok[0]), int(tok[1]), int(tok[2])
+    if m < 0 or n < 0 or nnz < 0 or len(tok) < 3 + 2 * nnz + m + 1:
+        raise ValueError("truncated CSR file")
+    val = np.array(tok[3:3 + nnz], dtype=np.float64)
+    ci = np.array(tok[3 + nnz:3 + 2 * nnz], dtype=np.int64)
+    rp = np.array(tok[3 + 2 * nnz:3 + 2 * nnz + m + 1], dtype=np.int64)
+    nz = int(rp[-1])  # the reference takes nnz from the last row pointer
+    return CSR(rp, ci[:nz], val[:nz], m, n)
+
+
+def write_csr(path, A, m=None, n=None):
+    """Write A (CSR object or scipy CSR) in the same text format, values to full precision."""
+    rp, ci, val = _csr(A)
+    m = len(rp) - 1 if m is None else m
+    n = (getattr(A, "n", None) or (A.shape[1] if hasattr(A, "shape") else m)) if n is None else n
+    with open(path, "w") as f:
+        f.write(f"{m} {n} {int(rp[-1])}\n")
+        f.write(" ".join(repr(float(v)) for v in val[:rp[-1]]) + "\n")
+        f.write(" ".join(str(int(c)) for c in ci[:rp[-1]]) + "\n")
+        f.write(" ".join(str(int(r)) for r in rp) + "\n")
+
+
 def build_laplacian(dims, coeff, width=1, h=1.0):
     """Variable-coefficient -div(a grad u) operator on the padded nodal field `coeff`
     ((N+2w)^d, indexed [i][j][k]), face coefficient = mean of the two nodal values, Dirichlet
