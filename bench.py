@@ -210,7 +210,7 @@ def run_serve(args):
     advanced by the measured GPU service time."""
     import torch
     import umf
-    from serve import Request, UpdateServer, box_change
+    from serve import Request, UpdateServer
     torch.cuda.set_device(0)
     N, ms, seed = 95, 8, 5
     dims = (N, N, N)
@@ -223,17 +223,18 @@ def run_serve(args):
     out = {"metric": "update+solve request latency, continuous batching", "unit": "ms", "n_gpus": 1,
            "higher_is_better": False, "data": "synthetic",
            "config": {"workload": f"3D 7-point -div(a grad u), 95^3 grid (n={n}), a~lognormal(0,1), maxSideLength {ms}; "
-                                  f"requests: a x LN(0,2) on a U{{4..16}}-side node box + U{{1..32}} N(0,1) RHS, "
-                                  f"Poisson arrivals", "requests_per_rate": args.requests},
+                                  f"requests: a x LN(0,2) on a U{{4..16}}-side node box (applied on the device) + "
+                                  f"U{{1..32}} N(0,1) RHS, Poisson arrivals", "requests_per_rate": args.requests},
            "forms": []}
-    # one operator state throughout: the FP32 pass continues from the FP64 pass's last round
+    # one operator state throughout: the FP32 pass continues from the FP64 pass's last round;
+    # the coefficient field and the operator values live on the device, where each request's
+    # box change is applied inside its round
     d_val = torch.from_numpy(v).cuda()
-    vals = v.copy()
-    cf = coeff.copy()
+    d_coeff = torch.from_numpy(coeff.copy()).cuda()
     for prec in ("fp64", "fp32"):
         plan.SetPrecision(prec, tol=args.ir_tol)
         plan.Factor()
-        srv = UpdateServer(plan, d_val, torch)
+        srv = UpdateServer(plan, d_val, torch, d_coeff=d_coeff, h=h)
         form = {"precision": prec, "rates": []}
         for rate in args.rates:
             reqs = []
@@ -243,9 +244,8 @@ def run_serve(args):
                 side = int(rng.integers(4, 17))
                 lo = [int(x) for x in rng.integers(0, N + 2 - side, size=3)]
                 fac = float(np.exp(2.0 * rng.standard_normal()))
-                pos, val, mod = box_change(dims, cf, vals, lo, side, fac, rp, ci, h)
                 k = int(rng.integers(1, 33))
-                reqs.append(Request(pos, val, mod, rng.standard_normal((k, n)), t))
+                reqs.append(Request(None, None, None, rng.standard_normal((k, n)), t, box=(lo, side, fac)))
             srv.rounds = []
             lat = srv.run(reqs) * 1e3
             # accuracy of the last round: every right-hand side of its last request

@@ -116,6 +116,20 @@ int umf_apply_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, i
  * Sparse::operator*(Vec), src/LinAlg/sparse.cpp:63-77), for residuals. */
 int umf_spmv_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, int ld);
 
+/* Coefficient-change request on the harness operator -div(a grad u) of
+ * test_include/DE/buildLaplacian.hpp:10-132, :244-391 (the nodal, padded coefficient field a;
+ * face value (a_p + a_q)/2; stencil width = the plan's width), done on the device: scale a by
+ * `factor` on the box of padded nodes [lo, lo + side) per axis, recompute in d_val the values of
+ * every row the change reaches (d_val: the plan's CSR pattern, holding the generator's values),
+ * and return the modified DOFs -- the columns of the entries whose value changed, the
+ * reference's rule test_src/DE/testMF2D.cpp:143-150 -- ascending in mod_dofs (host, capacity
+ * cap; *n_mod = their count, also when cap is too small).  d_coeff: device array of a, last
+ * index fastest, (N_axis + 2w) per axis; h: the generator's grid spacing.  The reference
+ * rebuilds the whole operator on the host (its test mains); this keeps coefficient-change
+ * requests on the device.  Synchronous. */
+int umf_box_scale_device(umf_plan* plan, double* d_coeff, double h, const int* lo, int side, double factor,
+                         double* d_val, int* mod_dofs, int cap, int* n_mod);
+
 /* Last error message of the calling thread. */
 const char* umf_last_error(void);
 

@@ -8,6 +8,7 @@
 //   solve      k_solve_fwd, k_solve_bwd, k_sep_acc           (multi-RHS level sweeps)
 //   apply      k_apply_up, k_apply_down
 //   residual   k_spmv
+//   operator   k_box_scale, k_lap_rows, k_lap_collect (coefficient-change requests)
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -1323,6 +1324,91 @@ __global__ __launch_bounds__(256) void k_cg_p(i64 n16, const double* __restrict_
   if (!((active >> c) & 1)) return;
   const double b = rz[c] != 0.0 ? rz_new[c] / rz[c] : 0.0;
   p[e] = z[e] + b * p[e];
+}
+
+// ----------------------------------------------------------------------------- operator values
+// Coefficient-change requests on the harness operator -div(a grad u)
+// (test_include/DE/buildLaplacian.hpp:10-132, :244-391): a is the padded nodal field,
+// (N_a + 2w) per axis, stored a[i][j](k) with the last index fastest; DOF id = i + j Nx (+ k Nx Ny).
+struct LapGeom {
+  int dim, w, N[3], P[3];  // interior and padded extents
+  double hh;               // h^2
+  double cf[4];            // stencil weights cf[1..w]
+};
+
+__device__ __forceinline__ i64 lap_cidx(const LapGeom& G, const int c[3]) {  // padded coords
+  return G.dim == 2 ? (i64)c[0] * G.P[1] + c[1] : ((i64)c[0] * G.P[1] + c[1]) * G.P[2] + c[2];
+}
+
+// a *= factor on the padded box [lo, lo + side)^dim
+__global__ __launch_bounds__(256) void k_box_scale(LapGeom G, double* __restrict__ a, int lo0, int lo1, int lo2,
+                                                   int side, double factor) {
+  const i64 cnt = G.dim == 2 ? (i64)side * side : (i64)side * side * side;
+  for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (i64)gridDim.x * 256) {
+    const int c[3] = {lo0 + (int)(e % side), lo1 + (int)(e / side % side), lo2 + (int)(e / ((i64)side * side))};
+    a[lap_cidx(G, c)] *= factor;
+  }
+}
+
+// Recompute the values of the rows in the interior box [r0, r1) (per axis) in the generator's
+// entry order and arithmetic: negative offsets (slowest axis first, far to near), the diagonal
+// (-sum over k = w..1 and the axes of cf[k] (a1 + a2) / h^2), positive offsets (fastest axis
+// first, near to far); off-diagonal value cf[k] (a_p + a_q) / 2 / h^2.  Entries whose value
+// changes flag their column with the epoch.
+__global__ __launch_bounds__(256) void k_lap_rows(LapGeom G, const double* __restrict__ a, const int* __restrict__ rp,
+                                                  double* __restrict__ val, int b0, int b1, int b2, int e0,
+                                                  int e1, int e2, int* __restrict__ flag, int epoch) {
+#pragma clang fp contract(off)
+  const int ext[3] = {e0, e1, e2};
+  const i64 cnt = (i64)e0 * e1 * e2;
+  for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (i64)gridDim.x * 256) {
+    const int p[3] = {b0 + (int)(e % ext[0]), b1 + (int)(e / ext[0] % ext[1]), b2 + (int)(e / ((i64)ext[0] * ext[1]))};
+    const int row = p[0] + G.N[0] * (p[1] + G.N[1] * p[2]);
+    const int stride[3] = {1, G.N[0], G.N[0] * G.N[1]};
+    auto coef = [&](int ax, int s) {  // a at the padded position of p shifted by s along ax
+      int c[3] = {p[0] + G.w, p[1] + G.w, p[2] + G.w};
+      c[ax] += s;
+      return a[lap_cidx(G, c)];
+    };
+    int k_pos = rp[row];
+    auto put = [&](int col, double v) {
+      if (val[k_pos] != v) flag[col] = epoch;
+      val[k_pos++] = v;
+    };
+    for (int ax = G.dim - 1; ax >= 0; ax--)
+      for (int k = G.w; k >= 1; k--)
+        if (p[ax] - k >= 0) {
+          const double ah = 0.5 * (coef(ax, -k) + coef(ax, -k + 1));
+          put(row - k * stride[ax], G.cf[k] * ah / G.hh);
+        }
+    double diag = 0.0;
+    for (int k = G.w; k >= 1; k--)
+      for (int ax = 0; ax < G.dim; ax++) {
+        const double a1 = 0.5 * (coef(ax, -k) + coef(ax, -k + 1));
+        const double a2 = 0.5 * (coef(ax, k) + coef(ax, k - 1));
+        diag = diag - G.cf[k] * (a1 + a2) / G.hh;
+      }
+    put(row, diag);
+    for (int ax = 0; ax < G.dim; ax++)
+      for (int k = 1; k <= G.w; k++)
+        if (p[ax] + k < G.N[ax]) {
+          const double ah = 0.5 * (coef(ax, k) + coef(ax, k - 1));
+          put(row + k * stride[ax], G.cf[k] * ah / G.hh);
+        }
+  }
+}
+
+// DOFs of the interior box [b, b + e) flagged with the epoch, appended to list (order
+// arbitrary; the count in *cnt)
+__global__ __launch_bounds__(256) void k_lap_collect(LapGeom G, int b0, int b1, int b2, int e0, int e1, int e2,
+                                                     const int* __restrict__ flag, int epoch, int* __restrict__ list,
+                                                     int* __restrict__ cnt) {
+  const i64 n = (i64)e0 * e1 * e2;
+  for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < n; e += (i64)gridDim.x * 256) {
+    const int p0 = b0 + (int)(e % e0), p1 = b1 + (int)(e / e0 % e1), p2 = b2 + (int)(e / ((i64)e0 * e1));
+    const int d = p0 + G.N[0] * (p1 + G.N[1] * p2);
+    if (flag[d] == epoch) list[atomicAdd(cnt, 1)] = d;
+  }
 }
 
 }  // namespace umf

@@ -90,6 +90,11 @@ struct umf_plan {
   int* d_err = nullptr;
   int* d_flags = nullptr;  // block flags of the blocked solve sweeps
   int n_flags = 0;
+  // coefficient-change requests: per-DOF change flags (epoch-tagged), the collected list
+  int* d_bflag = nullptr;
+  int* d_blist = nullptr;  // count, then the DOFs
+  int blist_cap = 0;
+  int bepoch = 0;
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
@@ -785,7 +790,7 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
-                  p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots};
+                  p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -1170,6 +1175,81 @@ int umf_spmv_device(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
                      p->d_val, x, ld, y, ld, nrhs);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(p->stream));
+  return UMF_OK;
+}
+
+int umf_box_scale_device(umf_plan* p, double* d_coeff, double h, const int* lo, int side, double factor,
+                         double* d_val, int* mod_dofs, int cap, int* n_mod) {
+  if (!p || !d_coeff || !lo || !d_val || !n_mod || (cap > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  if (p->P.forest) return fail(UMF_EINVAL, "coefficient requests take a single-problem plan");
+  const Grid& g = p->P.g;
+  const int dim = g.dim, w = g.w;
+  if (!(h > 0) || side < 1) return fail(UMF_EINVAL, "bad h or box side");
+  for (int a = 0; a < dim; a++)
+    if (lo[a] < 0 || lo[a] + side > g.N[a] + 2 * w) return fail(UMF_EINVAL, "box outside the padded field");
+  LapGeom G{};
+  G.dim = dim;
+  G.w = w;
+  G.hh = h * h;
+  const double cf[4][4] = {{0, 0, 0, 0}, {0.0, -1.0, 0, 0}, {0.0, -4.0 / 3.0, 1.0 / 12.0, 0}, {0.0, -1.5, 3.0 / 20.0, -1.0 / 90.0}};
+  if (w < 1 || w > 3) return fail(UMF_EINVAL, "stencil width must be 1..3");
+  for (int k = 0; k < 4; k++) G.cf[k] = cf[w][k];
+  for (int a = 0; a < 3; a++) {
+    G.N[a] = a < dim ? g.N[a] : 1;
+    G.P[a] = a < dim ? g.N[a] + 2 * w : 1;
+  }
+  // rows reached: interior nodes within w of the changed padded nodes (interior = padded - w),
+  // columns changed: within w of those rows
+  int r0[3] = {0, 0, 0}, re[3] = {1, 1, 1}, c0[3] = {0, 0, 0}, ce[3] = {1, 1, 1};
+  for (int a = 0; a < dim; a++) {
+    const int b = std::max(lo[a] - 2 * w, 0), e = std::min(lo[a] + side, g.N[a]);  // e > b: lo < N + 2w
+    r0[a] = b;
+    re[a] = e - b;
+    c0[a] = std::max(b - w, 0);
+    ce[a] = std::min(e + w, g.N[a]) - c0[a];
+  }
+  // the rows must be laid out as the generator lays them out: 1 + (valid neighbours) entries
+  for (int z = r0[2]; z < r0[2] + re[2]; z++)
+    for (int y = r0[1]; y < r0[1] + re[1]; y++)
+      for (int x = r0[0]; x < r0[0] + re[0]; x++) {
+        const int q[3] = {x, y, z};
+        int cnt = 1;
+        for (int a = 0; a < dim; a++) cnt += std::min(q[a], w) + std::min(g.N[a] - 1 - q[a], w);
+        const int row = x + g.N[0] * (y + g.N[1] * z);
+        if (p->P.rp[row + 1] - p->P.rp[row] != cnt)
+          return fail(UMF_EPATTERN, "operator pattern is not the harness generator's (row " + std::to_string(row) + ")");
+      }
+  const int ncand = ce[0] * ce[1] * ce[2];
+  if (!p->d_bflag) {
+    HIPCHK(hipMalloc((void**)&p->d_bflag, sizeof(int) * std::max(p->P.n, 1)));
+    HIPCHK(hipMemset(p->d_bflag, 0, sizeof(int) * std::max(p->P.n, 1)));
+  }
+  if (p->blist_cap < ncand) {
+    if (p->d_blist) HIPCHK(hipFree(p->d_blist));
+    p->d_blist = nullptr;
+    HIPCHK(hipMalloc((void**)&p->d_blist, sizeof(int) * (ncand + 1)));
+    p->blist_cap = ncand;
+  }
+  p->bepoch = p->bepoch == 0x7fffffff ? 1 : p->bepoch + 1;
+  const i64 nbox = (i64)std::pow((double)side, dim) + 0.5, nrow = (i64)re[0] * re[1] * re[2];
+  auto grid = [](i64 cnt) { return dim3((unsigned)std::min<i64>((cnt + 255) / 256, 1024)); };
+  hipLaunchKernelGGL(k_box_scale, grid(nbox), dim3(256), 0, p->stream, G, d_coeff, lo[0], lo[1], dim == 3 ? lo[2] : 0,
+                     side, factor);
+  hipLaunchKernelGGL(k_lap_rows, grid(nrow), dim3(256), 0, p->stream, G, (const double*)d_coeff, (const int*)p->d_rp,
+                     d_val, r0[0], r0[1], r0[2], re[0], re[1], re[2], p->d_bflag, p->bepoch);
+  HIPCHK(hipMemsetAsync(p->d_blist, 0, sizeof(int), p->stream));
+  hipLaunchKernelGGL(k_lap_collect, grid(ncand), dim3(256), 0, p->stream, G, c0[0], c0[1], c0[2], ce[0], ce[1], ce[2],
+                     (const int*)p->d_bflag, p->bepoch, p->d_blist + 1, p->d_blist);
+  HIPCHK(hipGetLastError());
+  std::vector<int> out((size_t)ncand + 1);
+  HIPCHK(hipMemcpyAsync(out.data(), p->d_blist, sizeof(int) * (ncand + 1), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const int cnt = out[0];
+  std::sort(out.begin() + 1, out.begin() + 1 + cnt);
+  *n_mod = cnt;
+  if (cnt > cap) return fail(UMF_EINVAL, "mod_dofs capacity too small (" + std::to_string(cnt) + " needed)");
+  std::copy(out.begin() + 1, out.begin() + 1 + cnt, mod_dofs);
   return UMF_OK;
 }
 

@@ -3,11 +3,13 @@
 A request changes the coefficient field on a box of nodes (the reference's Update use case,
 test_src/DE/testMF3D.cpp:150-180: a local coefficient change, then Update with the modified
 DOFs) and asks for the solution of k right-hand sides with the updated operator.  The server
-keeps the operator values resident on the GPU.  Whenever it becomes free it takes every request
-that has arrived: their value changes are applied in arrival order (device scatter), one
-Update runs over the union of their modified DOFs (the dirty sets of all boxes at once, so the
-shared ancestors up to the root are refactored once per round, not once per request), and one
-multi-RHS Solve answers all their right-hand sides.  Each request therefore sees the operator
+keeps the operator values (and, for box requests, the nodal coefficient field) resident on the
+GPU.  Whenever it becomes free it takes every request that has arrived: their changes are
+applied in arrival order (device scatter of given values, or the box scaling and the operator
+rows it reaches recomputed on the device), one Update runs over the union of their modified
+DOFs (the dirty sets of all boxes at once, so the shared ancestors up to the root are
+refactored once per round, not once per request), and one multi-RHS Solve answers all their
+right-hand sides.  Each request therefore sees the operator
 with every change of its round applied.
 """
 import time
@@ -18,10 +20,13 @@ from umf import build_laplacian
 
 
 class Request:
-    """Value change (CSR positions and new values), its modified DOFs, right-hand sides."""
+    """A value change -- CSR positions and new values with their modified DOFs, or a coefficient
+    box change box = (lo, side, factor) done on the device by the server -- and right-hand
+    sides."""
 
-    def __init__(self, pos, val, mod, rhs, arrival=0.0):
+    def __init__(self, pos, val, mod, rhs, arrival=0.0, box=None):
         self.pos, self.val, self.mod, self.rhs, self.arrival = pos, val, mod, rhs, arrival
+        self.box = box
         self.done = None
         self.x = None
 
@@ -62,18 +67,29 @@ def box_change(dims, coeff, vals, lo, side, factor, rp, ci, h):
 
 class UpdateServer:
     """Serves Requests on `plan` (an MF2D/MF3D already factored with the values in `d_val`, a
-    torch device tensor kept in sync with the plan)."""
+    torch device tensor kept in sync with the plan).  Box requests need the padded nodal
+    coefficient field on the device (`d_coeff`) and the generator's grid spacing `h`: the
+    server scales it and recomputes the reached operator values there
+    (MF.box_scale_device), so no operator is rebuilt on the host."""
 
-    def __init__(self, plan, d_val, torch):
+    def __init__(self, plan, d_val, torch, d_coeff=None, h=None):
         self.plan, self.d_val, self.torch = plan, d_val, torch
+        self.d_coeff, self.h = d_coeff, h
         self.rounds = []  # (requests in the round, right-hand sides, seconds)
 
     def serve_round(self, reqs):
         torch = self.torch
         t0 = time.perf_counter()
+        mods = []
         for r in reqs:  # arrival order: a later change of the same entry wins
-            self.d_val[r.pos_d] = r.val_d
-        mod = np.unique(np.concatenate([r.mod for r in reqs]))
+            if r.box is not None:
+                lo, side, f = r.box
+                r.mod = self.plan.box_scale_device(self.d_coeff.data_ptr(), self.h, lo, side, f,
+                                                   self.d_val.data_ptr())
+            else:
+                self.d_val[r.pos_d] = r.val_d
+            mods.append(r.mod)
+        mod = np.unique(np.concatenate(mods))
         self.plan.update_device(self.d_val.data_ptr(), mod)
         k = [r.rhs_d.shape[0] for r in reqs]
         B = torch.cat([r.rhs_d for r in reqs], dim=0)  # (sum k) x n, row = right-hand side
@@ -94,8 +110,9 @@ class UpdateServer:
         lasts its measured GPU time.  Sets r.done; returns the latencies done - arrival."""
         torch = self.torch
         for r in reqs:
-            r.pos_d = torch.from_numpy(r.pos).cuda()
-            r.val_d = torch.from_numpy(r.val).cuda()
+            if r.box is None:
+                r.pos_d = torch.from_numpy(r.pos).cuda()
+                r.val_d = torch.from_numpy(r.val).cuda()
             r.rhs_d = torch.from_numpy(np.ascontiguousarray(r.rhs)).cuda()
         order = sorted(range(len(reqs)), key=lambda i: reqs[i].arrival)
         t, i = 0.0, 0

@@ -67,6 +67,8 @@ _apply = _sig("umf_apply", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
 _solve_dev = _sig("umf_solve_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _apply_dev = _sig("umf_apply_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _spmv_dev = _sig("umf_spmv_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_box_scale_dev = _sig("umf_box_scale_device", C.c_int,
+                      [_vp, _vp, C.c_double, _ip, C.c_int, C.c_double, _vp, _ip, C.c_int, _ip])
 _last_error = _sig("umf_last_error", C.c_char_p, [])
 _levels = _sig("umf_levels", C.c_int, [_vp])
 _leaf_width = _sig("umf_leaf_width", C.c_int, [_vp, C.c_int])
@@ -88,7 +90,7 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision",
-            "umf_batch_create", "umf_batch_info"]
+            "umf_batch_create", "umf_batch_info", "umf_box_scale_device"]
 
 
 def _i(a):
@@ -131,6 +133,7 @@ class _MF:
         rp, ci, val = _csr(A)
         self.n = len(rp) - 1
         self.dims = tuple(dims)
+        self.width = width
         h = _vp()
         nz = dims[2] if len(dims) == 3 else 1
         if analyze_only:  # host-side symbolic analysis only, no device memory
@@ -192,6 +195,19 @@ class _MF:
 
     def apply_device(self, d_x, d_y, nrhs, ld=None):
         _check(_apply_dev(self._h, d_x, d_y, nrhs, ld or self.n))
+
+    def box_scale_device(self, d_coeff, h, lo, side, factor, d_val):
+        """Coefficient-change request on the device (umf_box_scale_device): scale the padded
+        nodal field (device pointer d_coeff) by `factor` on the box [lo, lo + side) of padded
+        nodes, recompute the reached operator values in d_val (device pointer), and return the
+        modified DOFs (ascending int32 array) for update_device."""
+        lo = np.ascontiguousarray(lo, dtype=np.int32)
+        cap = int(np.prod([min(side + 4 * self.width, n) for n in self.dims]))
+        out = np.empty(max(cap, 1), np.int32)
+        cnt = C.c_int()
+        _check(_box_scale_dev(self._h, d_coeff, float(h), _i(lo), int(side), float(factor), d_val, _i(out), cap,
+                              C.byref(cnt)))
+        return out[:cnt.value].copy()
 
     def spmv_device(self, d_x, d_y, nrhs, ld=None):
         _check(_spmv_dev(self._h, d_x, d_y, nrhs, ld or self.n))
