@@ -19,6 +19,7 @@ struct FrontDev {
   int n, m, N, nsep;
   int sep_pos[6], sep_len[6];
   int flag_off;  // first of the ceil(n/64) block flags of the blocked solve sweeps
+  int fly_off;   // its entries by panel tile in the fly table (leaf fronts), -1 = none
 };
 struct SepDev {
   i64 off;  // arena offset of the k x k A11

@@ -59,6 +59,21 @@ __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids,
   }
 }
 
+// Leaf fronts whose below-diagonal panel tiles take their entries in the panel step (FrontDev::
+// fly_off): zero only what is read before it is written -- each 64-column block's diagonal block
+// (rows [c, end of the block) of column c < n) and the trailing block (rows [c, N) of c >= n).
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_fly(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                  T* __restrict__ A) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  T* a = A + F.off;
+  for (int c = blockIdx.x; c < F.N; c += gridDim.x) {
+    T* col = a + (i64)c * F.N;
+    const int rend = c < F.n ? min(NB * (c / NB + 1), F.n) : F.N;
+    for (int r = c + threadIdx.x; r < rend; r += 256) col[r] = T(0);
+  }
+}
+
 // CSR scatter triples and 64x64 tiles of block copies into each listed matrix.  All writes of
 // one matrix are to distinct entries (the planner emits disjoint programs), so no ordering or
 // atomics are needed; the matrix must have been zeroed by k_zero_lower.
@@ -488,7 +503,10 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
 
 template <class T>
 __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    T* __restrict__ A, int jb, int left, int xt, int nf) {
+                                                    T* __restrict__ A, int jb, int left, int xt, int nf, int fly,
+                                                    const int* __restrict__ fly_tab,
+                                                    const Triple* __restrict__ fly_tri,
+                                                    const double* __restrict__ csr_val) {
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
   const FrontDev F = fr[ids[fi]];
@@ -514,6 +532,8 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
   const int K = left ? jb : 0;
   tile_gemm_nt(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
   T* pan = A + F.off + (i64)jb * F.N + row0;
+  // the tile's input: the assembled front, or (fly) its CSR entries added below
+  const bool from_csr = fly && F.fly_off >= 0;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -521,10 +541,19 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : T(0);
+        sm.c.X[j * (NB + 1) + i] =
+            (i < nrow && j < kb) ? (from_csr ? T(0) : pan[(i64)j * F.N + i]) - acc[bi][bj][r] : T(0);
       }
   const T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
   __syncthreads();
+  if (from_csr) {  // entries (row, col) of this tile: col in [jb, jb + kb), row in [row0, row0 + 64)
+    const int* tb = fly_tab + F.fly_off + (jb / NB) * ((F.N + NB - 1) / NB + 1) + tile;
+    for (int e = tb[0] + t; e < tb[1]; e += 256) {
+      const Triple tr = fly_tri[e];
+      sm.c.X[(tr.col - jb) * (NB + 1) + (tr.row - row0)] += T(csr_val[tr.k]);
+    }
+    __syncthreads();
+  }
   // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k), where Linv(j, k) = 0 for k > j and X(i, k) = 0
   // for k >= kb: block column bj of the wave needs k < min(32 wc + 16 bj + 16, kb)
 #pragma unroll

@@ -742,7 +742,47 @@ std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int 
   }
   P.tri_off[P.mats.size()] = (i64)P.triples.size();
   P.tile_off[P.mats.size()] = (i64)P.tiles.size();
+  build_fly(P);
   return "";
+}
+
+// Split the entries of every front without block copies by where the left-looking
+// factorization first reads them: the tile of column block cb = col/64 and rows
+// [s0 + 64t, s0 + 64t + 64), s0 = the end of cb's diagonal block (row >= s0, col < n), or
+// otherwise the diagonal block / trailing block, which stay zeroed-and-scattered.
+void build_fly(Plan& P) {
+  const int NF = P.nfronts();
+  P.fly_off.assign(NF, -1);
+  P.fly_tab.clear();
+  P.fly_tri.clear();
+  P.rest_tri.clear();
+  P.rest_off.assign(P.mats.size() + 1, 0);
+  for (int mid = 0; mid < (int)P.mats.size(); mid++) {
+    P.rest_off[mid] = (i64)P.rest_tri.size();
+    if (mid >= NF || P.tile_off[mid + 1] != P.tile_off[mid]) continue;
+    const FrontInfo& F = P.fronts[mid];
+    const int n = F.n, N = F.n + F.m, nbc = (n + 63) / 64, NT = (N + 63) / 64;
+    if (n == 0) continue;
+    std::vector<std::vector<Triple>> bins((size_t)nbc * NT);
+    for (i64 t = P.tri_off[mid]; t < P.tri_off[mid + 1]; t++) {
+      const Triple& tr = P.triples[t];
+      const int cb = tr.col / 64;
+      const int s0 = std::min(64 * (cb + 1), n);
+      if (tr.col < n && tr.row >= s0)
+        bins[(size_t)cb * NT + (tr.row - s0) / 64].push_back(tr);
+      else
+        P.rest_tri.push_back(tr);
+    }
+    P.fly_off[mid] = (int)P.fly_tab.size();
+    for (int cb = 0; cb < nbc; cb++) {
+      for (int t = 0; t < NT; t++) {
+        P.fly_tab.push_back((int)P.fly_tri.size());
+        for (const Triple& tr : bins[(size_t)cb * NT + t]) P.fly_tri.push_back(tr);
+      }
+      P.fly_tab.push_back((int)P.fly_tri.size());
+    }
+  }
+  P.rest_off[P.mats.size()] = (i64)P.rest_tri.size();
 }
 
 // ------------------------------------------------------------------------------ batches

@@ -106,6 +106,17 @@ struct Plan {
   std::vector<Tile> tiles;
   std::vector<i64> tile_off;  // size mats+1
 
+  // Leaf fronts (assembled from CSR entries only) at left-looking levels skip the dense zeroing:
+  // the entries below a column block's diagonal block go straight into the panel step that
+  // forms that tile (fly_*), the rest (diagonal blocks, the trailing boundary block) are zeroed
+  // and scattered as usual (rest_*).  Empty for a batch.
+  std::vector<int> fly_off;      // per front: its table in fly_tab, -1 = not such a front
+  std::vector<int> fly_tab;      // per front: for column block cb, row tile t below it:
+                                 // fly_tri[fly_tab[off + cb*(NT+1) + t] .. +1], NT = ceil(N/64)
+  std::vector<Triple> fly_tri;   // grouped by (column block, row tile)
+  std::vector<Triple> rest_tri;  // the front's other entries
+  std::vector<i64> rest_off;     // per matrix (size mats+1)
+
   int nfronts() const { return (int)fronts.size(); }
   int nseps_total() const { return (int)seps.size(); }
 
@@ -143,6 +154,8 @@ struct Forest {
 
 // Merges the problems' plans (moved into the forest) into one plan of the block-diagonal batch.
 void merge_plans(Plan& M, std::vector<Plan>&& subs);
+// the fly_* / rest_* split of build_plan's assembly programs
+void build_fly(Plan& P);
 
 // Builds the plan; returns an empty string on success or the error message.  `geom_error`
 // is set when the failure is the reference's geometry exception.

@@ -81,6 +81,10 @@ struct umf_plan {
   SepDev* d_seps = nullptr;
   Triple* d_tri = nullptr;
   i64* d_tri_off = nullptr;
+  int* d_fly_tab = nullptr;  // leaf fronts' entries by panel tile (Plan::fly_*, rest_*)
+  Triple* d_fly_tri = nullptr;
+  Triple* d_rest_tri = nullptr;
+  i64* d_rest_off = nullptr;
   BlockOp* d_ops = nullptr;
   Tile* d_tiles = nullptr;
   i64* d_tile_off = nullptr;
@@ -298,23 +302,43 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
              (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
     if (L.f_cnt == 0) continue;
+    // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
+    //    64 at a time.  Levels with many fronts go left-looking (each block column is formed
+    //    once by an MFMA GEMM over the finished columns, then factored; no trailing
+    //    read-modify-write).  Levels with few large fronts go right-looking in column groups of
+    //    GW: each block updates only the rest of its group (K = 64), and the group then updates
+    //    the trailing matrix once (K = GW), so the trailing read-modify-write happens once per GW
+    //    columns.
+    const bool left = L.f_cnt >= 32;
+    // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
+    // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
+    bool fly = left && !P.rest_off.empty();
+    for (int i = 0; fly && i < L.f_cnt; i++) fly = P.fly_off[S.ids[L.f_off + i]] >= 0;
     // 2. front assembly
     {
       double zb, ab;
       asm_work(L.f_off, L.f_cnt, zb, ab);
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
-             (const MatDev*)p->d_mats, arena);
-      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off, (const MatDev*)p->d_mats,
-             arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
-             (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+      if (fly) {
+        zb = ab = 0;
+        for (int i = 0; i < L.f_cnt; i++) {
+          const int f = S.ids[L.f_off + i];
+          const double n = P.fronts[f].n, m = P.fronts[f].m;
+          zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
+          ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
+        }
+        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, arena);
+        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off,
+               (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
+               (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+      } else {
+        launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
+               (const MatDev*)p->d_mats, arena);
+        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off,
+               (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
+               (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+      }
     }
-    // 3. blocked partial Cholesky of the pivot columns, 64 at a time.  Levels with many fronts
-    //    go left-looking (each block column is formed once by an MFMA GEMM over the finished
-    //    columns, then factored; no trailing read-modify-write).  Levels with few large fronts go
-    //    right-looking in column groups of GW: each block updates only the rest of its group
-    //    (K = 64), and the group then updates the trailing matrix once (K = GW), so the trailing
-    //    read-modify-write happens once per GW columns.
-    const bool left = L.f_cnt >= 32;
     const int GW = 256;
     // FP32: 128 x 128 tiles for a wide update (half the operand traffic) when it still yields
     // 2048+ tiles; in FP64 the 128-tile kernel holds 2 waves per SIMD and measured slower
@@ -342,7 +366,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
         fdiag += (double)kb * kb * kb / 3.0;
         // TRSM (substitution count) + left-looking GEMM + the future diagonal tile's update
         fpan += nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
-        bpan += 16.0 * nr * kb + 8.0 * nr * K;
+        bpan += (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
         if (left) continue;
         const int ce = std::min(F.n, gend);
         if (k0 + kb < ce) {
@@ -367,7 +391,8 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
       if (rows > 0) {
         const int pt = (rows + NB - 1) / NB;
         launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt);
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt,
+               fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
       }
       if (tin > 0)
         launch(p, "k_gemm_group", fin, bin, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
@@ -790,7 +815,8 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
-                  p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist};
+                  p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist,
+                  p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -898,6 +924,7 @@ int setup_device(umf_plan* p, umf_plan** out) {
     D.dinv_off = F.dinv_off;
     D.flag_off = p->n_flags;
     p->n_flags += (F.n + NB - 1) / NB;
+    D.fly_off = P.fly_off.empty() ? -1 : P.fly_off[&F - &P.fronts[0]];
     xo += (i64)D.N * RB;
     p->fronts_h.push_back(D);
   }
@@ -939,6 +966,12 @@ int setup_device(umf_plan* p, umf_plan** out) {
   UP(p->d_ops, P.ops);
   UP(p->d_tiles, P.tiles);
   UP(p->d_tile_off, P.tile_off);
+  if (!P.rest_off.empty()) {
+    UP(p->d_fly_tab, P.fly_tab);
+    UP(p->d_fly_tri, P.fly_tri);
+    UP(p->d_rest_tri, P.rest_tri);
+    UP(p->d_rest_off, P.rest_off);
+  }
   UP(p->d_rp, P.rp);
   UP(p->d_ci, P.ci);
   UP(p->d_val, p->val_h);
