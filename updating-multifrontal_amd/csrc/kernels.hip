@@ -45,6 +45,7 @@ template <> __device__ __forceinline__ int mrow<double>(int q, int r) { return q
 template <> __device__ __forceinline__ int mrow<float>(int q, int r) { return MROW_F32(q, r); }
 
 #define NB 64  // panel width of the blocked factorization and of the solve blocks
+#define SPIN_LIMIT (1 << 22)  // polls of a bounded wait between workgroups of one launch
 
 // ----------------------------------------------------------------------------- assembly
 // Zero the lower triangle (row >= col) of each listed matrix.
@@ -60,17 +61,30 @@ __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids,
 }
 
 // Leaf fronts whose below-diagonal panel tiles take their entries in the panel step (FrontDev::
-// fly_off): zero only what is read before it is written -- each 64-column block's diagonal block
-// (rows [c, end of the block) of column c < n) and the trailing block (rows [c, N) of c >= n).
+// fly_off): zero only what is read before it is written -- the lower triangles of the 64-column
+// blocks' diagonal blocks and of the trailing block.
+// Work unit u < nb = ceil(n/64): diagonal block u, 64 x 64 per workgroup; u >= nb: trailing
+// column n + (u - nb).
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_fly(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                   T* __restrict__ A) {
   const FrontDev F = fr[ids[blockIdx.y]];
   T* a = A + F.off;
-  for (int c = blockIdx.x; c < F.N; c += gridDim.x) {
-    T* col = a + (i64)c * F.N;
-    const int rend = c < F.n ? min(NB * (c / NB + 1), F.n) : F.N;
-    for (int r = c + threadIdx.x; r < rend; r += 256) col[r] = T(0);
+  const int nb = (F.n + NB - 1) / NB;
+  for (int u = blockIdx.x; u < nb + F.m; u += gridDim.x) {
+    if (u < nb) {
+      const int c0 = NB * u, kb = min(NB, F.n - c0);
+      T* blk = a + (i64)c0 * F.N + c0;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int i = e & (NB - 1), j = e / NB;
+        if (i >= j && i < kb && j < kb) blk[(i64)j * F.N + i] = T(0);
+      }
+    } else {
+      const int c = F.n + (u - nb);
+      T* col = a + (i64)c * F.N;
+      for (int r = c + threadIdx.x; r < F.N; r += 256) col[r] = T(0);
+    }
   }
 }
 
@@ -373,11 +387,12 @@ struct DiagSmem {
 };
 
 // Factor the block held in Ts (lower triangle D(i, j) at Ts[j*LD + i], kb x kb, identity
-// padding beyond kb, strict upper triangle zero): L to blk (column stride ld), Linv to dinv.
-// fid / jb name the front and the block's first column in an error report.
+// padding beyond kb, strict upper triangle zero): L to blk (column stride ld), Linv to dinv,
+// unless `store` is false (L and Linv then stay in LDS only).  fid / jb name the front and the
+// block's first column in an error report.
 template <class T>
 __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, i64 ld, int kb, T* __restrict__ dinv,
-                          int* __restrict__ err, int fid, int jb) {
+                          int* __restrict__ err, int fid, int jb, bool store = true) {
   T* Ts = sm.Ts;
   T* rs = sm.rs;
   T* U = Ts + LD;
@@ -427,11 +442,12 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
     }
     __syncthreads();
     // columns K0..K0+15 of L are final: store them now, overlapping the remaining work
+    if (store)
 #pragma unroll
-    for (int e = t; e < 16 * NB; e += 256) {
-      const int i = e & 63, j = K0 + (e >> 6);
-      if (i < kb && j < kb && i >= j) blk[(i64)j * ld + i] = Ts[j * LD + i];
-    }
+      for (int e = t; e < 16 * NB; e += 256) {
+        const int i = e & 63, j = K0 + (e >> 6);
+        if (i < kb && j < kb && i >= j) blk[(i64)j * ld + i] = Ts[j * LD + i];
+      }
     // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
     for (int p = w; p < 6; p += 4) {
       int cnt = 0, bi = -1, bj = -1;
@@ -474,11 +490,12 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
       for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
     __syncthreads();
   }
+  if (store)
 #pragma unroll
-  for (int e = t; e < NB * NB; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    dinv[e] = r >= c ? U[r * LD + c] : T(0);  // column-major, zero above the diagonal
-  }
+    for (int e = t; e < NB * NB; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      dinv[e] = r >= c ? U[r * LD + c] : T(0);  // column-major, zero above the diagonal
+    }
 }
 
 template <class T>
@@ -499,6 +516,92 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
     sm.Ts[j * LD + i] = v;
   }
   diag_core(sm, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
+}
+
+// Right-looking levels: the diagonal block's factorization and the panel below it in one
+// launch, one workgroup per 64-row tile of rows [jb + kb, N) (at least one per front).  Every
+// workgroup factors the diagonal block in its own LDS -- the same input and the same arithmetic,
+// so the copies agree bit for bit -- and applies the inverse to its row tile.  The front's last
+// workgroup also stores L_jj and Linv_jj, after every workgroup of the front has loaded the
+// block: each one counts itself in arrive[front] once its loads are done, and the last one
+// (dispatched after the others, so the wait is short and always ends) waits for the full count,
+// then resets it for the next launch.
+template <class T>
+__global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                     T* __restrict__ A, int jb, int xt, int nf,
+                                                     int* __restrict__ arrive, int* __restrict__ err) {
+  int tile, fi;
+  if (!wg_tile(xt, nf, tile, fi)) return;
+  const int fid = ids[fi];
+  const FrontDev F = fr[fid];
+  if (jb >= F.n) return;
+  const int kb = min(NB, F.n - jb), s0 = jb + kb;
+  const int nt = (F.N - s0 + NB - 1) / NB, nw = max(nt, 1);
+  if (tile >= nw) return;
+  __shared__ DiagSmem<T> sd;
+  __shared__ T X[NB * (NB + 1)];
+  const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
+  const int w = wave_id(), wr = w >> 1, wc = w & 1;
+  T* blk = A + F.off + (i64)jb * F.N + jb;
+  const int row0 = s0 + NB * tile, nrow = tile < nt ? min(NB, F.N - row0) : 0;
+  T* pan = A + F.off + (i64)jb * F.N + row0;
+#pragma unroll
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    T v = T(0);  // strict upper part: the inverse, initially zero
+    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
+    sd.Ts[j * LD + i] = v;
+    X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] : T(0);
+  }
+  __syncthreads();  // every load of this workgroup has returned
+  const bool writer = tile == nw - 1;
+  if (t == 0) {
+    __hip_atomic_fetch_add(arrive + fid, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (writer) {
+      int spins = 0;
+      while (__hip_atomic_load(arrive + fid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != nw) {
+        if (++spins > SPIN_LIMIT) {
+          atomicCAS(err + 3, 0, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __hip_atomic_store(arrive + fid, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  diag_core(sd, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb, writer);
+  __syncthreads();
+  if (nrow == 0) return;
+  // Y = X Linv^T with Linv(j, k) = U[j*LD + k] for k <= j (the entries above are L's)
+  const T* U = sd.Ts + LD;
+  V4<T> acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+#pragma unroll
+  for (int k4 = 0; k4 < NB / 4; k4++) {
+    const int k = k4 * 4 + q;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+      const T bv = X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const int j = wc * 32 + bj * 16 + c16;
+        const T av = k <= j ? U[j * LD + k] : T(0);
+        acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+      }
+    }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
+      }
 }
 
 template <class T>
@@ -990,7 +1093,6 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
 // A workgroup only waits for blocks owned by workgroups with smaller ids, which the hardware
 // dispatches first, and every wait is bounded: past the limit err[3] is set and the
 // workgroup goes on, so the launch always drains.
-#define SPIN_LIMIT (1 << 22)
 // The flag is polled with relaxed agent-scope loads (no cache maintenance per poll); one
 // acquire fence follows the successful poll.  Publication: every wave's stores are released at
 // agent scope before the barrier, then one relaxed store sets the flag.

@@ -93,6 +93,7 @@ struct umf_plan {
   double* d_val = nullptr;
   int* d_err = nullptr;
   int* d_flags = nullptr;  // block flags of the blocked solve sweeps
+  int* d_arrive = nullptr;  // per-front arrival counters of k_chol_dpanel (zero between launches)
   int n_flags = 0;
   // coefficient-change requests: per-DOF change flags (epoch-tagged), the collected list
   int* d_bflag = nullptr;
@@ -326,8 +327,8 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
           zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
           ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
         }
-        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, arena);
+        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>, dim3(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena);
         launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off,
                (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
                (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
@@ -386,9 +387,17 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
           }
         }
       }
-      launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
-             arena, k0, p->d_err);
-      if (rows > 0) {
+      // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
+      // repeats the diagonal factorization in each) run diagonal block and panel in one launch
+      const int ptd = std::max(1, (rows + NB - 1) / NB);
+      const bool fused = !left && (i64)ptd * L.f_cnt <= 512;
+      if (fused)
+        launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, L.f_cnt),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? ptd : 0, L.f_cnt, p->d_arrive, p->d_err);
+      else
+        launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+               arena, k0, p->d_err);
+      if (!fused && rows > 0) {
         const int pt = (rows + NB - 1) / NB;
         launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, L.f_cnt),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt,
@@ -484,6 +493,7 @@ int check_err(umf_plan* p) {
     g_err = buf;
     return 1 + h[2];
   }
+  if (h[3]) return fail(UMF_EDEVICE, "factorization: a workgroup wait exceeded its spin limit (results invalid)");
   return UMF_OK;
 }
 
@@ -816,7 +826,7 @@ void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
                   p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist,
-                  p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off};
+                  p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off, p->d_arrive};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
@@ -980,6 +990,10 @@ int setup_device(umf_plan* p, umf_plan** out) {
   if ((he = hipMalloc((void**)&p->d_flags, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMemset(p->d_flags, 0, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMalloc((void**)&p->d_arrive, sizeof(int) * std::max(P.nfronts(), 1))) != hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMemset(p->d_arrive, 0, sizeof(int) * std::max(P.nfronts(), 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   double fl = 0;
   for (int f = 0; f < P.nfronts(); f++) fl += P.front_flops(f);
