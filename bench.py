@@ -30,6 +30,20 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
+    summary of this workload (profiles/<kernel>_traffic_<workload>.md, calibrated on a plain
+    copy), or None when there is none: counters cannot be read inside the timed run."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"^bytes per launch:\s*([0-9.eE+-]+)\s*$", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 # workloads: name -> (dims, maxSideLength, update box side, update scale or None for one LN(0,2)
 # factor, seed); the first is the headline, the 3D ones are scale runs of the same step
 WORKLOADS = {
@@ -382,7 +396,8 @@ def main():
         if d["flops"] > 0:
             ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
             roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+                    "frac": ach / PEAK_FP64_TFLOPS,
+                    "traffic": measured_traffic(dom, args.workload)}
         else:
             ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
             roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
