@@ -1268,21 +1268,37 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 }
 
 // Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.
+// One workgroup per 64 rows, transposed through LDS so both sides are read and written in
+// contiguous runs (64 rows of a column; 64 x 16 consecutive work-block entries).
 template <class T>
 __global__ __launch_bounds__(256) void k_to_rows(int n, const double* __restrict__ B, int ld, int nr,
                                                  T* __restrict__ W) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (i64)n * RB) return;
-  const int i = (int)(e >> 4), r = (int)(e & 15);
-  W[e] = r < nr ? (T)B[(i64)r * ld + i] : T(0);
+  __shared__ double s[64][RB + 1];
+  const int i0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 64 * RB; e += 256) {
+    const int r = e >> 6, li = e & 63;
+    s[li][r] = (r < nr && i0 + li < n) ? B[(i64)r * ld + i0 + li] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * RB; e += 256) {
+    const int li = e / RB, r = e % RB;
+    if (i0 + li < n) W[(i64)(i0 + li) * RB + r] = (T)s[li][r];
+  }
 }
 template <class T>
 __global__ __launch_bounds__(256) void k_from_rows(int n, const T* __restrict__ W, double* __restrict__ B,
                                                    int ld, int nr) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (i64)n * nr) return;
-  const int r = (int)(e / n), i = (int)(e % n);
-  B[(i64)r * ld + i] = (double)W[(i64)i * RB + r];
+  __shared__ double s[64][RB + 1];
+  const int i0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 64 * RB; e += 256) {
+    const int li = e / RB, r = e % RB;
+    s[li][r] = i0 + li < n ? (double)W[(i64)(i0 + li) * RB + r] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * nr; e += 256) {
+    const int r = e >> 6, li = e & 63;
+    if (i0 + li < n) B[(i64)r * ld + i0 + li] = s[li][r];
+  }
 }
 // Element-wise conversion between the FP64 and the storage-type work blocks.
 template <class To, class From>

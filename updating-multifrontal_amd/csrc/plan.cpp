@@ -906,6 +906,7 @@ void merge_plans(Plan& M, std::vector<Plan>&& subs) {
   }
   M.tri_off[order.size()] = (i64)M.triples.size();
   M.tile_off[order.size()] = (i64)M.tiles.size();
+  build_fly(M);  // the merged leaf fronts' entries by panel tile (front-local rows, merged CSR)
   // the problems' own plans keep only what their dirty sets need (grid, sizes, level lists)
   for (Plan& P : S) {
     std::vector<int>().swap(P.rp);

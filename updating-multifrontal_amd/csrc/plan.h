@@ -109,7 +109,7 @@ struct Plan {
   // Leaf fronts (assembled from CSR entries only) at left-looking levels skip the dense zeroing:
   // the entries below a column block's diagonal block go straight into the panel step that
   // forms that tile (fly_*), the rest (diagonal blocks, the trailing boundary block) are zeroed
-  // and scattered as usual (rest_*).  Empty for a batch.
+  // and scattered as usual (rest_*).
   std::vector<int> fly_off;      // per front: its table in fly_tab, -1 = not such a front
   std::vector<int> fly_tab;      // per front: for column block cb, row tile t below it:
                                  // fly_tri[fly_tab[off + cb*(NT+1) + t] .. +1], NT = ceil(N/64)

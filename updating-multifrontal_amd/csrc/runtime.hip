@@ -686,7 +686,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   const i64 nnz = (i64)p->P.ci.size();
   launch(p, "k_sumsq", 0, 8.0 * nnz, k_sumsq, dim3(DOT_BLOCKS), nnz, (const double*)p->d_val, p->d_part);
   launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
-  launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3(g16), n, in, ld, nr, Bv);
+  launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3((n + 63) / 64), n, in, ld, nr, Bv);
   HIPCHK(hipMemsetAsync(X, 0, sizeof(double) * n16, p->stream));
   HIPCHK(hipMemcpyAsync(R, Bv, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
   dots(p, Bv, Bv, nullptr, nullptr, S2);
@@ -728,8 +728,8 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
            (const double*)QZ, Pd);
     std::swap(rz, rzn);
   }
-  launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3((unsigned)(((i64)n * nr + 255) / 256)), n,
-         (const double*)X, out, ld, nr);
+  launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3((n + 63) / 64), n, (const double*)X, out, ld,
+         nr);
   it_out = it;
   berr_out = 0;
   for (int c = 0; c < nr; c++) berr_out = std::max(berr_out, berr[c]);
@@ -766,7 +766,7 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   double berr = 0;
   for (int c0 = 0; c0 < nrhs; c0 += RB) {
     int nr = std::min(RB, nrhs - c0);
-    unsigned gb = (unsigned)(((i64)n * RB + 255) / 256), go = (unsigned)(((i64)n * nr + 255) / 256);
+    const unsigned gb = (unsigned)((n + 63) / 64), go = gb;  // 64 rows per workgroup
     const double* ci = d_in + (size_t)c0 * ldd;
     double* co = d_out + (size_t)c0 * ldd;
     if (!f32) {
