@@ -130,6 +130,15 @@ int umf_spmv_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, in
 int umf_box_scale_device(umf_plan* plan, double* d_coeff, double h, const int* lo, int side, double factor,
                          double* d_val, int* mod_dofs, int cap, int* n_mod);
 
+/* Plan files (no reference counterpart; the reference keeps a factorization only in memory):
+ * umf_plan_save writes a plan's geometry, operator (its current values) and, when factored, its
+ * factors in the precision they are held in; umf_plan_load rebuilds the plan from the file and
+ * restores the factors without refactoring, so Solve / Apply / Update can follow at once.  For
+ * reproducible inputs and warm starts of a serving process.  Single-problem plans only; the file
+ * is specific to this library's build (its front-pool layout is checked on load). */
+int umf_plan_save(umf_plan* plan, const char* path);
+int umf_plan_load(const char* path, umf_plan** out);
+
 /* Last error message of the calling thread. */
 const char* umf_last_error(void);
 

@@ -99,3 +99,17 @@ def test_csr_file_operator_factors_like_the_array_one(tmp_path):
         for b in range(G1.nboxes(ell)):
             for w in (0, 1):
                 assert np.array_equal(G1.box_dofs(ell, b, w), G2.box_dofs(ell, b, w))
+
+
+def test_plan_file_refuses_other_files(tmp_path):
+    """umf.load_plan / umf_plan_load check the file before any device work."""
+    p = tmp_path / "x.umf"
+    p.write_bytes(b"not a plan file at all" * 4)
+    with pytest.raises(ValueError, match="not a plan file"):
+        umf.load_plan(str(p))
+    # right magic, header cut short: refused by the library's own check
+    import struct
+    head = b"UMFPLAN1" + struct.pack("<12i", 2, 31, 31, 1, 1, 3, 961, 0, 0, 0, 50, 0)
+    p.write_bytes(head + struct.pack("<2q", 4805, 0) + struct.pack("<d", 1e-10) + b"\0" * 16)
+    with pytest.raises(ValueError, match="truncated"):
+        umf.load_plan(str(p))

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -71,6 +72,7 @@ struct Prof {
 
 struct umf_plan {
   Plan P;
+  int args[6] = {0, 0, 0, 0, 0, 0};  // dim, nx, ny, nz, width, max_side_length it was made with
   Prof prof;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -903,6 +905,8 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
     return fail(geom ? UMF_EGEOM : UMF_EINVAL, e);
   }
   p->val_h.assign(val, val + row_ptr[n]);
+  const int a[6] = {dim, nx, ny, nz, width, max_side_length};
+  std::copy(a, a + 6, p->args);
   return setup_device(p, out);
 }
 
@@ -987,6 +991,7 @@ int setup_device(umf_plan* p, umf_plan** out) {
   UP(p->d_val, p->val_h);
 #undef UP
   if ((he = hipMalloc((void**)&p->d_err, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMemset(p->d_err, 0, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMalloc((void**)&p->d_flags, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMemset(p->d_flags, 0, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
@@ -1108,6 +1113,118 @@ int umf_plan_destroy(umf_plan* p) {
   if (!p) return UMF_OK;
   free_plan(p);
   delete p;
+  return UMF_OK;
+}
+
+// ---- plan files: header, CSR pattern, current values, then (factored plans) the front pool
+// of the precision the factors are held in, entry by entry as the device holds it.
+namespace {
+const char kPlanMagic[8] = {'U', 'M', 'F', 'P', 'L', 'A', 'N', '1'};
+struct PlanFileHead {
+  char magic[8];
+  int32_t args[6];  // dim, nx, ny, nz, width, max_side_length
+  int32_t n, factored, prec, fact_prec, ir_max, elem_size;  // elem_size: 8, 4, or 0 (no pool)
+  int64_t nnz, arena_elems;
+  double ir_tol;
+};
+const size_t kChunk = size_t(1) << 26;  // bytes staged per host<->device copy
+}  // namespace
+
+int umf_plan_save(umf_plan* p, const char* path) {
+  if (!p || !path) return fail(UMF_EINVAL, "null argument");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (it holds no values)");
+  if (p->P.forest) return fail(UMF_EINVAL, "batch plans cannot be saved");
+  const Plan& P = p->P;
+  PlanFileHead h{};
+  std::copy(kPlanMagic, kPlanMagic + 8, h.magic);
+  std::copy(p->args, p->args + 6, h.args);
+  h.n = P.n;
+  h.nnz = (int64_t)P.ci.size();
+  h.factored = p->factored ? 1 : 0;
+  h.prec = p->prec;
+  h.fact_prec = p->fact_prec;
+  h.ir_max = p->ir_max;
+  h.ir_tol = p->ir_tol;
+  h.arena_elems = P.arena_elems;
+  h.elem_size = !p->factored ? 0 : p->fact_prec == UMF_PREC_FP32 ? 4 : 8;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  std::vector<double> val((size_t)h.nnz);
+  if (h.nnz) HIPCHK(hipMemcpy(val.data(), p->d_val, sizeof(double) * h.nnz, hipMemcpyDeviceToHost));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(UMF_EINVAL, std::string("cannot open ") + path + " for writing");
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(P.rp.data(), sizeof(int), P.rp.size(), f) == P.rp.size() &&
+            std::fwrite(P.ci.data(), sizeof(int), P.ci.size(), f) == P.ci.size() &&
+            std::fwrite(val.data(), sizeof(double), val.size(), f) == val.size();
+  if (ok && h.elem_size) {
+    const char* src = h.elem_size == 4 ? (const char*)p->arena32 : (const char*)p->arena;
+    const size_t total = (size_t)h.arena_elems * h.elem_size;
+    std::vector<char> buf(std::min(total, kChunk));
+    for (size_t o = 0; ok && o < total; o += kChunk) {
+      const size_t b = std::min(kChunk, total - o);
+      if (hipMemcpy(buf.data(), src + o, b, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+      ok = ok && std::fwrite(buf.data(), 1, b, f) == b;
+    }
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(UMF_EDEVICE, std::string("writing ") + path + " failed");
+  return UMF_OK;
+}
+
+int umf_plan_load(const char* path, umf_plan** out) {
+  if (!path || !out) return fail(UMF_EINVAL, "null argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(UMF_EINVAL, std::string("cannot open ") + path);
+  PlanFileHead h{};
+  auto bad = [&](const std::string& why) {
+    std::fclose(f);
+    return fail(UMF_EINVAL, std::string(path) + ": " + why);
+  };
+  if (std::fread(&h, sizeof h, 1, f) != 1 || !std::equal(kPlanMagic, kPlanMagic + 8, h.magic))
+    return bad("not a plan file");
+  if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0)
+    return bad("corrupt header");
+  std::vector<int> rp((size_t)h.n + 1), ci((size_t)h.nnz);
+  std::vector<double> val((size_t)h.nnz);
+  if (std::fread(rp.data(), sizeof(int), rp.size(), f) != rp.size() ||
+      std::fread(ci.data(), sizeof(int), ci.size(), f) != ci.size() ||
+      std::fread(val.data(), sizeof(double), val.size(), f) != val.size())
+    return bad("truncated");
+  if (rp[h.n] != h.nnz) return bad("corrupt CSR");
+  umf_plan* p = nullptr;
+  int rc = umf_plan_create(h.args[0], h.args[1], h.args[2], h.args[3], h.args[4], h.args[5], h.n, rp.data(), ci.data(),
+                           val.data(), &p);
+  if (rc) {
+    std::fclose(f);
+    return rc;
+  }
+  auto drop = [&](int code, const std::string& why) {
+    std::fclose(f);
+    umf_plan_destroy(p);
+    return fail(code, std::string(path) + ": " + why);
+  };
+  if (h.arena_elems != p->P.arena_elems) return drop(UMF_EINVAL, "front pool size differs from this build's plan");
+  p->prec = h.prec;
+  p->ir_tol = h.ir_tol;
+  p->ir_max = h.ir_max;
+  if (h.elem_size) {
+    if (h.elem_size == 4 && !p->arena32 &&
+        hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)) != hipSuccess)
+      return drop(UMF_ENOMEM, "FP32 front pool allocation failed");
+    char* dst = h.elem_size == 4 ? (char*)p->arena32 : (char*)p->arena;
+    const size_t total = (size_t)h.arena_elems * h.elem_size;
+    std::vector<char> buf(std::min(total, kChunk));
+    for (size_t o = 0; o < total; o += kChunk) {
+      const size_t b = std::min(kChunk, total - o);
+      if (std::fread(buf.data(), 1, b, f) != b) return drop(UMF_EINVAL, "truncated front pool");
+      if (hipMemcpy(dst + o, buf.data(), b, hipMemcpyHostToDevice) != hipSuccess)
+        return drop(UMF_EDEVICE, "front pool upload failed");
+    }
+    p->fact_prec = h.fact_prec;
+    p->factored = h.factored != 0;
+  }
+  std::fclose(f);
+  *out = p;
   return UMF_OK;
 }
 

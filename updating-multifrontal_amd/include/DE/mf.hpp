@@ -84,11 +84,29 @@ class Front {
     return Y;
   }
 
+  // Plan files (no reference counterpart): geometry, current operator values and, when
+  // factored, the factors; the file constructors of MF2D / MF3D restore them without
+  // refactoring (umf_plan_save / umf_plan_load).
+  void Save(const std::string& path) { check(umf_plan_save(plan_, path.c_str())); }
+
   umf_plan* plan() { return plan_; }
   umf_stats stats() const {
     umf_stats s;
     check(umf_get_stats(plan_, &s));
     return s;
+  }
+
+ protected:
+  Front(const std::string& planFile, int dim) : n_(0) {
+    check(umf_plan_load(planFile.c_str(), &plan_));
+    const bool is3d = umf_leaf_width(plan_, 2) >= 0;
+    if (is3d != (dim == 3)) {
+      umf_plan_destroy(plan_);
+      plan_ = nullptr;
+      throw std::invalid_argument(planFile + ": the plan is " + (is3d ? "3D" : "2D"));
+    }
+    const int rc = umf_batch_info(plan_, 0, nullptr, &n_, nullptr, nullptr);
+    check(rc < 0 ? rc : UMF_OK);
   }
 
  private:
@@ -104,6 +122,7 @@ class MF2D : public umf_detail::Front<Scalar> {
  public:
   MF2D(const LinAlg::Sparse<Scalar>& A, int Nx, int Ny, int width = 1, int maxSideLength = 4)
       : umf_detail::Front<Scalar>(A, 2, Nx, Ny, 1, width, maxSideLength) {}
+  explicit MF2D(const std::string& planFile) : umf_detail::Front<Scalar>(planFile, 2) {}
 };
 }  // namespace TreeFactor2D
 
@@ -113,6 +132,7 @@ class MF3D : public umf_detail::Front<Scalar> {
  public:
   MF3D(const LinAlg::Sparse<Scalar>& A, int Nx, int Ny, int Nz, int width = 1, int maxSideLength = 4)
       : umf_detail::Front<Scalar>(A, 3, Nx, Ny, Nz, width, maxSideLength) {}
+  explicit MF3D(const std::string& planFile) : umf_detail::Front<Scalar>(planFile, 3) {}
 };
 }  // namespace TreeFactor3D
 

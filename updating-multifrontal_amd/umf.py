@@ -67,6 +67,8 @@ _apply = _sig("umf_apply", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
 _solve_dev = _sig("umf_solve_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _apply_dev = _sig("umf_apply_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _spmv_dev = _sig("umf_spmv_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_save = _sig("umf_plan_save", C.c_int, [_vp, C.c_char_p])
+_load = _sig("umf_plan_load", C.c_int, [C.c_char_p, C.POINTER(_vp)])
 _box_scale_dev = _sig("umf_box_scale_device", C.c_int,
                       [_vp, _vp, C.c_double, _ip, C.c_int, C.c_double, _vp, _ip, C.c_int, _ip])
 _last_error = _sig("umf_last_error", C.c_char_p, [])
@@ -90,7 +92,7 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision",
-            "umf_batch_create", "umf_batch_info", "umf_box_scale_device"]
+            "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load"]
 
 
 def _i(a):
@@ -143,6 +145,11 @@ class _MF:
             _check(_create(len(dims), dims[0], dims[1], nz, width, maxSideLength, self.n, _i(rp), _i(ci),
                            _d(val), C.byref(h)))
         self._h = h
+
+    def save(self, path):
+        """Write the plan (geometry, current operator values, factors if factored) to `path`
+        (umf_plan_save); load_plan(path) restores it without refactoring."""
+        _check(_save(self._h, os.fsencode(path)))
 
     def analyze_update(self, modifiedDOFs):
         """Dirty sets an Update with these DOFs would factor (then read with dirty())."""
@@ -297,6 +304,23 @@ class MF3D(_MF):
 
     def __init__(self, A, Nx, Ny, Nz, width=1, maxSideLength=4, analyze_only=False):
         super().__init__(A, (Nx, Ny, Nz), width, maxSideLength, analyze_only)
+
+
+def load_plan(path):
+    """An MF2D / MF3D restored from a file written by save() (umf_plan_load): factored if it
+    was saved factored, in the same precision form."""
+    import struct
+    with open(path, "rb") as f:
+        head = f.read(8 + 4 * 12)
+    if len(head) < 8 + 4 * 12 or head[:8] != b"UMFPLAN1":
+        raise ValueError(f"{path}: not a plan file")
+    dim, nx, ny, nz, width, _maxs, n = struct.unpack("<7i", head[8:36])
+    h = _vp()
+    _check(_load(os.fsencode(path), C.byref(h)))
+    cls = MF2D if dim == 2 else MF3D
+    G = cls.__new__(cls)
+    G.n, G.dims, G.width, G._h = n, ((nx, ny) if dim == 2 else (nx, ny, nz)), width, h
+    return G
 
 
 class CSR:
