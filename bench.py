@@ -180,14 +180,37 @@ def backward_error(plan, d_B, d_X, d_R, vals, B, nrhs):
     return max(np.linalg.norm(R[j]) / (normA * np.linalg.norm(X[j]) + np.linalg.norm(B[j])) for j in range(nrhs))
 
 
-def fp32_form(plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
-    """The same step with FP32 fronts on FP32 MFMA and the 16-RHS Solve refined in FP64 to
-    backward error <= --ir-tol; same algorithmic flop numerator as the FP64 form."""
+FORM_DTYPE = {
+    "fp32": "f32 fronts (FP32 MFMA) + f64 refinement",
+    "bf16x3": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 2 bf16 pieces (3 products, f32 accumulate) "
+              "+ f64 refinement",
+    "bf16x6": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 3 bf16 pieces (6 products, f32 accumulate) "
+              "+ f64 refinement",
+}
+
+
+def low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
+    """The same step in a lower-precision form (FP32 fronts; GEMMs on FP32 MFMA or on BF16
+    pieces) with the Solve refined in FP64 to backward error <= --ir-tol; same algorithmic flop
+    numerator as the FP64 form."""
     import torch
-    plan.SetPrecision("fp32", tol=args.ir_tol, max_iter=50)
+    plan.SetPrecision(prec, tol=args.ir_tol, max_iter=100)
     plan.profile(False)
     for _ in range(max(args.warmup, 1)):
         step()
+    kern = {}
+    if not args.no_profile:  # untimed step with per-kernel events
+        plan.profile(True)
+        plan.profile_select(None)
+        plan.profile_reset()
+        step()
+        torch.cuda.synchronize()
+        for k, v in plan.profile_read().items():
+            d = kern.setdefault(k.split("@")[0], {"ms": 0.0, "launches": 0, "flops": 0.0})
+            for f in d:
+                d[f] += v[f]
+        plan.profile_reset()
+        plan.profile(False)
     tf = tu = ts = flops = 0.0
     iters = []
     torch.cuda.synchronize()
@@ -208,11 +231,14 @@ def fp32_form(plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
     st = plan.stats()
     be = backward_error(plan, d_B, d_X, d_R, vals[state["cur"]], B, nrhs)
     plan.SetPrecision("fp64")
+    progress(f"{prec} form: {secs / K * 1e3:.1f} ms per step, refinement iterations {iters}, berr {be:.1e}")
     return {"value": flops / secs / 1e12, "unit": "TFLOP/s", "ms_per_step": secs / K * 1e3,
             "t_factor_ms": tf / K, "t_update_ms": tu / K, "t_solve_ms": ts / K,
             "ir_iters": iters, "ir_tol": args.ir_tol, "backward_error": be, "ir_berr": st["ir_berr"],
-            "front_pool_gb": st["bytes_front_pool"] / 1e9,
-            "dtype": "f32 fronts (FP32 MFMA) + f64 refinement"}
+            "front_pool_gb": st["bytes_front_pool"] / 1e9, "dtype": FORM_DTYPE[prec],
+            "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in sorted(kern.items())},
+            "kernels_tflops": {k: round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 2)
+                               for k, v in sorted(kern.items()) if v["ms"] > 0 and v["flops"]}}
 
 
 def run_serve(args):
@@ -295,7 +321,9 @@ def main():
     ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (16; 4 for the batch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-MFMA + FP64-refinement form")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the lower-precision + FP64-refinement forms")
+    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp32", "bf16x3", "bf16x6"],
+                    help="lower-precision forms measured beside FP64 (each refined in FP64)")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
                     help="serve95: request arrival rates (per second)")
@@ -441,7 +469,8 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_fp32:
-        out["fp32_ir"] = fp32_form(plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
+        for prec in args.forms:
+            out[prec + "_ir"] = low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
     if not args.no_cpu_baseline and args.workload == "2d1023":
         out["cpu_baseline"] = cpu_baseline(nrhs=nrhs)
     print(json.dumps(out), flush=True)

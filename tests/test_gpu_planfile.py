@@ -67,6 +67,15 @@ def test_saved_fp32_form_and_unfactored_plan(tmp_path):
     H32 = umf.load_plan(p)
     assert np.array_equal(H32.Solve(b), x)
     assert H32.stats()["ir_iters"] == G.stats()["ir_iters"]
+    # the BF16-split form stores FP32 fronts too; its plan file restores the form
+    G.SetPrecision("bf16x3", tol=1e-12, max_iter=100)
+    G.Factor()
+    x = G.Solve(b)
+    G.save(p)
+    Hbf = umf.load_plan(p)
+    assert np.array_equal(Hbf.Solve(b), x)
+    Hbf.Factor()  # refactors in the restored form
+    assert np.array_equal(Hbf.Solve(b), x)
 
 
 def test_batch_plans_are_not_saved(tmp_path):

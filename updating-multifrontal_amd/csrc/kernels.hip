@@ -219,11 +219,38 @@ __device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __rest
   __syncthreads();
 }
 
+// Operand loads of the tile GEMMs are buffer loads: one resource per K chunk covers the chunk's
+// valid columns (base = the chunk's first column, size = its valid columns x ld), so columns at
+// or beyond K read as 0 without a branch, and each lane's byte offset within the chunk is fixed
+// for the whole loop.  Rows past the operand's end are clamped to its last row: their products
+// land only in accumulators outside the tile, which the epilogues skip.
+__device__ __forceinline__ const void* wave_uniform(const void* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (const void*)(((unsigned long long)hi << 32) | lo);
+}
+// buffer resource over `bytes` bytes at p (gfx9 raw buffer: stride 0, dword 3 = 0x00020000)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t chunk_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)wave_uniform(p), 0, __builtin_amdgcn_readfirstlane(bytes),
+                                           0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 // The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
 // (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
 // from LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
 // c+1 (loaded into registers one chunk earlier) is written to the other stage and chunk c+2 is
-// fetched, one barrier per 16-deep chunk.
+// fetched, one barrier per 16-deep chunk.  P and Q start at the tile's first row; pn, qn >= 1.
 template <class T, int TS>
 struct TileSmemT {
   T As[2][16][TS];
@@ -235,22 +262,31 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
                                                int pn, const T* __restrict__ Q, int ldq, int qn, int K,
                                                TileSmemT<T, TS>& sm) {
   constexpr int NL = TS / 64, NBK = TS / 32;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = w >> 1, wc = w & 1;
-  const int li = t & 63, lk = t >> 6;
+  const int li = lane, lk = w;
   T pa[4][NL], qb[4][NL];
-  auto fetch = [&](int kc) {
+  int po[4][NL], qo[4][NL];  // byte offsets of this lane's elements within a chunk
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = kc + lk + 4 * q;
-      const bool kin = k < K;
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+      const int row = li + 64 * u;
+      po[q][u] = ((lk + 4 * q) * ldp + min(row, pn - 1)) * (int)sizeof(T);
+      qo[q][u] = ((lk + 4 * q) * ldq + min(row, qn - 1)) * (int)sizeof(T);
+    }
+  auto fetch = [&](int kc) {
+    const int nk = max(min(16, K - kc), 0);
+    const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * (int)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * (int)sizeof(T));
+#pragma unroll
+    for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int u = 0; u < NL; u++) {
-        const int row = li + 64 * u;
-        pa[q][u] = (kin && row < pn) ? P[(i64)k * ldp + row] : T(0);
-        qb[q][u] = (kin && row < qn) ? Q[(i64)k * ldq + row] : T(0);
+        pa[q][u] = buf_load<T>(rp, po[q][u]);
+        qb[q][u] = buf_load<T>(rq, qo[q][u]);
       }
-    }
   };
   auto stage = [&](int b) {
 #pragma unroll
@@ -265,7 +301,7 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
   __syncthreads();  // the stages may still be read by an earlier use of the shared memory
   fetch(0);
   stage(0);
-  if (16 < K) fetch(16);
+  fetch(16);
   __syncthreads();
   int b = 0;
   for (int kc = 0; kc < K; kc += 16, b ^= 1) {
@@ -283,9 +319,131 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
 #pragma unroll
         for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
     }
-    if (kc + 16 < K) {
-      stage(b ^ 1);
-      if (kc + 32 < K) fetch(kc + 32);
+    stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 32);
+    __syncthreads();
+  }
+}
+
+// BF16-split variant of tile_gemm_nt_t for FP32 operands (the split precision form).  Each FP32
+// value x is written to LDS as NPC bfloat16 pieces, x ~ x_0 + x_1 + ... (x_0 = bf16(x),
+// x_1 = bf16(x - x_0), ...; each difference is exact in FP32), and P Q^T is formed on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation from the piece products whose indices sum
+// to < NPC, the smallest first: NPC = 2 takes x_0 y_0 + x_0 y_1 + x_1 y_0 (3 products, about
+// 16 significant bits per product), NPC = 3 the 6 products of the three pieces (about FP32's
+// 24).  bfloat16 has FP32's exponent range, so the pieces need no scaling.
+// The instruction's operand layout: lane l supplies row l%16 of A (column l%16 of B) with 8
+// consecutive k of group l/16 of the 32-deep step, one 16-byte LDS read; accumulator r of lane
+// l holds D[4 (l/16) + r][l%16], the layout of v_mfma_f32_16x16x4.  LDS rows hold 32 k as four
+// 16-byte groups plus one group of padding (80 B), so the 16-byte reads of 16 consecutive rows
+// fall on distinct banks.  Two LDS stages, one barrier per 32-deep chunk, as in tile_gemm_nt_t;
+// the loop body is one basic block, and scheduling groups interleave the MFMAs with the next
+// chunk's split, LDS writes and buffer loads.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ V4<float> mfma_bf(bf16x8 a, bf16x8 b, V4<float> c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int TS, int NPC>
+struct TileSmemBF {
+  bf16x8 P[2][NPC][TS][5];
+  bf16x8 Q[2][NPC][TS][5];
+};
+
+template <int NPC>
+__device__ __forceinline__ void split_bf(const float (&x)[8], bf16x8 (&pc)[NPC]) {
+  float r[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) r[j] = x[j];
+#pragma unroll
+  for (int p = 0; p < NPC; p++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const __bf16 h = (__bf16)r[j];
+      pc[p][j] = h;
+      r[j] -= (float)h;
+    }
+}
+
+template <int TS, int NPC>
+__device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 32], const float* __restrict__ P,
+                                                int ldp, int pn, const float* __restrict__ Q, int ldq, int qn, int K,
+                                                TileSmemBF<TS, NPC>& sm) {
+  constexpr int NL = TS / 64, NBK = TS / 32;
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wr = w >> 1, wc = w & 1;
+  const int li = lane, kg = w;  // staging: row li (+64u), k = 8 kg .. 8 kg + 7 of the chunk
+  float pa[NL][8], qb[NL][8];
+  int po[NL][8], qo[NL][8];
+#pragma unroll
+  for (int u = 0; u < NL; u++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int row = li + 64 * u;
+      po[u][j] = ((8 * kg + j) * ldp + min(row, pn - 1)) * 4;
+      qo[u][j] = ((8 * kg + j) * ldq + min(row, qn - 1)) * 4;
+    }
+  auto fetch = [&](int kc) {
+    const int nk = max(min(32, K - kc), 0);
+    const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * 4);
+    const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * 4);
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+#pragma unroll
+      for (int u = 0; u < NL; u++) {
+        pa[u][j] = buf_load<float>(rp, po[u][j]);
+        qb[u][j] = buf_load<float>(rq, qo[u][j]);
+      }
+  };
+  auto stage = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+      bf16x8 pp[NPC], qp[NPC];
+      split_bf<NPC>(pa[u], pp);
+      split_bf<NPC>(qb[u], qp);
+#pragma unroll
+      for (int p = 0; p < NPC; p++) {
+        sm.P[b][p][li + 64 * u][kg] = pp[p];
+        sm.Q[b][p][li + 64 * u][kg] = qp[p];
+      }
+    }
+  };
+  if (K <= 0) return;
+  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
+  fetch(0);
+  stage(0);
+  fetch(32);
+  __syncthreads();
+  const int q = lane >> 4, c16 = lane & 15;
+  int b = 0;
+  for (int kc = 0; kc < K; kc += 32, b ^= 1) {
+    bf16x8 av[NPC][NBK], bv[NPC][NBK];
+#pragma unroll
+    for (int p = 0; p < NPC; p++)
+#pragma unroll
+      for (int x = 0; x < NBK; x++) {
+        bv[p][x] = sm.P[b][p][wr * (TS / 2) + x * 16 + c16][q];
+        av[p][x] = sm.Q[b][p][wc * (TS / 2) + x * 16 + c16][q];
+      }
+#pragma unroll
+    for (int s = NPC - 1; s >= 0; s--)
+#pragma unroll
+      for (int i = 0; i <= s; i++)
+#pragma unroll
+        for (int bi = 0; bi < NBK; bi++)
+#pragma unroll
+          for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma_bf(av[i][bj], bv[s - i][bi], acc[bi][bj]);
+    stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 64);
+    constexpr int NM = NBK * NBK * NPC * (NPC + 1) / 2;  // MFMAs per chunk
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, NM / 8, 0);         // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x002, 12, 0);             // VALU
+      __builtin_amdgcn_sched_group_barrier(0x200, NPC == 2 ? 1 : 2, 0);  // LDS write
+      __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);              // buffer load
     }
     __syncthreads();
   }
@@ -785,9 +943,12 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         kw = group width, ccap = n: the group's trailing update).
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-template <class T, int MODE, int TS = 64>
-__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+// SPLIT = 0: the native MFMA of T; SPLIT = 2 or 3 (T = float): the product on BF16 pieces
+// (tile_gemm_nt_bf with NPC = SPLIT).
+template <class T, int MODE, int TS = 64, int SPLIT = 0>
+__global__ __launch_bounds__(256, TS == 128 && SPLIT < 3 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                               T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
+  static_assert(SPLIT == 0 || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
   const FrontDev F = fr[ids[fi]];
@@ -805,15 +966,22 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
   }
   int row0, col0;
   if (!tile_of<TS>(tile, s, rend, cend, row0, col0)) return;
+  row0 = __builtin_amdgcn_readfirstlane(row0);  // wave-uniform: scalar operand addresses
+  col0 = __builtin_amdgcn_readfirstlane(col0);
   constexpr int NBK = TS / 32;
-  __shared__ TileSmemT<T, TS> sm;
   V4<T> acc[NBK][NBK];
 #pragma unroll
   for (int i = 0; i < NBK; i++)
 #pragma unroll
     for (int j = 0; j < NBK; j++) acc[i][j] = vzero<T>();
   const T* base = A + F.off + (i64)kbase * F.N;
-  tile_gemm_nt_t<T, TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+  if constexpr (SPLIT == 0) {
+    __shared__ TileSmemT<T, TS> sm;
+    tile_gemm_nt_t<T, TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+  } else {
+    __shared__ TileSmemBF<TS, SPLIT> sm;
+    tile_gemm_nt_bf<TS, SPLIT>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+  }
   T* C = A + F.off;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll

@@ -1,6 +1,6 @@
 // mfma_ceiling.hip -- register-only throughput of the instructions the kernels use on this GPU:
-// v_mfma_f64_16x16x4, v_mfma_f32_16x16x4 (independent accumulator chains, no memory) and FP64
-// vector FMA, at 1-8 waves per SIMD.  The attainable rates set the compute roofline the
+// v_mfma_f64_16x16x4, v_mfma_f32_16x16x4, v_mfma_f32_16x16x32_bf16 (independent accumulator
+// chains, no memory) and FP64 vector FMA, at 1-8 waves per SIMD.  The attainable rates set the compute roofline the
 // kernels are judged against beside the published peaks.
 #include <hip/hip_runtime.h>
 #include <cstdio>
@@ -27,6 +27,24 @@ __global__ __launch_bounds__(256) void k32(float* out, int iters) {
   for (int it = 0; it < iters; it++) {
 #pragma unroll
     for (int i = 0; i < NA; i++) acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+  }
+  float s = 0;
+  for (int i = 0; i < NA; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+template <int NA>
+__global__ __launch_bounds__(256) void kbf(float* out, int iters) {
+  f4 acc[NA];
+  for (int i = 0; i < NA; i++) acc[i] = (f4){0, 0, 0, 0};
+  bf8 a, b;
+  for (int j = 0; j < 8; j++) {
+    a[j] = (__bf16)(1.0f + (threadIdx.x + j) * 1e-3f);
+    b[j] = (__bf16)(1.0f - (threadIdx.x + j) * 1e-3f);
+  }
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int i = 0; i < NA; i++) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
   }
   float s = 0;
   for (int i = 0; i < NA; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
@@ -68,6 +86,10 @@ int main() {
     hipEventElapsedTime(&ms, e0, e1);
     fl = (double)nb * 4 * iters * 4 * 2048.0;
     printf("f32 mfma 16x16x4, %d waves/SIMD, 4 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
+    hipEventRecord(e0); hipLaunchKernelGGL(kbf<4>, dim3(nb), dim3(256), 0, 0, (float*)d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    fl = (double)nb * 4 * iters * 4 * 16384.0;
+    printf("bf16 mfma 16x16x32, %d waves/SIMD, 4 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
     hipEventRecord(e0); hipLaunchKernelGGL(kfma, dim3(nb), dim3(256), 0, 0, d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
     hipEventElapsedTime(&ms, e0, e1);
     fl = (double)nb * 256 * iters * 8 * 2.0;

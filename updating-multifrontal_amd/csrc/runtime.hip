@@ -260,8 +260,19 @@ inline int tiles_of(int s, int rend, int cend, int ts = 64) {
 }
 
 // Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
+// k_gemm instance of a stage: the native MFMA of T, or for FP32 fronts with split = 2 / 3 the
+// product on that many BF16 pieces
+template <class T, int MODE, int TS>
+auto gemm_fn(int split) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (split == 2) return k_gemm<T, MODE, TS, 2>;
+    if (split == 3) return k_gemm<T, MODE, TS, 3>;
+  }
+  return k_gemm<T, MODE, TS, 0>;
+}
+
 template <class T>
-int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
+int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, int split) {
   const Plan& P = p->P;
   Sched S = make_sched(P, D);
   int rc = ensure_ids(p, S.ids.size());
@@ -346,6 +357,12 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
     // FP32: 128 x 128 tiles for a wide update (half the operand traffic) when it still yields
     // 2048+ tiles; in FP64 the 128-tile kernel holds 2 waves per SIMD and measured slower
     const bool f32 = sizeof(T) == 4;
+    // FP32 fronts block the trailing update on two levels: a finished group of GW columns
+    // updates only the rest of its outer group of GW2 columns (K = GW), and the finished outer
+    // group updates the trailing matrix once with K = GW2.  The FP32 and BF16 products are
+    // bound by operand and C traffic at K = GW; K = GW2 quarters the C read-modify-write per
+    // flop.  FP64 runs at its MFMA ceiling with K = GW (GW2 = GW: one level).
+    const int GW2 = f32 ? 4 * GW : GW;
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     const bool xcd = L.f_cnt >= 64;
@@ -356,9 +373,11 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
     };
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       const int g0 = k0 / GW * GW, gend = g0 + GW;
+      const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
       const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
-      int rows = 0, tin = 0, ttr = 0, ttr128 = 0;
-      double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0;
+      const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
+      int rows = 0, tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
+      double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
@@ -378,14 +397,24 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
           fin += 2.0 * kb * e;
           bin += 16.0 * e + 8.0 * kb * (nr + ce - s0);
         }
-        if (group_done) {
-          const int kg = std::min(GW, F.n - g0), sg = g0 + kg;
-          if (sg < F.n) {
-            ttr = std::max(ttr, tiles_of(sg, N, F.n));
-            ttr128 = std::max(ttr128, tiles_of(sg, N, F.n, 128));
-            const double e = lower_elems(sg, N, F.n);
+        if (group_done) {  // the group's update of the rest of its outer group
+          const int kg = std::min(GW, F.n - g0), sg = g0 + kg, ce = std::min(F.n, hend);
+          if (sg < ce) {
+            ttr = std::max(ttr, tiles_of(sg, N, ce));
+            ttr128 = std::max(ttr128, tiles_of(sg, N, ce, 128));
+            const double e = lower_elems(sg, N, ce);
             ftr += 2.0 * kg * e;
-            btr += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
+            btr += 16.0 * e + 8.0 * kg * (N - sg + ce - sg);
+          }
+        }
+        if (outer_done) {  // the outer group's trailing update
+          const int kg = std::min(GW2, F.n - h0), sg = h0 + kg;
+          if (sg < F.n) {
+            tou = std::max(tou, tiles_of(sg, N, F.n));
+            tou128 = std::max(tou128, tiles_of(sg, N, F.n, 128));
+            const double e = lower_elems(sg, N, F.n);
+            fou += 2.0 * kg * e;
+            bou += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
           }
         }
       }
@@ -406,16 +435,18 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
                fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
       }
       if (tin > 0)
-        launch(p, "k_gemm_group", fin, bin, k_gemm<T, 0>, xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
+        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, L.f_cnt);
-      if (ttr > 0) {
-        if (f32 && (i64)ttr128 * L.f_cnt >= 2048)
-          launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0, 128>, dim3(ttr128, L.f_cnt), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, 0, L.f_cnt);
+      auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
+        if (f32 && (i64)t128 * L.f_cnt >= 2048)
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, L.f_cnt), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, L.f_cnt);
         else
-          launch(p, "k_gemm_trail", ftr, btr, k_gemm<T, 0>, xcd ? dim3(xcd_grid(ttr)) : dim3(ttr, L.f_cnt),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, g0, GW, 0x7fffffff, xcd ? ttr : 0, L.f_cnt);
-      }
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, L.f_cnt),
+                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, L.f_cnt);
+      };
+      if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
+      if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
     }
     // 4. Schur complement into the trailing block
     if (L.maxm > 0) {
@@ -431,11 +462,11 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
       }
       if (tiles > 0) {
         if (sizeof(T) == 4 && (i64)tiles128 * L.f_cnt >= 2048)
-          launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1, 128>,
+          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split),
                  xcd ? dim3(xcd_grid(tiles128)) : dim3(tiles128, L.f_cnt), ids + L.f_off,
                  (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles128 : 0, L.f_cnt);
         else
-          launch(p, "k_gemm_schur", fl, by, k_gemm<T, 1>, xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
+          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 64>(split), xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
                  ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles : 0, L.f_cnt);
       }
     }
@@ -455,13 +486,16 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena) {
 
 // Factor over a dirty schedule in the precision the plan is set to (the FP32 front pool is
 // allocated on first use).
+// BF16 pieces per FP32 operand in the factorization's GEMMs of a precision form (0: none)
+int split_of(int prec) { return prec == UMF_PREC_BF16X3 ? 2 : prec == UMF_PREC_BF16X6 ? 3 : 0; }
+
 int factor_any(umf_plan* p, const Plan::Dirty& D) {
-  if (p->prec == UMF_PREC_FP32) {
+  if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
-    return run_factor<float>(p, D, p->arena32);
+    return run_factor<float>(p, D, p->arena32, split_of(p->prec));
   }
-  return run_factor<double>(p, D, p->arena);
+  return run_factor<double>(p, D, p->arena, 0);
 }
 
 // Copy cnt front-pool entries at offset off of the current factors to the host as double.
@@ -747,7 +781,7 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   int rc = ensure_solve_scratch(p);
   if (rc) return rc;
   int n = p->P.n;
-  const bool f32 = p->fact_prec == UMF_PREC_FP32;
+  const bool f32 = p->fact_prec != UMF_PREC_FP64;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
   // host operands are staged through a device column block first
   const double* d_in = in;
@@ -1146,7 +1180,7 @@ int umf_plan_save(umf_plan* p, const char* path) {
   h.ir_max = p->ir_max;
   h.ir_tol = p->ir_tol;
   h.arena_elems = P.arena_elems;
-  h.elem_size = !p->factored ? 0 : p->fact_prec == UMF_PREC_FP32 ? 4 : 8;
+  h.elem_size = !p->factored ? 0 : p->fact_prec != UMF_PREC_FP64 ? 4 : 8;
   HIPCHK(hipStreamSynchronize(p->stream));
   std::vector<double> val((size_t)h.nnz);
   if (h.nnz) HIPCHK(hipMemcpy(val.data(), p->d_val, sizeof(double) * h.nnz, hipMemcpyDeviceToHost));
@@ -1182,7 +1216,10 @@ int umf_plan_load(const char* path, umf_plan** out) {
   };
   if (std::fread(&h, sizeof h, 1, f) != 1 || !std::equal(kPlanMagic, kPlanMagic + 8, h.magic))
     return bad("not a plan file");
-  if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0)
+  auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_BF16X6; };
+  if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0 ||
+      !prec_ok(h.prec) || !prec_ok(h.fact_prec) ||
+      (h.elem_size != 0 && (h.elem_size == 8) != (h.fact_prec == UMF_PREC_FP64)))
     return bad("corrupt header");
   std::vector<int> rp((size_t)h.n + 1), ci((size_t)h.nnz);
   std::vector<double> val((size_t)h.nnz);
@@ -1564,13 +1601,15 @@ int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
 int umf_get_stats(const umf_plan* p, umf_stats* out) {
   if (!p || !out) return fail(UMF_EINVAL, "null argument");
   *out = p->stats;
-  out->bytes_front_pool = (p->prec == UMF_PREC_FP32 ? 4.0 : 8.0) * p->P.arena_elems;
+  out->bytes_front_pool = (p->prec != UMF_PREC_FP64 ? 4.0 : 8.0) * p->P.arena_elems;
   return UMF_OK;
 }
 
 int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   if (!p) return fail(UMF_EINVAL, "null plan");
-  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32) return fail(UMF_EINVAL, "unknown precision");
+  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3 &&
+      precision != UMF_PREC_BF16X6)
+    return fail(UMF_EINVAL, "unknown precision");
   if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
   if (precision != p->prec) p->factored = false;
   p->prec = precision;
