@@ -184,8 +184,6 @@ FORM_DTYPE = {
     "fp32": "f32 fronts (FP32 MFMA) + f64 refinement",
     "bf16x3": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 2 bf16 pieces (3 products, f32 accumulate) "
               "+ f64 refinement",
-    "bf16x6": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 3 bf16 pieces (6 products, f32 accumulate) "
-              "+ f64 refinement",
 }
 
 
@@ -271,7 +269,7 @@ def run_serve(args):
     # box change is applied inside its round
     d_val = torch.from_numpy(v).cuda()
     d_coeff = torch.from_numpy(coeff.copy()).cuda()
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         plan.SetPrecision(prec, tol=args.ir_tol)
         plan.Factor()
         srv = UpdateServer(plan, d_val, torch, d_coeff=d_coeff, h=h)
@@ -322,7 +320,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument("--no-fp32", action="store_true", help="skip the lower-precision + FP64-refinement forms")
-    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp32", "bf16x3", "bf16x6"],
+    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp32", "bf16x3"],
                     help="lower-precision forms measured beside FP64 (each refined in FP64)")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],

@@ -186,16 +186,15 @@ int umf_get_stats(const umf_plan* plan, umf_stats* out);
  *     and residuals, the FP32 multifrontal solve as preconditioner) until every column reaches
  *     the normwise backward error ||b - A x|| / (||A||_F ||x|| + ||b||) <= tol, at most max_iter
  *     iterations (UMF_ENOCONV when a column does not); umf_apply applies the FP32 factors.
- *   UMF_PREC_BF16X3, UMF_PREC_BF16X6: the FP32 form with the factorization's Schur and
- *     trailing-update GEMMs on v_mfma_f32_16x16x32_bf16: each FP32 operand is split into 2 (X3)
- *     or 3 (X6) bfloat16 pieces, and the products of the pieces whose indices sum to less than
- *     2 or 3 (3 or 6 products) are accumulated in FP32.  Fronts, pivot blocks, panels, solve
- *     and refinement are those of UMF_PREC_FP32.
+ *   UMF_PREC_BF16X3: the FP32 form with the factorization's Schur and trailing-update GEMMs
+ *     on v_mfma_f32_16x16x32_bf16: each FP32 operand is split into two bfloat16 pieces,
+ *     x = x0 + x1, and the three products x0 y0 + x0 y1 + x1 y0 are accumulated in FP32 (about
+ *     16 significant bits per product).  Fronts, pivot blocks, panels, solve and refinement are
+ *     those of UMF_PREC_FP32.
  * Takes effect at the next umf_factor (the plan must be factored again). */
 #define UMF_PREC_FP64 0
 #define UMF_PREC_FP32 1
 #define UMF_PREC_BF16X3 2
-#define UMF_PREC_BF16X6 3
 int umf_set_precision(umf_plan* plan, int precision, double tol, int max_iter);
 
 /* Per-kernel timing (no reference counterpart; the reference's HASCPP11TIMING timers are

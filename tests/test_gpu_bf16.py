@@ -1,16 +1,15 @@
-"""GPU: the BF16-split precision forms ('bf16x3', 'bf16x6') against the CPU oracle (FP64).
+"""GPU: the BF16-split precision form ('bf16x3') against the CPU oracle (FP64).
 
-Both forms keep the fronts in FP32 like the 'fp32' form and run the factorization's Schur and
-trailing-update GEMMs (k_gemm) on v_mfma_f32_16x16x32_bf16: every FP32 operand is split into 2
-or 3 bfloat16 pieces and the piece products of index sum < 2 or < 3 (3 or 6 of them) are
-accumulated in FP32.  The contract is the FP64 one on the refined solution (normwise backward
-error <= the requested tolerance, solution within the problem's conditioning of the oracle's).
-The forward check is the rigorous one: x - x* = A^-1 (b - A x), so each refined column must lie
-within ||A^-1||_2 ||b - A x|| of the oracle's solution (||A^-1||_2 from inverse iteration on the
-oracle's FP64 solve).  The factor blocks show which product ran: bfloat16 pieces carry 8
-significant bits each, so two pieces leave about 2^-16 of each operand and 3 products of 2
-pieces about 2^-15 relative per product -- far above FP32's 2^-24 -- while 3 pieces (2^-24 per
-operand, dropped products below 2^-24) sit at FP32's level.
+The form keeps the fronts in FP32 like the 'fp32' form and runs the factorization's Schur and
+trailing-update GEMMs (k_gemm) on v_mfma_f32_16x16x32_bf16: every FP32 operand x is split into
+two bfloat16 pieces x0 + x1 and the products x0 y0 + x0 y1 + x1 y0 are accumulated in FP32.
+The contract is the FP64 one on the refined solution (normwise backward error <= the requested
+tolerance, solution within the problem's conditioning of the oracle's).  The forward check is
+the rigorous one: x - x* = A^-1 (b - A x), so each refined column must lie within
+||A^-1||_2 ||b - A x|| of the oracle's solution (||A^-1||_2 from inverse iteration on the
+oracle's FP64 solve).  The factor blocks show that the split product ran: two bfloat16 pieces
+carry about 16 significant bits of each operand, so the blocks are markedly less accurate than
+the FP32 form's (2^-24 per product) yet within the FP32 form's block bound times ten.
 """
 import numpy as np
 import pytest
@@ -21,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 IR_TOL = 1e-12
 FWD_TOL = 1e-6    # bound on ||A^-1|| ||b - A x|| / ||x|| at backward error 1e-12
-FORMS = ["bf16x3", "bf16x6"]
+FORMS = ["bf16x3"]
 
 CASES = [
     ((64, 47), 5, "lognormal"),
@@ -114,9 +113,9 @@ def test_split_solve_reaches_fp64_accuracy(case, form):
 
 @pytest.mark.parametrize("case", [CASES[1], CASES[3]], ids=ids)
 def test_split_blocks_against_fp32(case):
-    """Factor blocks of the three FP32-storage forms against the FP64 oracle: two pieces are
-    markedly less accurate than FP32 MFMA (the split path really ran), three pieces are within a
-    small factor of FP32, and both stay within the FP32 form's block bound."""
+    """Factor blocks of the two FP32-storage forms against the FP64 oracle: the split product is
+    markedly less accurate than FP32 MFMA (the split path really ran) and stays within ten
+    times the FP32 form's block bound."""
     dims, maxs, kind = case
     _, A, G, O = _mk(dims, maxs, kind)
     O.factor()
@@ -131,8 +130,7 @@ def test_split_blocks_against_fp32(case):
     assert err["fp32"] <= 1e-3
     assert err["bf16x3"] <= 1e-2
     assert err["bf16x3"] >= 8 * err["fp32"]
-    assert err["bf16x6"] <= 4 * err["fp32"] + 1e-7
-    assert not np.array_equal(blocks["bf16x6"], blocks["fp32"])
+    assert not np.array_equal(blocks["bf16x3"], blocks["fp32"])
 
 
 @pytest.mark.parametrize("form", FORMS)

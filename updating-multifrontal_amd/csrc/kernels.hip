@@ -140,84 +140,9 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
     }
 }
 
-// ----------------------------------------------------------------------------- MFMA tile GEMM
-// acc += P Q^T for one 64 x 64 tile with 16x16x4 MFMAs (4 waves as 2 x 2, each 32 x 32 =
-// 2 x 2 blocks of 16 x 16).  P(i,k) = P[k*ldp + i], Q(j,k) = Q[k*ldq + j]; rows i >= pn and
-// j >= qn read as zero; k in [0, K).  The MFMA A operand is Q and B is P, so accumulator r of a
-// lane holds C(i, j) with i = 16-block row + lane%16 and j = 16-block col + mrow(lane/16, r)
-// (the MFMA leaves D[mrow(lane/16, r)][lane%16] in accumulator r), and the 16 lanes of a
-// group cover 16 consecutive rows of C.  A wave skips the MFMAs of its 16 x 16 blocks that lie
-// wholly in rows >= pn or columns >= qn, and with LOWER those wholly above the diagonal
-// (P = Q, only C's lower triangle wanted); those accumulators stay zero.
-template <class T>
-struct TileSmem {
-  T As[16][64];
-  T Bs[16][64];
-};
-
+// ----------------------------------------------------------------------------- MFMA tile GEMMs
 // the wave index as a scalar, so that branches on it are uniform
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-
-template <class T, bool LOWER = false>
-__device__ __forceinline__ void tile_gemm_nt(V4<T> (&acc)[2][2], const T* __restrict__ P, int ldp, int pn,
-                                             const T* __restrict__ Q, int ldq, int qn, int K, TileSmem<T>& sm) {
-  const int t = threadIdx.x, lane = t & 63, w = wave_id();
-  const int wr = w >> 1, wc = w & 1;
-  bool blk_on[2][2];
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++) {
-      const int r0 = wr * 32 + bi * 16, c0 = wc * 32 + bj * 16;
-      blk_on[bi][bj] = r0 < pn && c0 < qn && (!LOWER || r0 + 15 >= c0);
-    }
-  const int li = t & 63, lk = t >> 6;
-  const bool prow = li < pn, qrow = li < qn;
-  T pa[4], qb[4];
-  auto fetch = [&](int kc) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = kc + lk + 4 * q;
-      const bool kin = k < K;
-      pa[q] = (kin && prow) ? P[(i64)k * ldp + li] : T(0);
-      qb[q] = (kin && qrow) ? Q[(i64)k * ldq + li] : T(0);
-    }
-  };
-  if (K <= 0) return;
-  // the common case (every block of the wave on) runs a loop without per-MFMA branches
-  const bool all_on = blk_on[0][0] && blk_on[0][1] && blk_on[1][0] && blk_on[1][1];
-  auto run = [&](auto guard) {
-    fetch(0);
-    for (int kc = 0; kc < K; kc += 16) {
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        sm.As[lk + 4 * q][li] = pa[q];
-        sm.Bs[lk + 4 * q][li] = qb[q];
-      }
-      __syncthreads();
-      if (kc + 16 < K) fetch(kc + 16);
-#pragma unroll
-      for (int k4 = 0; k4 < 4; k4++) {
-        const int kr = k4 * 4 + (lane >> 4);
-#pragma unroll
-        for (int bi = 0; bi < 2; bi++) {
-          const T bv = sm.As[kr][wr * 32 + bi * 16 + (lane & 15)];
-#pragma unroll
-          for (int bj = 0; bj < 2; bj++) {
-            const T av = sm.Bs[kr][wc * 32 + bj * 16 + (lane & 15)];
-            if (!decltype(guard)::value || blk_on[bi][bj]) acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
-          }
-        }
-      }
-    }
-  };
-  if (all_on)
-    run(std::false_type{});
-  else
-    run(std::true_type{});
-  __syncthreads();
-}
 
 // Operand loads of the tile GEMMs are buffer loads: one resource per K chunk covers the chunk's
 // valid columns (base = the chunk's first column, size = its valid columns x ld), so columns at
@@ -246,9 +171,12 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
-// The same tile GEMM for a TS x TS tile (TS = 64 or 128): 4 waves as 2 x 2, each (TS/2)^2 =
-// (TS/32)^2 blocks of 16 x 16; per 4-deep k step a wave reads TS/32 + TS/32 operand fragments
-// from LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
+// acc += P Q^T for one TS x TS tile (TS = 64 or 128), P(i,k) = P[k*ldp + i], Q(j,k) = Q[k*ldq + j],
+// k in [0, K), on 16x16x4 MFMAs: 4 waves as 2 x 2, each (TS/2)^2 = (TS/32)^2 blocks of 16 x 16.
+// The MFMA A operand is Q and B is P, so accumulator r of a lane holds C(i, j) with i = 16-block
+// row + lane%16 and j = 16-block col + mrow(lane/16, r), and the 16 lanes of a group cover 16
+// consecutive rows of C.  Per 4-deep k step a wave reads TS/32 + TS/32 operand fragments from
+// LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
 // c+1 (loaded into registers one chunk earlier) is written to the other stage and chunk c+2 is
 // fetched, one barrier per 16-deep chunk.  P and Q start at the tile's first row; pn, qn >= 1.
 template <class T, int TS>
@@ -326,12 +254,12 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
 }
 
 // BF16-split variant of tile_gemm_nt_t for FP32 operands (the split precision form).  Each FP32
-// value x is written to LDS as NPC bfloat16 pieces, x ~ x_0 + x_1 + ... (x_0 = bf16(x),
-// x_1 = bf16(x - x_0), ...; each difference is exact in FP32), and P Q^T is formed on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation from the piece products whose indices sum
-// to < NPC, the smallest first: NPC = 2 takes x_0 y_0 + x_0 y_1 + x_1 y_0 (3 products, about
-// 16 significant bits per product), NPC = 3 the 6 products of the three pieces (about FP32's
-// 24).  bfloat16 has FP32's exponent range, so the pieces need no scaling.
+// value x is written to LDS as two bfloat16 pieces, x ~ x_0 + x_1 (x_0 = bf16(x),
+// x_1 = bf16(x - x_0); the difference is exact in FP32), and P Q^T is formed on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation from the three piece products
+// x_1 y_0 + x_0 y_1 + x_0 y_0 (the small ones first; x_1 y_1 is dropped): about 16 significant
+// bits per product.  bfloat16 has FP32's exponent range, so the pieces need no scaling.  (A
+// third piece and six products reach FP32's 24 bits but measured slower than FP32 MFMA.)
 // The instruction's operand layout: lane l supplies row l%16 of A (column l%16 of B) with 8
 // consecutive k of group l/16 of the 32-deep step, one 16-byte LDS read; accumulator r of lane
 // l holds D[4 (l/16) + r][l%16], the layout of v_mfma_f32_16x16x4.  LDS rows hold 32 k as four
@@ -345,32 +273,26 @@ __device__ __forceinline__ V4<float> mfma_bf(bf16x8 a, bf16x8 b, V4<float> c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-template <int TS, int NPC>
+template <int TS>
 struct TileSmemBF {
-  bf16x8 P[2][NPC][TS][5];
-  bf16x8 Q[2][NPC][TS][5];
+  bf16x8 P[2][2][TS][5];  // [stage][piece][row][group of 8 k]
+  bf16x8 Q[2][2][TS][5];
 };
 
-template <int NPC>
-__device__ __forceinline__ void split_bf(const float (&x)[8], bf16x8 (&pc)[NPC]) {
-  float r[8];
+__device__ __forceinline__ void split_bf(const float (&x)[8], bf16x8 (&pc)[2]) {
 #pragma unroll
-  for (int j = 0; j < 8; j++) r[j] = x[j];
-#pragma unroll
-  for (int p = 0; p < NPC; p++)
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const __bf16 h = (__bf16)r[j];
-      pc[p][j] = h;
-      r[j] -= (float)h;
-    }
+  for (int j = 0; j < 8; j++) {
+    const __bf16 h = (__bf16)x[j];
+    pc[0][j] = h;
+    pc[1][j] = (__bf16)(x[j] - (float)h);
+  }
 }
 
-template <int TS, int NPC>
+template <int TS>
 __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 32], const float* __restrict__ P,
                                                 int ldp, int pn, const float* __restrict__ Q, int ldq, int qn, int K,
-                                                TileSmemBF<TS, NPC>& sm) {
-  constexpr int NL = TS / 64, NBK = TS / 32;
+                                                TileSmemBF<TS>& sm) {
+  constexpr int NL = TS / 64, NBK = TS / 32, NPC = 2;
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = w >> 1, wc = w & 1;
@@ -401,8 +323,8 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
 #pragma unroll
     for (int u = 0; u < NL; u++) {
       bf16x8 pp[NPC], qp[NPC];
-      split_bf<NPC>(pa[u], pp);
-      split_bf<NPC>(qb[u], qp);
+      split_bf(pa[u], pp);
+      split_bf(qb[u], qp);
 #pragma unroll
       for (int p = 0; p < NPC; p++) {
         sm.P[b][p][li + 64 * u][kg] = pp[p];
@@ -437,13 +359,13 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
           for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma_bf(av[i][bj], bv[s - i][bi], acc[bi][bj]);
     stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
     fetch(kc + 64);
-    constexpr int NM = NBK * NBK * NPC * (NPC + 1) / 2;  // MFMAs per chunk
+    constexpr int NM = NBK * NBK * 3;  // MFMAs per chunk
 #pragma unroll
     for (int g = 0; g < 8; g++) {
-      __builtin_amdgcn_sched_group_barrier(0x008, NM / 8, 0);         // MFMA
-      __builtin_amdgcn_sched_group_barrier(0x002, 12, 0);             // VALU
-      __builtin_amdgcn_sched_group_barrier(0x200, NPC == 2 ? 1 : 2, 0);  // LDS write
-      __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);              // buffer load
+      __builtin_amdgcn_sched_group_barrier(0x008, NM / 8, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x002, 12, 0);      // VALU
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);       // LDS write
+      __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);       // buffer load
     }
     __syncthreads();
   }
@@ -762,6 +684,9 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
       }
 }
 
+// The left-looking GEMM runs on the native MFMA of T in every precision form: BF16 pieces
+// (tile_gemm_nt_bf) measured slower here, where the GEMMs are short and the 40 KB of split
+// stages halve the workgroups per CU.
 template <class T>
 __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                     T* __restrict__ A, int jb, int left, int xt, int nf, int fly,
@@ -776,7 +701,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
   const int s0 = jb + kb;  // first row below the block
   if (s0 + NB * tile >= F.N) return;
   __shared__ union {
-    TileSmem<T> g;
+    TileSmemT<T, 64> g;
     struct {
       T X[NB * (NB + 1)];
     } c;
@@ -788,10 +713,10 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
-  const int row0 = s0 + NB * tile;
+  const int row0 = __builtin_amdgcn_readfirstlane(s0 + NB * tile);
   const int nrow = min(NB, F.N - row0);
   const int K = left ? jb : 0;
-  tile_gemm_nt(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
+  tile_gemm_nt_t<T, 64>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
   T* pan = A + F.off + (i64)jb * F.N + row0;
   // the tile's input: the assembled front, or (fly) its CSR entries added below
   const bool from_csr = fly && F.fly_off >= 0;
@@ -943,12 +868,12 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         kw = group width, ccap = n: the group's trailing update).
 // MODE 1: Schur complement: P = Q = L21 = F[n:N, 0:n], K = n; C = F[n:N, n:N] lower gets
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
-// SPLIT = 0: the native MFMA of T; SPLIT = 2 or 3 (T = float): the product on BF16 pieces
-// (tile_gemm_nt_bf with NPC = SPLIT).
-template <class T, int MODE, int TS = 64, int SPLIT = 0>
-__global__ __launch_bounds__(256, TS == 128 && SPLIT < 3 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+// SPLIT = false: the native MFMA of T; true (T = float): the product on BF16 pieces
+// (tile_gemm_nt_bf).
+template <class T, int MODE, int TS = 64, bool SPLIT = false>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                               T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
-  static_assert(SPLIT == 0 || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
+  static_assert(!SPLIT || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
   const FrontDev F = fr[ids[fi]];
@@ -975,12 +900,12 @@ __global__ __launch_bounds__(256, TS == 128 && SPLIT < 3 ? 2 : 1) void k_gemm(co
 #pragma unroll
     for (int j = 0; j < NBK; j++) acc[i][j] = vzero<T>();
   const T* base = A + F.off + (i64)kbase * F.N;
-  if constexpr (SPLIT == 0) {
+  if constexpr (!SPLIT) {
     __shared__ TileSmemT<T, TS> sm;
     tile_gemm_nt_t<T, TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   } else {
-    __shared__ TileSmemBF<TS, SPLIT> sm;
-    tile_gemm_nt_bf<TS, SPLIT>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+    __shared__ TileSmemBF<TS> sm;
+    tile_gemm_nt_bf<TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   }
   T* C = A + F.off;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;

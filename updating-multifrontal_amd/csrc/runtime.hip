@@ -259,20 +259,19 @@ inline int tiles_of(int s, int rend, int cend, int ts = 64) {
   return T * (T + 1) / 2 + (RT - T) * T;
 }
 
-// Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
-// k_gemm instance of a stage: the native MFMA of T, or for FP32 fronts with split = 2 / 3 the
-// product on that many BF16 pieces
+// k_gemm instance of a stage: the native MFMA of T, or for FP32 fronts with `split` the
+// product on BF16 pieces
 template <class T, int MODE, int TS>
-auto gemm_fn(int split) {
+auto gemm_fn(bool split) {
   if constexpr (std::is_same<T, float>::value) {
-    if (split == 2) return k_gemm<T, MODE, TS, 2>;
-    if (split == 3) return k_gemm<T, MODE, TS, 3>;
+    if (split) return k_gemm<T, MODE, TS, true>;
   }
-  return k_gemm<T, MODE, TS, 0>;
+  return k_gemm<T, MODE, TS, false>;
 }
 
+// Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
 template <class T>
-int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, int split) {
+int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
   const Plan& P = p->P;
   Sched S = make_sched(P, D);
   int rc = ensure_ids(p, S.ids.size());
@@ -486,16 +485,13 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, int split) {
 
 // Factor over a dirty schedule in the precision the plan is set to (the FP32 front pool is
 // allocated on first use).
-// BF16 pieces per FP32 operand in the factorization's GEMMs of a precision form (0: none)
-int split_of(int prec) { return prec == UMF_PREC_BF16X3 ? 2 : prec == UMF_PREC_BF16X6 ? 3 : 0; }
-
 int factor_any(umf_plan* p, const Plan::Dirty& D) {
   if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
-    return run_factor<float>(p, D, p->arena32, split_of(p->prec));
+    return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3);
   }
-  return run_factor<double>(p, D, p->arena, 0);
+  return run_factor<double>(p, D, p->arena, false);
 }
 
 // Copy cnt front-pool entries at offset off of the current factors to the host as double.
@@ -1216,7 +1212,7 @@ int umf_plan_load(const char* path, umf_plan** out) {
   };
   if (std::fread(&h, sizeof h, 1, f) != 1 || !std::equal(kPlanMagic, kPlanMagic + 8, h.magic))
     return bad("not a plan file");
-  auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_BF16X6; };
+  auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_BF16X3; };
   if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0 ||
       !prec_ok(h.prec) || !prec_ok(h.fact_prec) ||
       (h.elem_size != 0 && (h.elem_size == 8) != (h.fact_prec == UMF_PREC_FP64)))
@@ -1607,8 +1603,7 @@ int umf_get_stats(const umf_plan* p, umf_stats* out) {
 
 int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   if (!p) return fail(UMF_EINVAL, "null plan");
-  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3 &&
-      precision != UMF_PREC_BF16X6)
+  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3)
     return fail(UMF_EINVAL, "unknown precision");
   if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
   if (precision != p->prec) p->factored = false;

@@ -50,15 +50,10 @@ class Front {
   void Factor() { check(umf_factor(plan_)); }
 
   // Precision form (no reference counterpart): Precision::FP32 keeps and factors the fronts in
-  // FP32 on FP32 MFMA, and Solve refines in FP64 to backward error <= tol.  BF16X3 / BF16X6
-  // are FP32 with the Schur and trailing-update GEMMs on BF16 MFMA over 2 / 3 bfloat16 pieces
-  // of each operand (FP32 accumulation).  Factor() again after switching.
-  enum class Precision {
-    FP64 = UMF_PREC_FP64,
-    FP32 = UMF_PREC_FP32,
-    BF16X3 = UMF_PREC_BF16X3,
-    BF16X6 = UMF_PREC_BF16X6
-  };
+  // FP32 on FP32 MFMA, and Solve refines in FP64 to backward error <= tol.  BF16X3 is FP32
+  // with the Schur and trailing-update GEMMs on BF16 MFMA over two bfloat16 pieces of each
+  // operand (3 products, FP32 accumulation).  Factor() again after switching.
+  enum class Precision { FP64 = UMF_PREC_FP64, FP32 = UMF_PREC_FP32, BF16X3 = UMF_PREC_BF16X3 };
   void SetPrecision(Precision p, double tol = 1e-10, int maxIter = 50) {
     check(umf_set_precision(plan_, (int)p, tol, maxIter));
   }

@@ -24,7 +24,7 @@ _lib = C.CDLL(LIB_PATH)
 
 UMF_OK = 0
 UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE, UMF_ENOCONV = range(-1, -9, -1)
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
 class RefinementError(ArithmeticError):
@@ -165,10 +165,9 @@ class _MF:
     # ---- precision form (no reference counterpart)
     def SetPrecision(self, precision="fp64", tol=1e-10, max_iter=50):
         """'fp64' (default, the reference's arithmetic) or 'fp32': FP32 fronts on FP32 MFMA, Solve
-        refined in FP64 (preconditioned CG) to backward error <= tol; 'bf16x3' / 'bf16x6': the
-        fp32 form with the Schur and trailing-update GEMMs on BF16 MFMA over 2 / 3 bfloat16
-        pieces of each FP32 operand (3 / 6 piece products, FP32 accumulation).  Factor() again
-        afterwards."""
+        refined in FP64 (preconditioned CG) to backward error <= tol; 'bf16x3': the fp32 form
+        with the Schur and trailing-update GEMMs on BF16 MFMA over two bfloat16 pieces of each
+        FP32 operand (3 piece products, FP32 accumulation).  Factor() again afterwards."""
         _check(_set_prec(self._h, PRECISIONS[precision], float(tol), int(max_iter)))
 
     # ---- the reference's public API
