@@ -173,6 +173,10 @@ Sched make_sched(const Plan& P, const Plan::Dirty& D) {
       L.f_grid = std::max(L.f_grid, grid_of(f));
     }
     L.f_cnt = (int)S.ids.size() - L.f_off;
+    // largest pivot block first: the 64-column steps of a level then launch over a prefix of
+    // its fronts (those with n > k0), not over every front of the level
+    std::stable_sort(S.ids.begin() + L.f_off, S.ids.end(),
+                     [&](int a, int b) { return P.fronts[a].n > P.fronts[b].n; });
     L.s_off = (int)S.ids.size();
     L.max_k = 0;
     if (ell >= 1)
@@ -269,6 +273,12 @@ auto gemm_fn(bool split) {
   return k_gemm<T, MODE, TS, false>;
 }
 
+// Workgroups per item (front, separator) of a level stage whose largest item can use `want`
+// of them: enough in all to fill the chip (about 32k), no more.  Every stage loops over its
+// item's work with a stride of gridDim.x, so a smaller grid only lengthens the loops; levels of
+// tens of thousands of small fronts and one large one then launch one workgroup per front.
+inline int per_item(int want, int cnt) { return std::max(1, std::min(want, (32768 + cnt - 1) / std::max(cnt, 1))); }
+
 // Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
 template <class T>
 int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
@@ -308,11 +318,12 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
     if (L.sm_cnt > 0) {
       double zb, ab;
       asm_work(L.sm_off, L.sm_cnt, zb, ab);
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(std::max(L.max_k, 1), 256), L.sm_cnt),
-             ids + L.sm_off, (const MatDev*)p->d_mats, arena);
-      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.s_grid, L.sm_cnt), ids + L.sm_off, (const MatDev*)p->d_mats,
-             arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
-             (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>,
+             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+             (const MatDev*)p->d_mats, arena);
+      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+             (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
+             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
     if (L.f_cnt == 0) continue;
     // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
@@ -339,15 +350,16 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
           zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
           ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
         }
-        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>, dim3(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt),
+        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>,
+               dim3(per_item(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt), L.f_cnt),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena);
-        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off,
+        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
                (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
                (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
       } else {
-        launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(std::min(L.maxN, 256), L.f_cnt), ids + L.f_off,
-               (const MatDev*)p->d_mats, arena);
-        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(L.f_grid, L.f_cnt), ids + L.f_off,
+        launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
+               ids + L.f_off, (const MatDev*)p->d_mats, arena);
+        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
                (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
                (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
       }
@@ -364,20 +376,23 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
     const int GW2 = f32 ? 4 * GW : GW;
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
-    const bool xcd = L.f_cnt >= 64;
-    auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((L.f_cnt + 7) / 8) * per_front); };
+    bool xcd = L.f_cnt >= 64;
+    int act = L.f_cnt;  // fronts taking part in the current step (a prefix: n > k0)
+    auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((act + 7) / 8) * per_front); };
     auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
       const double nc = cend - s, nr = rend - s;
       return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
     };
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
+      while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
+      xcd = act >= 64;
       const int g0 = k0 / GW * GW, gend = g0 + GW;
       const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
       const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
       const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
       int rows = 0, tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
       double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
-      for (int i = 0; i < L.f_cnt; i++) {
+      for (int i = 0; i < act; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
         int kb = std::min(NB, F.n - k0), N = F.n + F.m;
@@ -420,34 +435,36 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
       // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
       // repeats the diagonal factorization in each) run diagonal block and panel in one launch
       const int ptd = std::max(1, (rows + NB - 1) / NB);
-      const bool fused = !left && (i64)ptd * L.f_cnt <= 512;
+      const bool fused = !left && (i64)ptd * act <= 512;
       if (fused)
-        launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? ptd : 0, L.f_cnt, p->d_arrive, p->d_err);
+        launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? ptd : 0, act, p->d_arrive, p->d_err);
       else
-        launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+        launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
                arena, k0, p->d_err);
       if (!fused && rows > 0) {
         const int pt = (rows + NB - 1) / NB;
-        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, L.f_cnt,
+        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, act,
                fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
       }
       if (tin > 0)
-        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, L.f_cnt);
+        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
       auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
-        if (f32 && (i64)t128 * L.f_cnt >= 2048)
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, L.f_cnt), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, L.f_cnt);
+        if (f32 && (i64)t128 * act >= 2048)
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, act), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
         else
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, L.f_cnt),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, L.f_cnt);
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
+                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
       };
       if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
       if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
     }
-    // 4. Schur complement into the trailing block
+    // 4. Schur complement into the trailing block (every front of the level)
+    act = L.f_cnt;
+    xcd = L.f_cnt >= 64;
     if (L.maxm > 0) {
       int tiles = 0;
       double fl = 0, by = 0;
@@ -473,7 +490,8 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
     if (L.s_cnt > 0) {
       double by = 0;
       for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
-      launch(p, "k_sep_add", 0, by, k_sep_add<T>, dim3(std::min(std::max(L.max_k, 1), 256), L.s_cnt), ids + L.s_off,
+      launch(p, "k_sep_add", 0, by, k_sep_add<T>,
+             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.s_cnt), L.s_cnt), ids + L.s_off,
              (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
     }
   }
@@ -559,6 +577,10 @@ AllLists all_lists(const Plan& P) {
     A.f_off[ell] = (int)A.ids.size();
     for (int f : P.box_gid[ell]) A.ids.push_back(f);
     A.f_cnt[ell] = (int)A.ids.size() - A.f_off[ell];
+    // smallest fronts first: the forward sweep launches each size class with its own kernel
+    std::stable_sort(A.ids.begin() + A.f_off[ell], A.ids.end(), [&](int a, int b) {
+      return P.fronts[a].n + P.fronts[a].m < P.fronts[b].n + P.fronts[b].m;
+    });
     A.s_off[ell] = (int)A.ids.size();
     if (ell >= 1)
       for (int s : P.sep_gid[ell]) A.ids.push_back(s);
@@ -610,30 +632,43 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
       nbt = std::max(nbt, (F.n + NB - 1) / NB + (F.m + NB - 1) / NB);
     }
   };
-  // FP64 levels of many fronts of at most 64 G rows: the forward sweep keeps the right-hand-side
-  // rows in registers (G = 6, 12 or 18 chunks of 64 rows)
-  auto chunks = [&](int ell) {
-    int mx = 0;
-    for (int f : P.box_gid[ell]) mx = std::max(mx, P.fronts[f].n + P.fronts[f].m);
-    const int c = (mx + NB - 1) / NB;
-    return c <= 6 ? 6 : c <= 12 ? 12 : c <= 18 ? 18 : 0;
-  };
+  // FP64 levels of many fronts: fronts of at most 64 G rows keep the right-hand-side rows in
+  // registers in the forward sweep (G = 6, 12 or 18 chunks of 64 rows), larger ones stream.
+  // The level's fronts are sorted by size, so each class is a contiguous run of the list.
   auto fwd = [&](int ell) {
     work(ell, fl, by);
     g_level = ell;
-    // (FP32 keeps the streaming kernel: the register form measured slower there)
-    const int G = (sizeof(T) == 8 && !blocked(ell)) ? chunks(ell) : 0;
-    const dim3 gf(A.f_cnt[ell]);
-    const int* li = ids + A.f_off[ell];
-    if (G == 6)
-      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 6>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
-    else if (G == 12)
-      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 12>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
-    else if (G == 18)
-      launch(p, "k_solve_fwd", fl, by, k_solve_fwd_r<T, 18>, gf, li, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
-    else if (!blocked(ell)) {
-      launch(p, "k_solve_fwd", fl, by, k_solve_fwd<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-             (const T*)arena, (const int*)p->d_dofs, w, xs);
+    if (!blocked(ell)) {
+      const int* li = ids + A.f_off[ell];
+      const int cnt = A.f_cnt[ell];
+      int i = 0;
+      auto run_end = [&](int rows) {  // end of the run of fronts with n + m <= rows
+        int e = i;
+        while (e < cnt && P.fronts[A.ids[A.f_off[ell] + e]].n + P.fronts[A.ids[A.f_off[ell] + e]].m <= rows) e++;
+        return e;
+      };
+      // work is attributed to the launches in proportion to their fronts
+      auto part = [&](int c, double x) { return x * c / std::max(cnt, 1); };
+      if (sizeof(T) == 8) {  // (FP32 keeps the streaming kernel: the register form measured slower there)
+        int e = run_end(6 * NB);
+        if (e > i)
+          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_r<T, 6>, dim3(e - i), li + i, fr,
+                 (const T*)arena, (const int*)p->d_dofs, w, xs);
+        i = e;
+        e = run_end(12 * NB);
+        if (e > i)
+          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_r<T, 12>, dim3(e - i), li + i, fr,
+                 (const T*)arena, (const int*)p->d_dofs, w, xs);
+        i = e;
+        e = run_end(18 * NB);
+        if (e > i)
+          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_r<T, 18>, dim3(e - i), li + i, fr,
+                 (const T*)arena, (const int*)p->d_dofs, w, xs);
+        i = e;
+      }
+      if (cnt > i)
+        launch(p, "k_solve_fwd", part(cnt - i, fl), part(cnt - i, by), k_solve_fwd<T>, dim3(cnt - i), li + i, fr,
+               (const T*)arena, (const int*)p->d_dofs, w, xs);
     } else {
       int nbi, nbt;
       blocks(ell, nbi, nbt);
