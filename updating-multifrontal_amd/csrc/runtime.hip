@@ -378,6 +378,23 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     bool xcd = L.f_cnt >= 64;
     int act = L.f_cnt;  // fronts taking part in the current step (a prefix: n > k0)
+    std::vector<int> seg_t, suf;  // per active front: panel row tiles, flops, bytes of the step
+    std::vector<double> seg_f, seg_b;
+    // suf[i] = max t[i..cnt); then, unless the runs would save at least half of the one
+    // launch's workgroups, suf is flattened to one run (each launch costs a few microseconds)
+    auto suffix_max = [&](const std::vector<int>& t, int cnt) {
+      suf.assign(cnt + 1, 0);
+      for (int i = cnt - 1; i >= 0; i--) suf[i] = std::max(suf[i + 1], t[i]);
+      i64 one = (i64)suf[0] * cnt, runs = 0;
+      for (int i = 0; i < cnt;) {
+        int e = i + 1;
+        while (e < cnt && 2 * suf[e] > suf[i]) e++;
+        runs += (i64)suf[i] * (e - i);
+        i = e;
+      }
+      if (2 * runs > one)
+        for (int i = 0; i < cnt; i++) suf[i] = suf[0];
+    };
     auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((act + 7) / 8) * per_front); };
     auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
       const double nc = cend - s, nr = rend - s;
@@ -392,6 +409,9 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
       const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
       int rows = 0, tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
       double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
+      seg_t.assign(act, 0);
+      seg_f.assign(act, 0.0);
+      seg_b.assign(act, 0.0);
       for (int i = 0; i < act; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
@@ -401,8 +421,11 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
         double s0 = k0 + kb, nr = N - s0;
         fdiag += (double)kb * kb * kb / 3.0;
         // TRSM (substitution count) + left-looking GEMM + the future diagonal tile's update
-        fpan += nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
-        bpan += (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
+        seg_t[i] = (N - k0 - kb + NB - 1) / NB;
+        seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
+        seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
+        fpan += seg_f[i];
+        bpan += seg_b[i];
         if (left) continue;
         const int ce = std::min(F.n, gend);
         if (k0 + kb < ce) {
@@ -442,11 +465,26 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
       else
         launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
                arena, k0, p->d_err);
-      if (!fused && rows > 0) {
-        const int pt = (rows + NB - 1) / NB;
-        launch(p, "k_chol_panel", fpan, bpan, k_chol_panel<T>, xcd ? dim3(xcd_grid(pt)) : dim3(pt, act),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xcd ? pt : 0, act,
-               fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
+      // the panel launches over at most a few runs of the active fronts: a run ends where the
+      // largest row-tile count of the fronts after it has halved, so that a level of many small
+      // fronts and a few tall ones does not give every front the tallest one's grid
+      suffix_max(seg_t, act);
+      for (int i = 0; !fused && rows > 0 && i < act;) {
+        const int pt = suf[i];
+        int e = i;
+        double f = 0, b = 0;
+        while (e < act && (e == i || 2 * suf[e] > pt)) {
+          f += seg_f[e];
+          b += seg_b[e];
+          e++;
+        }
+        const int c = e - i;
+        const bool xc = c >= 64;
+        if (pt > 0)
+          launch(p, "k_chol_panel", f, b, k_chol_panel<T>, xc ? dim3(8 * ((c + 7) / 8) * pt) : dim3(pt, c),
+                 ids + L.f_off + i, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xc ? pt : 0, c,
+                 fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
+        i = e;
       }
       if (tin > 0)
         launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
@@ -463,27 +501,43 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
       if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
     }
     // 4. Schur complement into the trailing block (every front of the level)
-    act = L.f_cnt;
-    xcd = L.f_cnt >= 64;
     if (L.maxm > 0) {
-      int tiles = 0;
-      double fl = 0, by = 0;
-      int tiles128 = 0;
+      // per front: 64- and 128-tiles of the trailing block, flops, bytes; launched over runs
+      // ending where the largest tile count after them halves, as the panel steps
+      std::vector<int> t64(L.f_cnt), t128(L.f_cnt);
+      seg_f.assign(L.f_cnt, 0.0);
+      seg_b.assign(L.f_cnt, 0.0);
       for (int i = 0; i < L.f_cnt; i++) {
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
-        tiles = std::max(tiles, tiles_of(F.n, F.n + F.m, F.n + F.m));
-        tiles128 = std::max(tiles128, tiles_of(F.n, F.n + F.m, F.n + F.m, 128));
-        fl += (double)F.n * F.m * (F.m + 1);
-        by += 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
+        t64[i] = tiles_of(F.n, F.n + F.m, F.n + F.m);
+        t128[i] = tiles_of(F.n, F.n + F.m, F.n + F.m, 128);
+        seg_f[i] = (double)F.n * F.m * (F.m + 1);
+        seg_b[i] = 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
       }
-      if (tiles > 0) {
-        if (sizeof(T) == 4 && (i64)tiles128 * L.f_cnt >= 2048)
-          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split),
-                 xcd ? dim3(xcd_grid(tiles128)) : dim3(tiles128, L.f_cnt), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles128 : 0, L.f_cnt);
-        else
-          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 64>(split), xcd ? dim3(xcd_grid(tiles)) : dim3(tiles, L.f_cnt),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xcd ? tiles : 0, L.f_cnt);
+      suffix_max(t128, L.f_cnt);
+      std::vector<int> suf128 = suf;
+      suffix_max(t64, L.f_cnt);
+      for (int i = 0; i < L.f_cnt;) {
+        const int tiles = suf[i], tiles128 = suf128[i];
+        int e = i;
+        double fl = 0, by = 0;
+        while (e < L.f_cnt && (e == i || 2 * suf[e] > tiles)) {
+          fl += seg_f[e];
+          by += seg_b[e];
+          e++;
+        }
+        const int c = e - i;
+        const bool xc = c >= 64;
+        auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
+        if (tiles > 0) {
+          if (sizeof(T) == 4 && (i64)tiles128 * c >= 2048)
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split), grid(tiles128), ids + L.f_off + i,
+                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
+          else
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 64>(split), grid(tiles), ids + L.f_off + i,
+                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
+        }
+        i = e;
       }
     }
     // 5. separators of this level receive both neighbours' Schur blocks
