@@ -86,10 +86,18 @@ int main() {
     hipEventElapsedTime(&ms, e0, e1);
     fl = (double)nb * 4 * iters * 4 * 2048.0;
     printf("f32 mfma 16x16x4, %d waves/SIMD, 4 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
+    hipEventRecord(e0); hipLaunchKernelGGL(k32<8>, dim3(nb), dim3(256), 0, 0, (float*)d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    fl = (double)nb * 4 * iters * 8 * 2048.0;
+    printf("f32 mfma 16x16x4, %d waves/SIMD, 8 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
     hipEventRecord(e0); hipLaunchKernelGGL(kbf<4>, dim3(nb), dim3(256), 0, 0, (float*)d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
     hipEventElapsedTime(&ms, e0, e1);
     fl = (double)nb * 4 * iters * 4 * 16384.0;
     printf("bf16 mfma 16x16x32, %d waves/SIMD, 4 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
+    hipEventRecord(e0); hipLaunchKernelGGL(kbf<8>, dim3(nb), dim3(256), 0, 0, (float*)d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    fl = (double)nb * 4 * iters * 8 * 16384.0;
+    printf("bf16 mfma 16x16x32, %d waves/SIMD, 8 chains: %.1f TFLOP/s\n", wpc, fl / ms / 1e9);
     hipEventRecord(e0); hipLaunchKernelGGL(kfma, dim3(nb), dim3(256), 0, 0, d, iters); hipEventRecord(e1); hipEventSynchronize(e1);
     hipEventElapsedTime(&ms, e0, e1);
     fl = (double)nb * 256 * iters * 8 * 2.0;
