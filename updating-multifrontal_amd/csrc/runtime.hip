@@ -368,12 +368,13 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
     // FP32: 128 x 128 tiles for a wide update (half the operand traffic) when it still yields
     // 2048+ tiles; in FP64 the 128-tile kernel holds 2 waves per SIMD and measured slower
     const bool f32 = sizeof(T) == 4;
-    // FP32 fronts block the trailing update on two levels: a finished group of GW columns
-    // updates only the rest of its outer group of GW2 columns (K = GW), and the finished outer
-    // group updates the trailing matrix once with K = GW2.  The FP32 and BF16 products are
-    // bound by operand and C traffic at K = GW; K = GW2 quarters the C read-modify-write per
-    // flop.  FP64 runs at its MFMA ceiling with K = GW (GW2 = GW: one level).
-    const int GW2 = f32 ? 4 * GW : GW;
+    // The trailing update is blocked on two levels: a finished group of GW columns updates only
+    // the rest of its outer group of GW2 columns (K = GW), and the finished outer group updates
+    // the trailing matrix once with K = GW2.  At K = GW the tile GEMMs are bound by operand and
+    // C traffic: with K = 1024 the 3D 95^3 FP64 trailing update goes from 43 to 52 TFLOP/s, and
+    // the FP32 / BF16-split tiles of a 16k-front microbenchmark from 76 / 103 to 126 / 247;
+    // GW2 = 2048 measured the same as 1024.
+    const int GW2 = 4 * GW;
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     bool xcd = L.f_cnt >= 64;
