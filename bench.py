@@ -130,6 +130,40 @@ def algorithmic_flops(plan, nrhs):
     return fl_solve * nrhs
 
 
+def cpu_threads():
+    """Host cores this process may use (the GPU job's CPU share), at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline_threaded(nrhs=16):
+    """The same sample on the reference built against threaded MKL (oracle/_ref/libumfref_mt.so)
+    on all of this job's host cores.  It runs in a child process started before this one touches
+    the GPU: the threaded and sequential MKL layers cannot share a process.  None when that build
+    is absent or the child fails."""
+    import subprocess
+    lib = os.path.join(ROOT, "oracle", "_ref", "libumfref_mt.so")
+    if not os.path.exists(lib):
+        return None
+    t = cpu_threads()
+    env = dict(os.environ, UMF_REF_SO=lib, MKL_NUM_THREADS=str(t), OMP_NUM_THREADS=str(t))
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", "--nrhs", str(nrhs)],
+                           env=env, capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0 or not r.stdout.strip():
+        progress(f"threaded CPU baseline failed: {r.stderr.strip()[-300:]}")
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cores"] = t
+    out["sample"] = out["sample"].replace("MKL sequential", f"threaded MKL ({t} threads)")
+    return out
+
+
 def cpu_baseline(N=511, seed=2, nrhs=16, max_side=16):
     """The reference itself (oracle/_ref, MKL sequential, 1 thread) on a bounded sample of the
     same workload: the same generator at 511^2 (5 levels, the same 31^2 leaf fronts),
@@ -323,10 +357,14 @@ def main():
     ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp32", "bf16x3"],
                     help="lower-precision forms measured beside FP64 (each refined in FP64)")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
+    ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)  # child of the CPU baseline
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
                     help="serve95: request arrival rates (per second)")
     ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
     args = ap.parse_args()
+    if args.cpu_sample:  # the threaded CPU baseline's child process: no GPU work
+        print(json.dumps(cpu_baseline(nrhs=args.nrhs or 16)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the sharded path is design-only)", file=sys.stderr)
@@ -334,6 +372,10 @@ def main():
 
     if args.workload == "serve95":
         return run_serve(args)
+    cpu_mt = None
+    if not args.no_cpu_baseline and args.workload == "2d1023":
+        progress(f"threaded CPU baseline ({cpu_threads()} threads)")
+        cpu_mt = cpu_baseline_threaded(nrhs=args.nrhs or 16)
     import torch
     import umf
     torch.cuda.set_device(0)
@@ -470,7 +512,12 @@ def main():
         for prec in args.forms:
             out[prec + "_ir"] = low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
     if not args.no_cpu_baseline and args.workload == "2d1023":
-        out["cpu_baseline"] = cpu_baseline(nrhs=nrhs)
+        one = cpu_baseline(nrhs=nrhs)
+        if cpu_mt is not None:  # all host cores (threaded MKL), with the 1-thread run beside it
+            cpu_mt["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "kind", "seconds")}
+            out["cpu_baseline"] = cpu_mt
+        else:
+            out["cpu_baseline"] = one
     print(json.dumps(out), flush=True)
 
 

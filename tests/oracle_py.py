@@ -13,7 +13,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libumfref.so")
+# UMF_REF_SO selects another build of the harness (the threaded-MKL one for the all-cores CPU
+# baseline, which runs in a process of its own: the two MKL threading layers do not mix)
+REF_SO = os.environ.get("UMF_REF_SO") or os.path.join(ROOT, "oracle", "_ref", "libumfref.so")
 
 _libs = {}
 
