@@ -273,6 +273,25 @@ def low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
                                for k, v in sorted(kern.items()) if v["ms"] > 0 and v["flops"]}}
 
 
+def factor_graph_timing(plan, torch, reps=5):
+    """Factor() alone, direct launches against the captured HIP graph (umf_set_graph): mean ms
+    of `reps` calls each after one untimed call (the graph's capture)."""
+    plan.profile(False)
+    res = {}
+    for mode in (False, True):
+        plan.use_graph(mode)
+        plan.Factor()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.Factor()
+        torch.cuda.synchronize()
+        res["graph_ms" if mode else "direct_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    plan.use_graph(False)
+    progress(f"Factor alone: direct {res['direct_ms']:.2f} ms, graph {res['graph_ms']:.2f} ms")
+    return res
+
+
 def run_serve(args):
     """Serving: Poisson arrivals of coefficient-change + solve requests on one 3D 95^3
     factorization (a ~ lognormal(0,1), maxSideLength 8, seed 5); a request multiplies a box of
@@ -508,6 +527,7 @@ def main():
         "kernels_by_level": {k: round(v["ms"], 4) for k, v in sorted(prof_lv.items())},
         "cpu_baseline": None,
     }
+    out["factor_graph"] = factor_graph_timing(plan, torch)
     if not args.no_fp32:
         for prec in args.forms:
             out[prec + "_ir"] = low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)

@@ -212,6 +212,14 @@ int umf_profile_read(const umf_plan* plan, int i, char* name, int name_len, doub
 /* hipStream_t of the plan (for callers ordering their own work), returned as void*. */
 void* umf_stream(umf_plan* plan);
 
+/* Factorization as a HIP graph (no reference counterpart; the reference factors with host
+ * calls, src/DE/mf2d.cpp:449-500): with on != 0, umf_factor runs the whole level loop as one
+ * captured graph, captured at the first umf_factor after enabling (and again after a precision
+ * change) and replayed by the later ones; results are bit-identical to the direct launches.
+ * umf_update and the solves keep direct launches (their schedules depend on the call).  Not
+ * used while per-kernel timing is enabled. */
+int umf_set_graph(umf_plan* plan, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,12 @@ template <> __device__ __forceinline__ int mrow<float>(int q, int r) { return MR
 #define SPIN_LIMIT (1 << 22)  // polls of a bounded wait between workgroups of one launch
 
 // ----------------------------------------------------------------------------- assembly
+// Zero n ints (the factorization's error flags at its start: a kernel node, so that a captured
+// factorization holds no memset node).
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // Zero the lower triangle (row >= col) of each listed matrix.
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids, const MatDev* __restrict__ mats,

@@ -105,6 +105,12 @@ struct umf_plan {
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
+  // the full factorization captured as a HIP graph (umf_set_graph): its own id list, the
+  // precision it was captured in
+  bool use_graph = false;
+  hipGraphExec_t fgraph = nullptr;
+  int fgraph_prec = -1;
+  int* d_fids = nullptr;
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
   double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
   double* d_w = nullptr;   // solve work block n x RB, row-major
@@ -280,19 +286,24 @@ auto gemm_fn(bool split) {
 inline int per_item(int want, int cnt) { return std::max(1, std::min(want, (32768 + cnt - 1) / std::max(cnt, 1))); }
 
 // Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
+// ids_pre: the schedule's id list already on the device (a factorization being captured into a
+// graph: no synchronization, allocation or copy may enter the stream).
 template <class T>
-int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
+int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const int* ids_pre = nullptr) {
   const Plan& P = p->P;
   Sched S = make_sched(P, D);
-  int rc = ensure_ids(p, S.ids.size());
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
-  p->ids_h = S.ids;
-  if (!p->ids_h.empty())
-    HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
-                          p->stream));
-  HIPCHK(hipMemsetAsync(p->d_err, 0, sizeof(int) * 4, p->stream));
-  const int* ids = p->d_ids;
+  const int* ids = ids_pre;
+  if (!ids) {
+    int rc = ensure_ids(p, S.ids.size());
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
+    p->ids_h = S.ids;
+    if (!p->ids_h.empty())
+      HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                            p->stream));
+    ids = p->d_ids;
+  }
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
   const int NF = P.nfronts();
   auto lowerN = [](double N) { return N * (N + 1) / 2; };
   for (int ell = P.L - 1; ell >= 0; ell--) {
@@ -558,13 +569,49 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split) {
 
 // Factor over a dirty schedule in the precision the plan is set to (the FP32 front pool is
 // allocated on first use).
-int factor_any(umf_plan* p, const Plan::Dirty& D) {
+int factor_any(umf_plan* p, const Plan::Dirty& D, const int* ids_pre = nullptr) {
   if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
-    return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3);
+    return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3, ids_pre);
   }
-  return run_factor<double>(p, D, p->arena, false);
+  return run_factor<double>(p, D, p->arena, false, ids_pre);
+}
+
+// The full factorization as one HIP graph launch: captured on first use (and again after a
+// precision change) with the plan's fixed buffers and its own id list, then replayed.  Every
+// input it reads (operator values, front pool) lives at a fixed address, so a replay computes
+// what the eager launches would.
+int factor_graph(umf_plan* p, const Plan::Dirty& D) {
+  if (!p->fgraph || p->fgraph_prec != p->prec) {
+    if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
+    p->fgraph = nullptr;
+    Sched S = make_sched(p->P, D);
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->d_fids) (void)hipFree(p->d_fids);
+    p->d_fids = nullptr;
+    HIPCHK(hipMalloc((void**)&p->d_fids, sizeof(int) * std::max<size_t>(S.ids.size(), 1)));
+    if (!S.ids.empty()) HIPCHK(hipMemcpy(p->d_fids, S.ids.data(), sizeof(int) * S.ids.size(), hipMemcpyHostToDevice));
+    if (p->prec != UMF_PREC_FP64 && !p->arena32)
+      HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
+    HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = factor_any(p, D, p->d_fids);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+    if (rc || ec != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(UMF_EDEVICE, std::string("graph capture failed: ") + hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&p->fgraph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      p->fgraph = nullptr;
+      return fail(UMF_EDEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+    }
+    p->fgraph_prec = p->prec;
+  }
+  HIPCHK(hipGraphLaunch(p->fgraph, p->stream));
+  return UMF_OK;
 }
 
 // Copy cnt front-pool entries at offset off of the current factors to the host as double.
@@ -951,6 +998,8 @@ void free_plan(umf_plan* p) {
                   p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off, p->d_arrive};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
+  if (p->d_fids) (void)hipFree(p->d_fids);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
@@ -1358,7 +1407,7 @@ int umf_factor(umf_plan* p) {
   p->factored = false;
   p->fact_prec = p->prec;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
-  int rc = factor_any(p, D);
+  int rc = p->use_graph && !p->prof.on ? factor_graph(p, D) : factor_any(p, D);
   if (rc) return rc;
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -1700,6 +1749,13 @@ int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   p->prec = precision;
   p->ir_tol = tol;
   p->ir_max = max_iter;
+  return UMF_OK;
+}
+
+int umf_set_graph(umf_plan* p, int on) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  p->use_graph = on != 0;
   return UMF_OK;
 }
 

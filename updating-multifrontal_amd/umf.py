@@ -83,6 +83,7 @@ _front_block = _sig("umf_get_front_block", C.c_int, [_vp, C.c_int, C.c_int, C.c_
 _sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip])
 _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 _set_prec = _sig("umf_set_precision", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
+_set_graph = _sig("umf_set_graph", C.c_int, [_vp, C.c_int])
 _batch_create = _sig("umf_batch_create", C.c_int, [C.c_int, _ip, C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp),
                                                    C.c_int, C.POINTER(_vp)])
 _batch_info = _sig("umf_batch_info", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip])
@@ -91,7 +92,7 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
-            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision",
+            "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision", "umf_set_graph",
             "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load"]
 
 
@@ -169,6 +170,11 @@ class _MF:
         with the Schur and trailing-update GEMMs on BF16 MFMA over two bfloat16 pieces of each
         FP32 operand (3 piece products, FP32 accumulation).  Factor() again afterwards."""
         _check(_set_prec(self._h, PRECISIONS[precision], float(tol), int(max_iter)))
+
+    def use_graph(self, on=True):
+        """Run Factor() as one captured HIP graph (captured at the next Factor, then replayed;
+        bit-identical to the direct launches)."""
+        _check(_set_graph(self._h, 1 if on else 0))
 
     # ---- the reference's public API
     def Factor(self):
