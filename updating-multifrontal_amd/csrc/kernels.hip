@@ -612,9 +612,14 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
 // block: each one counts itself in arrive[front] once its loads are done, and the last one
 // (dispatched after the others, so the wait is short and always ends) waits for the full count,
 // then resets it for the next launch.
+// Lazy column groups (kstart < jb or dcap > 0): the group's earlier block columns [kstart, jb)
+// have not been applied to this block column, so each row tile first takes the GEMM over them
+// (left-looking inside the group), and a row tile that is the diagonal block of a later block
+// column below dcap takes this block's rank-64 update, so that the later step finds its
+// diagonal block complete.  The group's trailing update at its end is unchanged.
 template <class T>
 __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                     T* __restrict__ A, int jb, int xt, int nf,
+                                                     T* __restrict__ A, int jb, int kstart, int dcap, int xt, int nf,
                                                      int* __restrict__ arrive, int* __restrict__ err) {
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
@@ -624,20 +629,46 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
   const int kb = min(NB, F.n - jb), s0 = jb + kb;
   const int nt = (F.N - s0 + NB - 1) / NB, nw = max(nt, 1);
   if (tile >= nw) return;
-  __shared__ DiagSmem<T> sd;
-  __shared__ T X[NB * (NB + 1)];
+  __shared__ union {
+    TileSmemT<T, 64> g;  // the in-group GEMM's stages, before the block is loaded
+    struct {
+      DiagSmem<T> sd;
+      T X[NB * (NB + 1)];
+    } f;
+  } sm;
+  DiagSmem<T>& sd = sm.f.sd;
+  T* X = sm.f.X;
   const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
   const int w = wave_id(), wr = w >> 1, wc = w & 1;
   T* blk = A + F.off + (i64)jb * F.N + jb;
-  const int row0 = s0 + NB * tile, nrow = tile < nt ? min(NB, F.N - row0) : 0;
+  const int row0 = __builtin_amdgcn_readfirstlane(s0 + NB * tile), nrow = tile < nt ? min(NB, F.N - row0) : 0;
   T* pan = A + F.off + (i64)jb * F.N + row0;
+  V4<T> acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  const int K = nrow > 0 ? jb - kstart : 0;
+  if (K > 0) {
+    const T* G = A + F.off + (i64)kstart * F.N;
+    tile_gemm_nt_t<T, 64>(acc, G + row0, F.N, nrow, G + jb, F.N, kb, K, sm.g);
+  }
+  // X = the tile minus the GEMM (its stages are done: the GEMM ends on a barrier)
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : T(0);
+      }
 #pragma unroll
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e & 63, j = e >> 6;
     T v = T(0);  // strict upper part: the inverse, initially zero
     if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
     sd.Ts[j * LD + i] = v;
-    X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] : T(0);
   }
   __syncthreads();  // every load of this workgroup has returned
   const bool writer = tile == nw - 1;
@@ -660,7 +691,6 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
   if (nrow == 0) return;
   // Y = X Linv^T with Linv(j, k) = U[j*LD + k] for k <= j (the entries above are L's)
   const T* U = sd.Ts + LD;
-  V4<T> acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -687,6 +717,57 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
         if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
+      }
+  if (row0 >= min(F.n, dcap)) return;
+  // rank-64 update of the later diagonal block D = A(row0.., row0..) of the group: D -= Y Y^T,
+  // lower triangle only (the wave above the diagonal and the upper block of the diagonal waves
+  // skip)
+  __syncthreads();  // every wave is done reading X
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
+      }
+  __syncthreads();
+  if (wr < wc) return;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  auto syrk = [&](auto diag_wave) {
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = k4 * 4 + q;
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        const T bv = X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) {
+          if (decltype(diag_wave)::value && bj > bi) continue;
+          const T av = X[k * (NB + 1) + wc * 32 + bj * 16 + c16];
+          acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+        }
+      }
+    }
+  };
+  if (wr == wc)
+    syrk(std::true_type{});
+  else
+    syrk(std::false_type{});
+  const int dn = min(NB, F.n - row0);
+  T* D = A + F.off + (i64)row0 * F.N + row0;
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] -= acc[bi][bj][r];
       }
 }
 

@@ -412,14 +412,29 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
       const double nc = cend - s, nr = rend - s;
       return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
     };
+    // right-looking levels: a column group whose first step runs fused (k_chol_dpanel) applies
+    // its block columns lazily -- each step's panel takes the GEMM over the group's earlier
+    // columns and updates the group's later diagonal blocks -- instead of a group GEMM launch
+    // after every step (one launch fewer on the step chain)
+    bool lazy = false;
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
       xcd = act >= 64;
       const int g0 = k0 / GW * GW, gend = g0 + GW;
+      int rows = 0;
+      for (int i = 0; i < act; i++) {
+        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+        if (k0 < F.n) rows = std::max(rows, F.n + F.m - k0 - std::min(NB, F.n - k0));
+      }
+      // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
+      // repeats the diagonal factorization in each) run diagonal block and panel in one launch
+      const int ptd = std::max(1, (rows + NB - 1) / NB);
+      if (k0 == g0) lazy = !left && (i64)ptd * act <= 512;
+      const bool fused = !left && (lazy || (i64)ptd * act <= 512);
       const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
       const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
       const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
-      int rows = 0, tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
+      int tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
       double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
       seg_t.assign(act, 0);
       seg_f.assign(act, 0.0);
@@ -428,19 +443,19 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
         if (k0 >= F.n) continue;
         int kb = std::min(NB, F.n - k0), N = F.n + F.m;
-        double K = left ? k0 : 0;
-        rows = std::max(rows, N - k0 - kb);
+        double K = left ? k0 : lazy ? k0 - g0 : 0;
         double s0 = k0 + kb, nr = N - s0;
         fdiag += (double)kb * kb * kb / 3.0;
-        // TRSM (substitution count) + left-looking GEMM + the future diagonal tile's update
+        // TRSM (substitution count) + left-looking GEMM + the later diagonal blocks' update
+        const double dcols = left ? F.n - s0 : lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
         seg_t[i] = (N - k0 - kb + NB - 1) / NB;
-        seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + (left ? (F.n - s0) * kb * kb : 0.0);
+        seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + dcols * kb * kb;
         seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
         fpan += seg_f[i];
         bpan += seg_b[i];
         if (left) continue;
         const int ce = std::min(F.n, gend);
-        if (k0 + kb < ce) {
+        if (!lazy && k0 + kb < ce) {
           tin = std::max(tin, tiles_of(k0 + kb, N, ce));
           const double e = lower_elems(s0, N, ce);
           fin += 2.0 * kb * e;
@@ -467,13 +482,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
           }
         }
       }
-      // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
-      // repeats the diagonal factorization in each) run diagonal block and panel in one launch
-      const int ptd = std::max(1, (rows + NB - 1) / NB);
-      const bool fused = !left && (i64)ptd * act <= 512;
       if (fused)
         launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, act),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, xcd ? ptd : 0, act, p->d_arrive, p->d_err);
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, lazy ? g0 : k0, lazy ? gend : 0,
+               xcd ? ptd : 0, act, p->d_arrive, p->d_err);
       else
         launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
                arena, k0, p->d_err);
