@@ -377,6 +377,8 @@ def main():
                     help="lower-precision forms measured beside FP64 (each refined in FP64)")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
     ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)  # child of the CPU baseline
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the plan's autotuner (GEMM tile size, outer group width) before timing")
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
                     help="serve95: request arrival rates (per second)")
     ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
@@ -429,6 +431,10 @@ def main():
 
     progress(f"workload {args.workload}: n={n}, {plan.levels} levels, front pool "
              f"{plan.stats()['bytes_front_pool'] / 1e9:.1f} GB")
+    tuned = None
+    if args.autotune:
+        tuned = plan.autotune(reps=2)
+        progress(f"autotune: {tuned}")
     for i in range(args.warmup):
         step()
         progress(f"warmup step {i} done")
@@ -510,6 +516,8 @@ def main():
             "workload": wdesc,
             "nrhs": nrhs,
             "precision": "fp64",
+            "tune": plan.tune(),
+            "autotuned": tuned is not None,
         },
         "t_factor_ms": tf / K,
         "t_update_ms": tu / K,

@@ -220,6 +220,19 @@ void* umf_stream(umf_plan* plan);
  * used while per-kernel timing is enabled. */
 int umf_set_graph(umf_plan* plan, int on);
 
+/* Tuning of the factorization's GEMM stages (no reference counterpart; the reference's BLAS
+ * picks its own blocking): gw2 = width of the outer column groups of the two-level trailing
+ * update (256..8192, a multiple of 256; default 1024), tile = 64 or 128 for the trailing and
+ * Schur GEMM tiles, or 0 (default) for 128 on FP32 fronts when a stage still has 2048 such
+ * tiles, else 64.  The tile size does not change results (each entry accumulates the same
+ * products in the same order); gw2 changes the order of the trailing updates. */
+int umf_set_tune(umf_plan* plan, int gw2, int tile);
+int umf_get_tune(const umf_plan* plan, int* gw2, int* tile);
+/* The autotuner: factors the plan's current operator in its precision with every candidate
+ * (gw2 in {1024, 2048} x tile in {0, 64, 128}), `reps` times each, keeps the fastest by the
+ * factorization's event time (*best_ms, may be NULL) and refactors with it. */
+int umf_autotune(umf_plan* plan, int reps, double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,11 @@ struct umf_plan {
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
+  // tuning of the factorization's GEMM stages (umf_set_tune / umf_autotune): the outer column
+  // group width of the two-level trailing update, and the trailing/Schur tile size (0: 128 for
+  // FP32 fronts when that still yields 2048 tiles, else 64)
+  int tune_gw2 = 1024;
+  int tune_tile = 0;
   // the full factorization captured as a HIP graph (umf_set_graph): its own id list, the
   // precision it was captured in
   bool use_graph = false;
@@ -385,7 +390,9 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     // C traffic: with K = 1024 the 3D 95^3 FP64 trailing update goes from 43 to 52 TFLOP/s, and
     // the FP32 / BF16-split tiles of a 16k-front microbenchmark from 76 / 103 to 126 / 247;
     // GW2 = 2048 measured the same as 1024.
-    const int GW2 = 4 * GW;
+    const int GW2 = p->tune_gw2;
+    // 128 x 128 GEMM tiles for a stage of `tiles128` such tiles in all
+    auto wide = [&](i64 tiles128) { return p->tune_tile == 128 || (p->tune_tile == 0 && f32 && tiles128 >= 2048); };
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     bool xcd = L.f_cnt >= 64;
@@ -514,7 +521,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
       auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
-        if (f32 && (i64)t128 * act >= 2048)
+        if (wide((i64)t128 * act))
           launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, act), ids + L.f_off,
                  (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
         else
@@ -554,7 +561,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         const bool xc = c >= 64;
         auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
         if (tiles > 0) {
-          if (sizeof(T) == 4 && (i64)tiles128 * c >= 2048)
+          if (wide((i64)tiles128 * c))
             launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split), grid(tiles128), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
           else
@@ -1762,6 +1769,62 @@ int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   p->ir_tol = tol;
   p->ir_max = max_iter;
   return UMF_OK;
+}
+
+int umf_set_tune(umf_plan* p, int gw2, int tile) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (gw2 < 256 || gw2 > 8192 || gw2 % 256 != 0)
+    return fail(UMF_EINVAL, "outer group width must be 256..8192, a multiple of 256");
+  if (tile != 0 && tile != 64 && tile != 128) return fail(UMF_EINVAL, "tile must be 0 (by size), 64 or 128");
+  if (gw2 != p->tune_gw2 || tile != p->tune_tile) {
+    if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);  // captured with the old settings
+    p->fgraph = nullptr;
+  }
+  p->tune_gw2 = gw2;
+  p->tune_tile = tile;
+  return UMF_OK;
+}
+
+int umf_get_tune(const umf_plan* p, int* gw2, int* tile) {
+  if (!p || !gw2 || !tile) return fail(UMF_EINVAL, "null argument");
+  *gw2 = p->tune_gw2;
+  *tile = p->tune_tile;
+  return UMF_OK;
+}
+
+int umf_autotune(umf_plan* p, int reps, double* best_ms) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  reps = std::max(reps, 1);
+  const bool graph = p->use_graph, prof = p->prof.on;
+  p->use_graph = false;  // direct launches while timing
+  p->prof.on = false;
+  int best_g = p->tune_gw2, best_t = p->tune_tile, rc = UMF_OK;
+  double best = 1e300;
+  for (int g : {1024, 2048})
+    for (int t : {0, 64, 128}) {
+      if (t == 0 && p->prec == UMF_PREC_FP64) continue;  // the rule picks 64 for FP64 fronts
+      p->tune_gw2 = g;
+      p->tune_tile = t;
+      double ms = 1e300;
+      for (int r = 0; r < reps && rc == UMF_OK; r++) {
+        rc = umf_factor(p);
+        ms = std::min(ms, (double)p->stats.t_factor_ms);
+      }
+      if (rc) break;
+      if (ms < best) {
+        best = ms;
+        best_g = g;
+        best_t = t;
+      }
+    }
+  p->use_graph = graph;
+  p->prof.on = prof;
+  if (rc) return rc;
+  rc = umf_set_tune(p, best_g, best_t);
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
+  return umf_factor(p);  // the factors of the chosen settings
 }
 
 int umf_set_graph(umf_plan* p, int on) {
