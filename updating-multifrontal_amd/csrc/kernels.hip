@@ -1323,16 +1323,26 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
   const int rowc = rok ? row : r0;
   V4<T> acc = vzero<T>();
   const int jmax = interior ? b : nbi;
+  // this block's inverse (factor data, final before the launch) is read before any wait
+  T li[NB / 4];
+  if (interior) {
+    const T* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) li[k4] = Li[(4 * k4 + q) * NB + 16 * w + c16];
+  }
   for (int j = 0; j < jmax; j++) {
-    wait_block(fl + j, epoch, err);
     const int jb = NB * j, kb = min(NB, F.n - jb);
     T av[NB / 4], bv[NB / 4];
 #pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block j's wait
+      const int k = 4 * k4 + q;
+      av[k4] = (rok && k < kb) ? L[(i64)(jb + k) * F.N + rowc] : T(0);
+    }
+    wait_block(fl + j, epoch, err);
+#pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      const bool kin = k < kb;
-      av[k4] = (rok && kin) ? L[(i64)(jb + k) * F.N + rowc] : T(0);
-      bv[k4] = kin ? X[(i64)(jb + k) * RB + c16] : T(0);
+      bv[k4] = k < kb ? X[(i64)(jb + k) * RB + c16] : T(0);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1351,13 +1361,9 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
     vb[i][c16] = r0 + i < rend ? W[(i64)di[r0 + i] * RB + c16] - acc[r] : T(0);
   }
   __syncthreads();
-  const T* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
   V4<T> a2 = vzero<T>();
 #pragma unroll
-  for (int k4 = 0; k4 < NB / 4; k4++) {
-    const int c = 4 * k4 + q;
-    a2 = mfma4(Li[c * NB + 16 * w + c16], vb[c][c16], a2);
-  }
+  for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(li[k4], vb[4 * k4 + q][c16], a2);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = r0 + 16 * w + mrow<T>(q, r);
@@ -1409,16 +1415,26 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 #pragma unroll
       for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
   }
+  // this block's inverse (factor data, final before the launch) is read before any wait
+  T li[NB / 4];
+  {
+    const T* Li = A + F.dinv_off + (i64)ib * NB * NB;
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) li[k4] = Li[(16 * w + c16) * NB + 4 * k4 + q];
+  }
   for (int kbk = nbi - 1; kbk > ib; kbk--) {  // interior blocks below, in publication order
-    wait_block(fl + kbk, epoch, err);
     const int k0 = NB * kbk, kn = min(NB, F.n - k0);
     T av[NB / 4], bv[NB / 4];
 #pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block kbk's wait
+      const int k = 4 * k4 + q;
+      av[k4] = (k < kn && cin) ? Lc[k0 + k] : T(0);
+    }
+    wait_block(fl + kbk, epoch, err);
+#pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      const bool kin = k < kn;
-      av[k4] = (kin && cin) ? Lc[k0 + k] : T(0);
-      bv[k4] = kin ? X[(i64)(k0 + k) * RB + c16] : T(0);
+      bv[k4] = k < kn ? X[(i64)(k0 + k) * RB + c16] : T(0);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1429,13 +1445,9 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
     zb[c][c16] = c < kb ? W[(i64)di[jb + c] * RB + c16] - acc[r] : T(0);
   }
   __syncthreads();
-  const T* Li = A + F.dinv_off + (i64)ib * NB * NB;
   V4<T> a2 = vzero<T>();
 #pragma unroll
-  for (int k4 = 0; k4 < NB / 4; k4++) {
-    const int c = 4 * k4 + q;
-    a2 = mfma4(Li[(16 * w + c16) * NB + c], zb[c][c16], a2);
-  }
+  for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(li[k4], zb[4 * k4 + q][c16], a2);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = 16 * w + mrow<T>(q, r);
