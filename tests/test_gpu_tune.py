@@ -71,3 +71,30 @@ def test_autotune_picks_a_candidate_and_solves():
     assert res["tile"] in (0, 64, 128)
     x = G.Solve(b)
     assert relerr(x, O.solve(b)) <= 1e-8
+
+
+def test_autotune_with_graph_and_failures():
+    """A captured factorization is recaptured after the autotuner changed the settings (the
+    replay equals a direct factorization with them); on an operator that fails every candidate
+    the settings are left as they were."""
+    import umf
+    A, G = _mk(dims=(19, 21, 17), maxs=4, seed=7)
+    b = np.random.default_rng(5).standard_normal(A.n)
+    G.use_graph(True)
+    G.Factor()  # captured with the defaults
+    res = G.autotune(reps=1)
+    G.Factor()  # a replay of the settings the autotuner chose
+    x = G.Solve(b)
+    _, H = _mk(dims=(19, 21, 17), maxs=4, seed=7)
+    H.set_tune(res["gw2"], res["tile"])
+    H.Factor()
+    assert np.array_equal(x, H.Solve(b))
+
+    d = (31, 31)
+    bad = laplacian(d, coeff_field(d, "uniform", seed=1))
+    bad.val = -bad.val  # negative definite
+    B = umf.MF2D(bad, *d, width=1, maxSideLength=3)
+    B.set_tune(2048, 128)
+    with pytest.raises(ArithmeticError):
+        B.autotune(reps=1)
+    assert B.tune() == {"gw2": 2048, "tile": 128}

@@ -1797,9 +1797,12 @@ int umf_autotune(umf_plan* p, int reps, double* best_ms) {
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
   reps = std::max(reps, 1);
   const bool graph = p->use_graph, prof = p->prof.on;
+  const int g_in = p->tune_gw2, t_in = p->tune_tile;
+  if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);  // captured with the settings in force now
+  p->fgraph = nullptr;
   p->use_graph = false;  // direct launches while timing
   p->prof.on = false;
-  int best_g = p->tune_gw2, best_t = p->tune_tile, rc = UMF_OK;
+  int best_g = g_in, best_t = t_in, rc = UMF_OK;
   double best = 1e300;
   for (int g : {1024, 2048})
     for (int t : {0, 64, 128}) {
@@ -1820,7 +1823,11 @@ int umf_autotune(umf_plan* p, int reps, double* best_ms) {
     }
   p->use_graph = graph;
   p->prof.on = prof;
-  if (rc) return rc;
+  if (rc) {  // a candidate failed (a non-SPD operator fails them all): the settings stay as they were
+    p->tune_gw2 = g_in;
+    p->tune_tile = t_in;
+    return rc;
+  }
   rc = umf_set_tune(p, best_g, best_t);
   if (rc) return rc;
   if (best_ms) *best_ms = best;
