@@ -402,14 +402,14 @@ __device__ __forceinline__ bool wg_tile(int xt, int nf, int& tile, int& f) {
 
 // ----------------------------------------------------------------------------- factorization
 // Blocked partial Cholesky of the pivot columns [0, n) of each front, 64 columns per block.
-//   k_chol_diag:  the diagonal block (already carrying every earlier block's update) is
-//                 factored in LDS, T = L_jj L_jj^T, together with Linv_jj = L_jj^{-1}.
+//   k_chol_diag:  the diagonal block is factored in LDS, T = L_jj L_jj^T, together with
+//                 Linv_jj = L_jj^{-1}.  Left-looking it first takes the update by every
+//                 finished column, D = A(jb.., jb..) - L(jb.., 0:jb) L(jb.., 0:jb)^T;
+//                 right-looking the block already carries it.
 //   k_chol_panel: rows below the block: X = (A(rows, jb block) - L(rows, 0:K) L(jb.., 0:K)^T)
 //                 Linv_jj^T, i.e. the panel TRSM as a product with the inverted diagonal block.
-//                 Left-looking (K = jb): the off-diagonal tiles are formed once, from the
-//                 untouched input; the workgroup then applies its rank-64 update to the one
-//                 future diagonal tile it owns (rows = cols = its row range).  Right-looking
-//                 (K = 0): the trailing region was updated by k_gemm<0>.
+//                 Left-looking (K = jb): every tile is formed once, from the untouched input.
+//                 Right-looking (K = 0): the trailing region was updated by k_gemm<0>.
 // A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
 // acc += P Q^T for 16 x 16 blocks held in LDS, one wave, one MFMA per 4 k (K = 16):
 // P(i,k) = P[i*psi + k*psk], Q(j,k) = Q[j*qsj + k*qsk]; accumulator r of a lane is
@@ -585,23 +585,42 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
 }
 
 template <class T>
+// left = 1 (left-looking levels): the block first takes the update by every finished column,
+// D = A(jb.., jb..) - L(jb.., 0:jb) L(jb.., 0:jb)^T, on MFMA (the panel steps leave the later
+// diagonal blocks untouched).
 __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   T* __restrict__ A, int jb, int* __restrict__ err) {
+                                                   T* __restrict__ A, int jb, int left, int* __restrict__ err) {
   const int fid = ids[blockIdx.x];
   const FrontDev F = fr[fid];
   if (jb >= F.n) return;
   const int kb = min(NB, F.n - jb);
-  __shared__ DiagSmem<T> sm;
-  const int t = threadIdx.x;
+  __shared__ union {
+    TileSmemT<T, 64> g;  // the update's stages, before the block is loaded
+    DiagSmem<T> d;
+  } sm;
   T* blk = A + F.off + (i64)jb * F.N + jb;
+  V4<T> acc[2][2];
 #pragma unroll
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e & 63, j = e >> 6;
-    T v = T(0);  // strict upper part: the inverse, initially zero
-    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
-    sm.Ts[j * LD + i] = v;
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  if (left && jb > 0) {
+    const T* Lr = A + F.off + jb;  // rows jb.., columns 0:jb
+    tile_gemm_nt_t<T, 64>(acc, Lr, F.N, kb, Lr, F.N, kb, jb, sm.g);
   }
-  diag_core(sm, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
+  // strict upper part: the inverse, initially zero; identity padding beyond kb
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        T v = T(0);
+        if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] - acc[bi][bj][r] : (i == j ? T(1) : T(0));
+        sm.d.Ts[j * LD + i] = v;
+      }
+  diag_core(sm.d, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
 }
 
 // Right-looking levels: the diagonal block's factorization and the panel below it in one
@@ -872,56 +891,6 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
         if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
-      }
-  if (!left || row0 >= F.n) return;
-  // rank-64 update of the future diagonal tile D = A(row0.., row0..): D -= Y Y^T, lower
-  // triangle only (the wave above the diagonal and the upper block of the diagonal waves skip)
-  __syncthreads();
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
-      }
-  __syncthreads();
-  if (wr < wc) return;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
-  auto syrk = [&](auto diag_wave) {
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {
-      const int k = k4 * 4 + q;
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) {
-        const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
-          if (decltype(diag_wave)::value && bj > bi) continue;
-          const T av = sm.c.X[k * (NB + 1) + wc * 32 + bj * 16 + c16];
-          acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
-        }
-      }
-    }
-  };
-  if (wr == wc)
-    syrk(std::true_type{});
-  else
-    syrk(std::false_type{});
-  const int dn = min(NB, F.n - row0);
-  T* D = A + F.off + (i64)row0 * F.N + row0;
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] -= acc[bi][bj][r];
       }
 }
 

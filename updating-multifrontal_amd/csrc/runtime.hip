@@ -452,9 +452,11 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         int kb = std::min(NB, F.n - k0), N = F.n + F.m;
         double K = left ? k0 : lazy ? k0 - g0 : 0;
         double s0 = k0 + kb, nr = N - s0;
-        fdiag += (double)kb * kb * kb / 3.0;
-        // TRSM (substitution count) + left-looking GEMM + the later diagonal blocks' update
-        const double dcols = left ? F.n - s0 : lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
+        // the block's factorization (+ left-looking: its update by the K finished columns)
+        fdiag += (double)kb * kb * kb / 3.0 + (double)kb * kb * K * (left ? 1 : 0);
+        // TRSM (substitution count) + left-looking GEMM + (lazy groups) the later diagonal
+        // blocks' update
+        const double dcols = lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
         seg_t[i] = (N - k0 - kb + NB - 1) / NB;
         seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + dcols * kb * kb;
         seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
@@ -495,7 +497,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
                xcd ? ptd : 0, act, p->d_arrive, p->d_err);
       else
         launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
-               arena, k0, p->d_err);
+               arena, k0, left ? 1 : 0, p->d_err);
       // the panel launches over at most a few runs of the active fronts: a run ends where the
       // largest row-tile count of the fronts after it has halved, so that a level of many small
       // fronts and a few tall ones does not give every front the tallest one's grid
