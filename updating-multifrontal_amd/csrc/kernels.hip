@@ -191,7 +191,9 @@ struct TileSmemT {
   T Bs[2][16][TS];
 };
 
-template <class T, int TS>
+// LOWER (P = Q, only the lower triangle of the product wanted): the wave above the diagonal
+// (wr < wc) only stages operands.
+template <class T, int TS, bool LOWER = false>
 __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], const T* __restrict__ P, int ldp,
                                                int pn, const T* __restrict__ Q, int ldq, int qn, int K,
                                                TileSmemT<T, TS>& sm) {
@@ -238,9 +240,10 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
   fetch(16);
   __syncthreads();
   int b = 0;
+  const bool mma = !LOWER || wr >= wc;
   for (int kc = 0; kc < K; kc += 16, b ^= 1) {
 #pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) {
+    for (int k4 = 0; k4 < 4 && mma; k4++) {
       const int kr = k4 * 4 + (lane >> 4);
       T av[NBK], bv[NBK];
 #pragma unroll
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
   if (left && jb > 0) {
     const T* Lr = A + F.off + jb;  // rows jb.., columns 0:jb
-    tile_gemm_nt_t<T, 64>(acc, Lr, F.N, kb, Lr, F.N, kb, jb, sm.g);
+    tile_gemm_nt_t<T, 64, true>(acc, Lr, F.N, kb, Lr, F.N, kb, jb, sm.g);
   }
   // strict upper part: the inverse, initially zero; identity padding beyond kb
 #pragma unroll
