@@ -387,6 +387,24 @@ __device__ __forceinline__ int acc_col(int bj, int r) {
   return ((threadIdx.x >> 6) & 1) * 32 + bj * 16 + mrow<T>((threadIdx.x & 63) >> 4, r);
 }
 
+// The 64 x 64 tile at p (column stride ld, nr x nc valid, nr, nc >= 1) in the accumulator
+// layout, entries outside it zero.  Every load is unconditional at a clamped address, so the
+// loads are in flight together instead of one guarded load and wait per entry.  `on` false
+// gives zeros without reading.
+template <class T>
+__device__ __forceinline__ void tile_read(T (&v)[2][2][4], const T* __restrict__ p, i64 ld, int nr, int nc, bool on) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        const T x = on ? p[(i64)min(j, nc - 1) * ld + min(i, nr - 1)] : T(0);
+        v[bi][bj][r] = (i < nr && j < nc) ? x : T(0);
+      }
+}
+
 // Workgroup -> (tile, front id index).  xt == 0: 2-D grid (tile = blockIdx.x, front =
 // blockIdx.y).  xt > 0: 1-D grid of 8 * ceil(nf / 8) * xt workgroups laid out so that the xt
 // tiles of a front get workgroup ids congruent mod 8; ids are dealt to the 8 XCDs round-robin,
@@ -607,6 +625,8 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  T bv[2][2][4];  // the block, read before the update's GEMM
+  tile_read(bv, blk, F.N, kb, kb, true);
   if (left && jb > 0) {
     const T* Lr = A + F.off + jb;  // rows jb.., columns 0:jb
     tile_gemm_nt_t<T, 64, true>(acc, Lr, F.N, kb, Lr, F.N, kb, jb, sm.g);
@@ -620,7 +640,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, 
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
         T v = T(0);
-        if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] - acc[bi][bj][r] : (i == j ? T(1) : T(0));
+        if (i >= j) v = (i < kb && j < kb) ? bv[bi][bj][r] - acc[bi][bj][r] : (i == j ? T(1) : T(0));
         sm.d.Ts[j * LD + i] = v;
       }
   diag_core(sm.d, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
@@ -670,6 +690,10 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  // the row tile and the diagonal block, read before the in-group GEMM
+  T pv[2][2][4], bv[2][2][4];
+  tile_read(pv, pan, F.N, max(nrow, 1), kb, nrow > 0);
+  tile_read(bv, blk, F.N, kb, kb, true);
   const int K = nrow > 0 ? jb - kstart : 0;
   if (K > 0) {
     const T* G = A + F.off + (i64)kstart * F.N;
@@ -683,15 +707,11 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pan[(i64)j * F.N + i] - acc[bi][bj][r] : T(0);
+        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
+        T v = T(0);  // strict upper part: the inverse, initially zero; identity padding
+        if (i >= j) v = (i < kb && j < kb) ? bv[bi][bj][r] : (i == j ? T(1) : T(0));
+        sd.Ts[j * LD + i] = v;
       }
-#pragma unroll
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e & 63, j = e >> 6;
-    T v = T(0);  // strict upper part: the inverse, initially zero
-    if (i >= j) v = (i < kb && j < kb) ? blk[(i64)j * F.N + i] : (i == j ? T(1) : T(0));  // identity padding
-    sd.Ts[j * LD + i] = v;
-  }
   __syncthreads();  // every load of this workgroup has returned
   const bool writer = tile == nw - 1;
   if (t == 0) {
@@ -776,12 +796,14 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
       }
     }
   };
+  const int dn = min(NB, F.n - row0);
+  T* D = A + F.off + (i64)row0 * F.N + row0;
+  T dv[2][2][4];  // read while the product runs
+  tile_read(dv, D, F.N, dn, dn, true);
   if (wr == wc)
     syrk(std::true_type{});
   else
     syrk(std::false_type{});
-  const int dn = min(NB, F.n - row0);
-  T* D = A + F.off + (i64)row0 * F.N + row0;
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -789,7 +811,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] -= acc[bi][bj][r];
+        if (i < dn && j < dn && i >= j) D[(i64)j * F.N + i] = dv[bi][bj][r] - acc[bi][bj][r];
       }
 }
 
@@ -825,10 +847,13 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
   const int row0 = __builtin_amdgcn_readfirstlane(s0 + NB * tile);
   const int nrow = min(NB, F.N - row0);
   const int K = left ? jb : 0;
-  tile_gemm_nt_t<T, 64>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
   T* pan = A + F.off + (i64)jb * F.N + row0;
-  // the tile's input: the assembled front, or (fly) its CSR entries added below
+  // the tile's input: the assembled front, read before the GEMM (all loads in flight together,
+  // at clamped addresses), or (fly) its CSR entries added below
   const bool from_csr = fly && F.fly_off >= 0;
+  T pv[2][2][4];
+  tile_read(pv, pan, F.N, nrow, kb, !from_csr);
+  tile_gemm_nt_t<T, 64>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -836,8 +861,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        sm.c.X[j * (NB + 1) + i] =
-            (i < nrow && j < kb) ? (from_csr ? T(0) : pan[(i64)j * F.N + i]) - acc[bi][bj][r] : T(0);
+        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
       }
   const T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
   __syncthreads();
@@ -958,6 +982,27 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
   for (int i = 0; i < NBK; i++)
 #pragma unroll
     for (int j = 0; j < NBK; j++) acc[i][j] = vzero<T>();
+  // The tile's current values are read with unconditional loads at clamped (always valid)
+  // addresses -- entries outside the region are read and discarded -- so that the loads are in
+  // flight together instead of one guarded load and wait per entry.  64 x 64 tiles read theirs
+  // before the GEMM (its operands are other columns), 128 x 128 tiles 16 per lane after it.
+  T* C = A + F.off;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int BR = TS == 64 ? NBK : 1;  // 16-row blocks read per batch
+  T cv[BR][NBK][4];
+  auto read_c = [&](int bi0) {
+#pragma unroll
+    for (int b = 0; b < BR; b++)
+#pragma unroll
+      for (int bj = 0; bj < NBK; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = min(row0 + (w >> 1) * (TS / 2) + (bi0 + b) * 16 + (lane & 15), rend - 1);
+          const int j = min(col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r), cend - 1);
+          cv[b][bj][r] = C[(i64)j * F.N + i];
+        }
+  };
+  if constexpr (TS == 64) read_c(0);
   const T* base = A + F.off + (i64)kbase * F.N;
   if constexpr (!SPLIT) {
     __shared__ TileSmemT<T, TS> sm;
@@ -966,34 +1011,37 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
     __shared__ TileSmemBF<TS> sm;
     tile_gemm_nt_bf<TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   }
-  T* C = A + F.off;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int bi = 0; bi < NBK; bi++)
+  for (int bi0 = 0; bi0 < NBK; bi0 += BR) {
+    if constexpr (TS != 64) read_c(bi0);
 #pragma unroll
-    for (int bj = 0; bj < NBK; bj++)
+    for (int b = 0; b < BR; b++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = row0 + (w >> 1) * (TS / 2) + bi * 16 + (lane & 15);
-        const int j = col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r);
-        if (i >= rend || j >= cend || i < j) continue;
-        T* c = C + (i64)j * F.N + i;
-        if (MODE == 0) {
-          *c -= acc[bi][bj][r];
-        } else {
-          const int ti = i - F.n, tj = j - F.n;
-          int ri = -1, rj = -2;
+      for (int bj = 0; bj < NBK; bj++)
 #pragma unroll
-          for (int q = 0; q < 6; q++) {
-            if (q < F.nsep) {
-              if (ti >= F.sep_pos[q] && ti < F.sep_pos[q] + F.sep_len[q]) ri = q;
-              if (tj >= F.sep_pos[q] && tj < F.sep_pos[q] + F.sep_len[q]) rj = q;
-            }
-          }
+        for (int r = 0; r < 4; r++) {
+          const int bi = bi0 + b;
+          const int i = row0 + (w >> 1) * (TS / 2) + bi * 16 + (lane & 15);
+          const int j = col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r);
+          if (i >= rend || j >= cend || i < j) continue;
+          T* c = C + (i64)j * F.N + i;
           const T sv = -acc[bi][bj][r];
-          *c = (ri == rj) ? sv : (*c + sv);
+          if (MODE == 0) {
+            *c = cv[b][bj][r] + sv;
+          } else {
+            const int ti = i - F.n, tj = j - F.n;
+            int ri = -1, rj = -2;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+              if (q < F.nsep) {
+                if (ti >= F.sep_pos[q] && ti < F.sep_pos[q] + F.sep_len[q]) ri = q;
+                if (tj >= F.sep_pos[q] && tj < F.sep_pos[q] + F.sep_len[q]) rj = q;
+              }
+            }
+            *c = (ri == rj) ? sv : (cv[b][bj][r] + sv);
+          }
         }
-      }
+  }
 }
 
 
@@ -1005,13 +1053,28 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
 #define RB 16
 #define GU 4  // 16-row groups in flight per wave
 
+// *p when ok, else 0, with the load unconditional: p is valid either way (the caller clamps the
+// index), so the loads of an unrolled loop are issued together instead of one branch and wait
+// per load.
+template <class T>
+__device__ __forceinline__ T ld_or0(const T* p, bool ok) {
+  const T v = *p;
+  return ok ? v : T(0);
+}
+
 template <class T>
 __device__ __forceinline__ void gather_rhs(const FrontDev& F, T* __restrict__ X, const T* __restrict__ W,
                                            const int* __restrict__ di, const int* __restrict__ db) {
-  for (int e = threadIdx.x; e < F.N * RB; e += 256) {
-    const int i = e >> 4, r = e & 15;
-    const int d = i < F.n ? di[i] : (db ? db[i - F.n] : -1);
-    X[e] = d >= 0 ? W[(i64)d * RB + r] : T(0);
+  // interior rows, then boundary rows (zero without db); unrolled so that several rows' loads
+  // are in flight together
+  const int ni = F.n * RB, na = F.N * RB;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < ni; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
+  if (db) {
+#pragma unroll 4
+    for (int e = ni + threadIdx.x; e < na; e += 256) X[e] = W[(i64)db[(e >> 4) - F.n] * RB + (e & 15)];
+  } else {
+    for (int e = ni + threadIdx.x; e < na; e += 256) X[e] = T(0);
   }
 }
 
@@ -1038,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
       for (int k4 = 0; k4 < NB / 4; k4++) {
         const int k = 4 * k4 + q;
         const T av = Li[k * NB + 16 * w + c16];
-        const T bv = k < kb ? X[(i64)(jb + k) * RB + c16] : T(0);
+        const T bv = ld_or0(X + (i64)(jb + min(k, kb - 1)) * RB + c16, k < kb);
         acc = mfma4(av, bv, acc);
       }
       __syncthreads();
@@ -1053,15 +1116,17 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
     // rows below, GU 16-row groups per wave in flight; out-of-range rows read row N-1 and are
     // never written
     for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * GU) {
-      T av[GU][NB / 4];
+      T av[GU][NB / 4], xv[GU][4];
 #pragma unroll
       for (int u = 0; u < GU; u++) {
         const int row = min(g0 + 64 * u + c16, F.N - 1);
 #pragma unroll
         for (int k4 = 0; k4 < NB / 4; k4++) {
           const int k = 4 * k4 + q;
-          av[u][k4] = k < kb ? L[(i64)(jb + k) * F.N + row] : T(0);
+          av[u][k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + row, k < kb);
         }
+#pragma unroll
+        for (int r = 0; r < 4; r++) xv[u][r] = X[(i64)min(g0 + 64 * u + mrow<T>(q, r), F.N - 1) * RB + c16];
       }
 #pragma unroll
       for (int u = 0; u < GU; u++) {
@@ -1072,12 +1137,13 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int i = g0 + 64 * u + mrow<T>(q, r);
-          if (i < F.N) X[(i64)i * RB + c16] -= acc[r];
+          if (i < F.N) X[(i64)i * RB + c16] = xv[u][r] - acc[r];
         }
       }
     }
   }
   __syncthreads();
+#pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
 
@@ -1133,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
           const int i = i0 + 16 * u + 4 * k4 + q;
           const bool ok = cin && i < F.N;
           const int ic = min(i, F.N - 1);
-          av[u][k4] = ok ? Lc[ic] : T(0);
+          av[u][k4] = ld_or0(Lc + ic, ok);
           bv[u][k4] = X[(i64)ic * RB + c16];
         }
 #pragma unroll
@@ -1144,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int c = 16 * w + mrow<T>(q, r);
-      zb[c][c16] = c < kb ? X[(i64)(jb + c) * RB + c16] - acc[r] : T(0);
+      zb[c][c16] = ld_or0(X + (i64)(jb + min(c, kb - 1)) * RB + c16, c < kb) - (c < kb ? acc[r] : T(0));
     }
     __syncthreads();
     {  // x_j = Linv_j^T z: x(i, r) = sum_c Linv(c, i) z(c, r); wave w -> rows 16w..16w+15
@@ -1163,6 +1229,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
     }
   }
   __syncthreads();
+#pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
 
@@ -1195,8 +1262,8 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
       if (c == j)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = 16 * w + mrow<T>(q, r), gi = jb + i;
-          sb[i][c16] = i < kb ? W[(i64)di[gi] * RB + c16] - S[c][r] : T(0);
+          const int i = 16 * w + mrow<T>(q, r);
+          sb[i][c16] = ld_or0(W + (i64)di[jb + min(i, kb - 1)] * RB + c16, i < kb) - (i < kb ? S[c][r] : T(0));
         }
     __syncthreads();
     {  // x_j = Linv_j v: wave w -> rows 16w..16w+15
@@ -1221,11 +1288,12 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
       if (c < j || c >= nch) continue;
       const int row = NB * c + 16 * w + c16;
       const bool rok = row < F.N && row >= jb + kb;
+      const int rowc = min(row, F.N - 1);
       T av[NB / 4];
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) {
         const int k = 4 * k4 + q;
-        av[k4] = (rok && k < kb) ? L[(i64)(jb + k) * F.N + row] : T(0);
+        av[k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + rowc, rok && k < kb);
       }
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) S[c] = mfma4(av[k4], xb[4 * k4 + q][c16], S[c]);
@@ -1308,13 +1376,13 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block j's wait
       const int k = 4 * k4 + q;
-      av[k4] = (rok && k < kb) ? L[(i64)(jb + k) * F.N + rowc] : T(0);
+      av[k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + rowc, rok && k < kb);
     }
     wait_block(fl + j, epoch, err);
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      bv[k4] = k < kb ? X[(i64)(jb + k) * RB + c16] : T(0);
+      bv[k4] = ld_or0(X + (i64)(jb + min(k, kb - 1)) * RB + c16, k < kb);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1330,7 +1398,8 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = 16 * w + mrow<T>(q, r);
-    vb[i][c16] = r0 + i < rend ? W[(i64)di[r0 + i] * RB + c16] - acc[r] : T(0);
+    const bool ok = r0 + i < rend;
+    vb[i][c16] = ld_or0(W + (i64)di[min(r0 + i, rend - 1)] * RB + c16, ok) - (ok ? acc[r] : T(0));
   }
   __syncthreads();
   V4<T> a2 = vzero<T>();
@@ -1379,8 +1448,9 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
       for (int k4 = 0; k4 < 4; k4++) {
         const int i = i0 + 16 * u + 4 * k4 + q;
         const bool ok = i < F.N;
-        av[u][k4] = (ok && cin) ? Lc[i] : T(0);
-        bv[u][k4] = ok ? W[(i64)db[i - F.n] * RB + c16] : T(0);
+        const int ic = min(i, F.N - 1);
+        av[u][k4] = ld_or0(Lc + ic, ok && cin);
+        bv[u][k4] = ld_or0(W + (i64)db[ic - F.n] * RB + c16, ok);
       }
 #pragma unroll
     for (int u = 0; u < GU; u++)
@@ -1400,13 +1470,13 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block kbk's wait
       const int k = 4 * k4 + q;
-      av[k4] = (k < kn && cin) ? Lc[k0 + k] : T(0);
+      av[k4] = ld_or0(Lc + k0 + min(k, kn - 1), k < kn && cin);
     }
     wait_block(fl + kbk, epoch, err);
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      bv[k4] = k < kn ? X[(i64)(k0 + k) * RB + c16] : T(0);
+      bv[k4] = ld_or0(X + (i64)(k0 + min(k, kn - 1)) * RB + c16, k < kn);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1414,7 +1484,7 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int c = 16 * w + mrow<T>(q, r);
-    zb[c][c16] = c < kb ? W[(i64)di[jb + c] * RB + c16] - acc[r] : T(0);
+    zb[c][c16] = ld_or0(W + (i64)di[jb + min(c, kb - 1)] * RB + c16, c < kb) - (c < kb ? acc[r] : T(0));
   }
   __syncthreads();
   V4<T> a2 = vzero<T>();
