@@ -147,6 +147,15 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
 }
 
 // ----------------------------------------------------------------------------- MFMA tile GEMMs
+// Orders a wave's LDS accesses before its later ones, as seen by the other lanes of the same
+// wave (a wave's LDS operations complete in order; the fences keep the compiler from moving
+// accesses across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // the wave index as a scalar, so that branches on it are uniform
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -478,19 +487,19 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 
 // Cholesky of the 64 x 64 diagonal block at (jb, jb) of each front with its inverse, one
 // workgroup per front, as 4 x 4 blocks of 16 x 16 in LDS:
-//   for k: eliminate D_kk on [D_kk | I] without scaling the pivot rows (one thread per element,
-//   one barrier per column: step j reads column j and row j of the inverse part and writes only
-//   later columns / rows), then scale: L_kk = D' diag(r), Linv_kk = diag(r) V with r_j =
-//   1/sqrt(pivot j); panel L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one
-//   16-block per wave; then the off-diagonal blocks of the inverse,
-//   Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance i - j.
+//   for k: one wave eliminates D_kk on [D_kk | I] without scaling the pivot rows (elements in
+//   registers, 4 per lane; step j reads column j of D and row j of the inverse part V from LDS,
+//   and the lanes holding column j+1 / row j+1 write them back for the next step, ordered within
+//   the wave, with no workgroup barrier per pivot), then scales: L_kk = D' diag(r), Linv_kk =
+//   diag(r) V with r_j = 1/sqrt(pivot j); panel L_ik = A_ik Linv_kk^T and trailing
+//   A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave; then the off-diagonal blocks of the
+//   inverse, Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance i - j.
 // Ts[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
 // triangle: Linv(r, c) at U[r*LD + c], U = Ts + LD, for r >= c.
 #define LD (NB + 1)
 template <class T>
 struct DiagSmem {
   T Ts[LD * LD];
-  T rs[NB];
 };
 
 // Factor the block held in Ts (lower triangle D(i, j) at Ts[j*LD + i], kb x kb, identity
@@ -501,40 +510,81 @@ template <class T>
 __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, i64 ld, int kb, T* __restrict__ dinv,
                           int* __restrict__ err, int fid, int jb, bool store = true) {
   T* Ts = sm.Ts;
-  T* rs = sm.rs;
   T* U = Ts + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (t < NB) Ts[NB * LD + t] = T(0);
   __syncthreads();
   if (t < NB) U[t * LD + t] = 1.0;
-  const int er = t & 15, ec = t >> 4;  // element of a 16 x 16 block
   for (int k = 0; k < 4; k++) {
     const int K0 = 16 * k;
     T* Dk = Ts + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
     T* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
-    // (a) eliminate D_kk on [D_kk | I]: step j reads column j of D and row j of V and writes only
-    //     later columns / rows; the multiplier D(er, j) / D(j, j) uses 1/d (reciprocal + Newton)
-    for (int j = 0; j < 16; j++) {
-      __syncthreads();
-      const T d = Dk[j * LD + j];
-      const T ar = Dk[j * LD + er], ac = Dk[j * LD + ec], vj = Vk[j * LD + ec];
-      const T m = ar * rcp_nr(d);
-      if (er > j && ec > j && ec <= er) Dk[ec * LD + er] -= m * ac;
-      if (er > j && ec <= j) Vk[er * LD + ec] -= m * vj;
-      if (t == 0) {
-        rs[K0 + j] = d;  // pivot; turned into 1/sqrt(pivot) after the block
-        if (K0 + j < kb && !(d > T(0)) && atomicCAS(err, 0, 1) == 0) {
+    __syncthreads();
+    // (a) wave 0: lane (er, cg) holds D(er, c) and V(er, c), c = cg + 4u (entries above the
+    //     diagonal ride along unused).  Step j: rows er > j take m = D(er, j) / D(j, j) (1/d by
+    //     reciprocal + Newton); columns c > j of D take -m D(c, j), columns c <= j of V take
+    //     -m V(j, c).  Whether c > j is known per u at compile time except for one u per step.
+    if (w == 0) {
+      const int er = lane & 15, cg = lane >> 4;
+      T dv[4], vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        dv[u] = Dk[(cg + 4 * u) * LD + er];
+        vv[u] = cg + 4 * u == er ? T(1) : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const T d = Dk[j * LD + j], ar = Dk[j * LD + er];
+        T x[4], y[4];  // D(c, j) for c > j, V(j, c) for c <= j
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (4 * u + 3 > j) x[u] = Dk[j * LD + cg + 4 * u];
+          if (4 * u <= j) y[u] = Vk[j * LD + cg + 4 * u];
+        }
+        const T m = er > j ? ar * rcp_nr(d) : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (4 * u > j) {
+            dv[u] -= m * x[u];
+          } else if (4 * u + 3 <= j) {
+            vv[u] -= m * y[u];
+          } else {
+            const bool cd = cg + 4 * u > j;
+            dv[u] -= (cd ? m : T(0)) * x[u];
+            vv[u] -= (cd ? T(0) : m) * y[u];
+          }
+        }
+        if (j == 15) break;
+        wave_lds_sync();  // every lane's reads of step j precede the writes below
+        // column j+1 of D and row j+1 of V are final
+        const int uj = (j + 1) >> 2;
+        if (cg == ((j + 1) & 3) && er > j) Dk[(j + 1) * LD + er] = dv[uj];
+        if (er == j + 1)
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (4 * u <= j + 1 && cg + 4 * u <= j + 1) Vk[er * LD + cg + 4 * u] = vv[u];
+        wave_lds_sync();
+      }
+      wave_lds_sync();
+      // the pivots D'(c, c) are on the published diagonal; L = D' diag(r), Linv = diag(r) V
+      const T pr = Dk[er * LD + er];
+      if (cg == 0) {  // the first failing pivot of the block, in column order
+        const unsigned long long bad = __ballot(K0 + er < kb && !(pr > T(0)));
+        if (bad && lane == 0 && atomicCAS(err, 0, 1) == 0) {
           err[1] = fid;
-          err[2] = jb + K0 + j;
+          err[2] = jb + K0 + __builtin_ctzll(bad);
         }
       }
-    }
-    __syncthreads();
-    if (t < 16) rs[K0 + t] = rsqrt_nr(rs[K0 + t]);
-    __syncthreads();
-    if (er >= ec) {
-      Dk[ec * LD + er] *= rs[K0 + ec];
-      Vk[er * LD + ec] *= rs[K0 + er];
+      const T rr = rsqrt_nr(pr);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int c = cg + 4 * u;
+        const T rc = rsqrt_nr(Dk[c * LD + c]);
+        if (c <= er) {
+          Dk[c * LD + er] = dv[u] * rc;
+          Vk[er * LD + c] = vv[u] * rr;
+        }
+      }
     }
     __syncthreads();
     // (b) panel blocks below: L_ik = A_ik Linv_kk^T, wave w -> block i = k+1+w
@@ -609,7 +659,7 @@ template <class T>
 // left = 1 (left-looking levels): the block first takes the update by every finished column,
 // D = A(jb.., jb..) - L(jb.., 0:jb) L(jb.., 0:jb)^T, on MFMA (the panel steps leave the later
 // diagonal blocks untouched).
-__global__ __launch_bounds__(256) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+__global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    T* __restrict__ A, int jb, int left, int* __restrict__ err) {
   const int fid = ids[blockIdx.x];
   const FrontDev F = fr[fid];
@@ -819,7 +869,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
 // (tile_gemm_nt_bf) measured slower here, where the GEMMs are short and the 40 KB of split
 // stages halve the workgroups per CU.
 template <class T>
-__global__ __launch_bounds__(256) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+__global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                     T* __restrict__ A, int jb, int left, int xt, int nf, int fly,
                                                     const int* __restrict__ fly_tab,
                                                     const Triple* __restrict__ fly_tri,
