@@ -147,13 +147,52 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
 }
 
 // ----------------------------------------------------------------------------- MFMA tile GEMMs
-// Orders a wave's LDS accesses before its later ones, as seen by the other lanes of the same
-// wave (a wave's LDS operations complete in order; the fences keep the compiler from moving
-// accesses across this point).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Cross-lane moves of a float or double (as 32-bit halves): lane_read takes lane l's value (l
+// wave-uniform, readlane), lane_pull takes lane l's value for a per-lane l (ds_bpermute),
+// quad_bcast takes the value of lane q of the lane's own group of 4 (DPP quad permute).
+template <class T>
+__device__ __forceinline__ T lane_read(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
+}
+template <class T>
+__device__ __forceinline__ T lane_pull(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(4 * l, __builtin_bit_cast(int, v)));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_ds_bpermute(4 * l, (int)(unsigned)b);
+    const unsigned hi = __builtin_amdgcn_ds_bpermute(4 * l, (int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
+}
+template <int Q>
+__device__ __forceinline__ int quad_bcast_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, Q * 0x55, 0xF, 0xF, false);
+}
+template <class T>
+__device__ __forceinline__ T quad_bcast(T v, int q) {  // q: a constant after unrolling
+  auto one = [&](int x) {
+    switch (q) {
+      case 0: return quad_bcast_i<0>(x);
+      case 1: return quad_bcast_i<1>(x);
+      case 2: return quad_bcast_i<2>(x);
+      default: return quad_bcast_i<3>(x);
+    }
+  };
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, one(__builtin_bit_cast(int, v)));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = one((int)(unsigned)b), hi = one((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
 }
 
 // the wave index as a scalar, so that branches on it are uniform
@@ -488,12 +527,12 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 // Cholesky of the 64 x 64 diagonal block at (jb, jb) of each front with its inverse, one
 // workgroup per front, as 4 x 4 blocks of 16 x 16 in LDS:
 //   for k: one wave eliminates D_kk on [D_kk | I] without scaling the pivot rows (elements in
-//   registers, 4 per lane; step j reads column j of D and row j of the inverse part V from LDS,
-//   and the lanes holding column j+1 / row j+1 write them back for the next step, ordered within
-//   the wave, with no workgroup barrier per pivot), then scales: L_kk = D' diag(r), Linv_kk =
-//   diag(r) V with r_j = 1/sqrt(pivot j); panel L_ik = A_ik Linv_kk^T and trailing
-//   A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave; then the off-diagonal blocks of the
-//   inverse, Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance i - j.
+//   registers, 4 per lane; each step's column of D and row of the inverse part V move between
+//   lanes by readlane, DPP and ds_bpermute, with no LDS round trip or barrier per pivot), then
+//   scales: L_kk = D' diag(r), Linv_kk = diag(r) V with r_j = 1/sqrt(pivot j); panel
+//   L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave; then
+//   the off-diagonal blocks of the inverse, Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance
+//   i - j.
 // Ts[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
 // triangle: Linv(r, c) at U[r*LD + c], U = Ts + LD, for r >= c.
 #define LD (NB + 1)
@@ -520,12 +559,14 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
     T* Dk = Ts + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
     T* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
     __syncthreads();
-    // (a) wave 0: lane (er, cg) holds D(er, c) and V(er, c), c = cg + 4u (entries above the
-    //     diagonal ride along unused).  Step j: rows er > j take m = D(er, j) / D(j, j) (1/d by
-    //     reciprocal + Newton); columns c > j of D take -m D(c, j), columns c <= j of V take
-    //     -m V(j, c).  Whether c > j is known per u at compile time except for one u per step.
+    // (a) wave 0, all in registers: lane (er, cg) = (lane/4, lane%4) holds D(er, c) and V(er, c),
+    //     c = cg + 4u (entries above the diagonal ride along unused).  Step j: the pivot D(j, j)
+    //     comes by readlane, D(er, j) by a broadcast within the lane's quad, D(c, j) and V(j, c)
+    //     by ds_bpermute; rows er > j take m = D(er, j) / D(j, j) (1/d by reciprocal + Newton),
+    //     columns c > j of D take -m D(c, j), columns c <= j of V take -m V(j, c).  Whether
+    //     c > j is known per u at compile time except for one u per step.
     if (w == 0) {
-      const int er = lane & 15, cg = lane >> 4;
+      const int er = lane >> 2, cg = lane & 3;
       T dv[4], vv[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -534,12 +575,14 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
       }
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        const T d = Dk[j * LD + j], ar = Dk[j * LD + er];
+        const int uj = j >> 2, qj = j & 3;
+        const T d = lane_read(dv[uj], 4 * j + qj);
+        const T ar = quad_bcast(dv[uj], qj);
         T x[4], y[4];  // D(c, j) for c > j, V(j, c) for c <= j
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          if (4 * u + 3 > j) x[u] = Dk[j * LD + cg + 4 * u];
-          if (4 * u <= j) y[u] = Vk[j * LD + cg + 4 * u];
+          if (4 * u + 3 > j) x[u] = lane_pull(dv[uj], 4 * (cg + 4 * u) + qj);
+          if (4 * u <= j) y[u] = lane_pull(vv[u], 4 * j + cg);
         }
         const T m = er > j ? ar * rcp_nr(d) : T(0);
 #pragma unroll
@@ -554,32 +597,24 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
             vv[u] -= (cd ? T(0) : m) * y[u];
           }
         }
-        if (j == 15) break;
-        wave_lds_sync();  // every lane's reads of step j precede the writes below
-        // column j+1 of D and row j+1 of V are final
-        const int uj = (j + 1) >> 2;
-        if (cg == ((j + 1) & 3) && er > j) Dk[(j + 1) * LD + er] = dv[uj];
-        if (er == j + 1)
+      }
+      // pivots: D'(c, c) is dv[u] of lane (c, cg), D'(er, er) is dv[er/4] of lane (er, er%4)
+      T own = dv[0];
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            if (4 * u <= j + 1 && cg + 4 * u <= j + 1) Vk[er * LD + cg + 4 * u] = vv[u];
-        wave_lds_sync();
+      for (int u = 1; u < 4; u++)
+        if ((er >> 2) == u) own = dv[u];
+      const T pr = lane_pull(own, 4 * er + (er & 3));
+      const unsigned long long bad = __ballot(cg == 0 && K0 + er < kb && !(pr > T(0)));
+      if (bad && lane == 0 && atomicCAS(err, 0, 1) == 0) {  // the first failing pivot, in order
+        err[1] = fid;
+        err[2] = jb + K0 + (int)(__builtin_ctzll(bad) >> 2);
       }
-      wave_lds_sync();
-      // the pivots D'(c, c) are on the published diagonal; L = D' diag(r), Linv = diag(r) V
-      const T pr = Dk[er * LD + er];
-      if (cg == 0) {  // the first failing pivot of the block, in column order
-        const unsigned long long bad = __ballot(K0 + er < kb && !(pr > T(0)));
-        if (bad && lane == 0 && atomicCAS(err, 0, 1) == 0) {
-          err[1] = fid;
-          err[2] = jb + K0 + __builtin_ctzll(bad);
-        }
-      }
+      // L = D' diag(r), Linv = diag(r) V, r_j = 1/sqrt(pivot j)
       const T rr = rsqrt_nr(pr);
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int c = cg + 4 * u;
-        const T rc = rsqrt_nr(Dk[c * LD + c]);
+        const T rc = rsqrt_nr(lane_pull(dv[u], 4 * c + cg));
         if (c <= er) {
           Dk[c * LD + er] = dv[u] * rc;
           Vk[er * LD + c] = vv[u] * rr;
