@@ -147,6 +147,15 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
 }
 
 // ----------------------------------------------------------------------------- MFMA tile GEMMs
+// Orders a wave's LDS accesses before its later ones, as seen by the other lanes of the same
+// wave (a wave's LDS operations complete in order; the fences keep the compiler from moving
+// accesses across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Cross-lane moves of a float or double (as 32-bit halves): lane_read takes lane l's value (l
 // wave-uniform, readlane), lane_pull takes lane l's value for a per-lane l (ds_bpermute),
 // quad_bcast takes the value of lane q of the lane's own group of 4 (DPP quad permute).
@@ -553,12 +562,10 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (t < NB) Ts[NB * LD + t] = T(0);
   __syncthreads();
-  if (t < NB) U[t * LD + t] = 1.0;
   for (int k = 0; k < 4; k++) {
     const int K0 = 16 * k;
     T* Dk = Ts + K0 * LD + K0;  // D(r, c) at Dk[c*LD + r]
     T* Vk = U + K0 * LD + K0;  // V(r, c) at Vk[r*LD + c]
-    __syncthreads();
     // (a) wave 0, all in registers: lane (er, cg) = (lane/4, lane%4) holds D(er, c) and V(er, c),
     //     c = cg + 4u (entries above the diagonal ride along unused).  Step j: the pivot D(j, j)
     //     comes by readlane, D(er, j) by a broadcast within the lane's quad, D(c, j) and V(j, c)
@@ -622,33 +629,23 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
       }
     }
     __syncthreads();
-    // (b) panel blocks below: L_ik = A_ik Linv_kk^T, wave w -> block i = k+1+w
+    // (b) panel blocks below: L_ik = A_ik Linv_kk^T, wave w -> block i = k+1+w (each wave reads
+    //     and rewrites its own block)
     {
       const int i = k + 1 + w;
-      V4<T> acc = vzero<T>();
-      if (i < 4) mma16<0, 1>(acc, Ts + K0 * LD + 16 * i, 1, LD, Vk, LD, 1);
-      __syncthreads();
-      if (i < 4)
+      if (i < 4) {
+        V4<T> acc = vzero<T>();
+        mma16<0, 1>(acc, Ts + K0 * LD + 16 * i, 1, LD, Vk, LD, 1);
 #pragma unroll
         for (int q = 0; q < 4; q++) Ts[(K0 + mrow<T>(lane >> 4, q)) * LD + 16 * i + (lane & 15)] = acc[q];
+      }
     }
     __syncthreads();
-    // columns K0..K0+15 of L are final: store them now, overlapping the remaining work
-    if (store)
-#pragma unroll
-      for (int e = t; e < 16 * NB; e += 256) {
-        const int i = e & 63, j = K0 + (e >> 6);
-        if (i < kb && j < kb && i >= j) blk[(i64)j * ld + i] = Ts[j * LD + i];
-      }
-    // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i
-    for (int p = w; p < 6; p += 4) {
-      int cnt = 0, bi = -1, bj = -1;
-      for (int i2 = k + 1; i2 < 4; i2++)
-        for (int j2 = k + 1; j2 <= i2; j2++) {
-          if (cnt == p) { bi = i2; bj = j2; }
-          cnt++;
-        }
-      if (bi < 0) continue;
+    // (c) trailing blocks A_ij -= L_ik L_jk^T for k < j <= i.  Wave 0 takes the next diagonal
+    //     block (k+1, k+1), the only one the next elimination reads, and goes on to that
+    //     elimination without a barrier; waves 1-3 take the other blocks and store columns
+    //     K0..K0+15 of L meanwhile.
+    auto trail = [&](int bi, int bj) {
       V4<T> acc = vzero<T>();
       mma16(acc, Ts + K0 * LD + 16 * bi, 1, LD, Ts + K0 * LD + 16 * bj, 1, LD);
       // a diagonal block keeps its strict upper triangle, which holds the inverse
@@ -656,6 +653,30 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
       for (int q = 0; q < 4; q++)
         if (bi != bj || (lane & 15) >= mrow<T>(lane >> 4, q))
           Ts[(16 * bj + mrow<T>(lane >> 4, q)) * LD + 16 * bi + (lane & 15)] -= acc[q];
+    };
+    if (w == 0) {
+      if (k < 3) trail(k + 1, k + 1);
+      wave_lds_sync();
+    } else {
+      if (store)
+        for (int e = t - 64; e < 16 * NB; e += 192) {
+          const int i = e & 63, j = K0 + (e >> 6);
+          if (i < kb && j < kb && i >= j) blk[(i64)j * ld + i] = Ts[j * LD + i];
+        }
+      for (int p = w - 1;; p += 3) {
+        int cnt = 0, bi = -1, bj = -1;
+        for (int i2 = k + 1; i2 < 4; i2++)
+          for (int j2 = k + 1; j2 <= i2; j2++) {
+            if (i2 == k + 1 && j2 == k + 1) continue;
+            if (cnt == p) {
+              bi = i2;
+              bj = j2;
+            }
+            cnt++;
+          }
+        if (bi < 0) break;
+        trail(bi, bj);
+      }
     }
   }
   __syncthreads();
