@@ -363,7 +363,8 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = w >> 1, wc = w & 1;
   const int li = lane, kg = w;  // staging: row li (+64u), k = 8 kg .. 8 kg + 7 of the chunk
-  float pa[NL][8], qb[NL][8];
+  // two register sets of fetched operands, so a chunk's loads have two chunks of work to land
+  float pa[2][NL][8], qb[2][NL][8];
   int po[NL][8], qo[NL][8];
 #pragma unroll
   for (int u = 0; u < NL; u++)
@@ -373,7 +374,8 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
       po[u][j] = ((8 * kg + j) * ldp + min(row, pn - 1)) * 4;
       qo[u][j] = ((8 * kg + j) * ldq + min(row, qn - 1)) * 4;
     }
-  auto fetch = [&](int kc) {
+  auto fetch = [&](int kc, auto set) {
+    constexpr int S = decltype(set)::value;
     const int nk = max(min(32, K - kc), 0);
     const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * 4);
     const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * 4);
@@ -381,16 +383,17 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
     for (int j = 0; j < 8; j++)
 #pragma unroll
       for (int u = 0; u < NL; u++) {
-        pa[u][j] = buf_load<float>(rp, po[u][j]);
-        qb[u][j] = buf_load<float>(rq, qo[u][j]);
+        pa[S][u][j] = buf_load<float>(rp, po[u][j]);
+        qb[S][u][j] = buf_load<float>(rq, qo[u][j]);
       }
   };
-  auto stage = [&](int b) {
+  auto stage = [&](int b, auto set) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int u = 0; u < NL; u++) {
       bf16x8 pp[NPC], qp[NPC];
-      split_bf(pa[u], pp);
-      split_bf(qb[u], qp);
+      split_bf(pa[S][u], pp);
+      split_bf(qb[S][u], qp);
 #pragma unroll
       for (int p = 0; p < NPC; p++) {
         sm.P[b][p][li + 64 * u][kg] = pp[p];
@@ -399,14 +402,18 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
     }
   };
   if (K <= 0) return;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
   __syncthreads();  // the stages may still be read by an earlier use of the shared memory
-  fetch(0);
-  stage(0);
-  fetch(32);
+  fetch(0, S0{});
+  fetch(32, S1{});
+  stage(0, S0{});
+  fetch(64, S0{});
   __syncthreads();
   const int q = lane >> 4, c16 = lane & 15;
-  int b = 0;
-  for (int kc = 0; kc < K; kc += 32, b ^= 1) {
+  // chunk kc (LDS stage b): its products, then chunk kc+32 (register set `next`) is staged and
+  // chunk kc+96 is fetched into that set
+  auto step = [&](int kc, int b, auto next) {
     bf16x8 av[NPC][NBK], bv[NPC][NBK];
 #pragma unroll
     for (int p = 0; p < NPC; p++)
@@ -423,8 +430,8 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
         for (int bi = 0; bi < NBK; bi++)
 #pragma unroll
           for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma_bf(av[i][bj], bv[s - i][bi], acc[bi][bj]);
-    stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
-    fetch(kc + 64);
+    stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 96, next);
     constexpr int NM = NBK * NBK * 3;  // MFMAs per chunk
 #pragma unroll
     for (int g = 0; g < 8; g++) {
@@ -434,6 +441,10 @@ __device__ __forceinline__ void tile_gemm_nt_bf(V4<float> (&acc)[TS / 32][TS / 3
       __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);       // buffer load
     }
     __syncthreads();
+  };
+  for (int kc = 0; kc < K; kc += 64) {
+    step(kc, 0, S1{});  // chunk kc+32 sits in set 1
+    if (kc + 32 < K) step(kc + 32, 1, S0{});
   }
 }
 
