@@ -250,16 +250,17 @@ struct TileSmemT {
 
 // LOWER (P = Q, only the lower triangle of the product wanted): the wave above the diagonal
 // (wr < wc) only stages operands.
-template <class T, int TS, bool LOWER = false>
+template <class T, int TS, bool LOWER = false, int DEPTH = 1>
 __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], const T* __restrict__ P, int ldp,
                                                int pn, const T* __restrict__ Q, int ldq, int qn, int K,
                                                TileSmemT<T, TS>& sm) {
+  static_assert(DEPTH == 1 || DEPTH == 2, "one or two chunks fetched ahead");
   constexpr int NL = TS / 64, NBK = TS / 32;
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = w >> 1, wc = w & 1;
   const int li = lane, lk = w;
-  T pa[4][NL], qb[4][NL];
+  T pa[DEPTH][4][NL], qb[DEPTH][4][NL];  // DEPTH register sets of fetched operands
   int po[4][NL], qo[4][NL];  // byte offsets of this lane's elements within a chunk
 #pragma unroll
   for (int q = 0; q < 4; q++)
@@ -269,7 +270,8 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
       po[q][u] = ((lk + 4 * q) * ldp + min(row, pn - 1)) * (int)sizeof(T);
       qo[q][u] = ((lk + 4 * q) * ldq + min(row, qn - 1)) * (int)sizeof(T);
     }
-  auto fetch = [&](int kc) {
+  auto fetch = [&](int kc, auto set) {
+    constexpr int S = decltype(set)::value;
     const int nk = max(min(16, K - kc), 0);
     const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * (int)sizeof(T));
     const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * (int)sizeof(T));
@@ -277,28 +279,27 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int u = 0; u < NL; u++) {
-        pa[q][u] = buf_load<T>(rp, po[q][u]);
-        qb[q][u] = buf_load<T>(rq, qo[q][u]);
+        pa[S][q][u] = buf_load<T>(rp, po[q][u]);
+        qb[S][q][u] = buf_load<T>(rq, qo[q][u]);
       }
   };
-  auto stage = [&](int b) {
+  auto stage = [&](int b, auto set) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int u = 0; u < NL; u++) {
-        sm.As[b][lk + 4 * q][li + 64 * u] = pa[q][u];
-        sm.Bs[b][lk + 4 * q][li + 64 * u] = qb[q][u];
+        sm.As[b][lk + 4 * q][li + 64 * u] = pa[S][q][u];
+        sm.Bs[b][lk + 4 * q][li + 64 * u] = qb[S][q][u];
       }
   };
   if (K <= 0) return;
-  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
-  fetch(0);
-  stage(0);
-  fetch(16);
-  __syncthreads();
-  int b = 0;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, DEPTH - 1>;
   const bool mma = !LOWER || wr >= wc;
-  for (int kc = 0; kc < K; kc += 16, b ^= 1) {
+  // chunk kc from LDS stage b; then the chunk after it (register set `next`) is staged and the
+  // chunk DEPTH + 1 ahead is fetched into that set
+  auto step = [&](int kc, int b, auto next) {
 #pragma unroll
     for (int k4 = 0; k4 < 4 && mma; k4++) {
       const int kr = k4 * 4 + (lane >> 4);
@@ -313,9 +314,24 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
 #pragma unroll
         for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
     }
-    stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
-    fetch(kc + 32);
+    stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 16 * (DEPTH + 1), next);
     __syncthreads();
+  };
+  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
+  fetch(0, S0{});
+  if constexpr (DEPTH == 2) fetch(16, S1{});
+  stage(0, S0{});
+  fetch(16 * DEPTH, S0{});
+  __syncthreads();
+  if constexpr (DEPTH == 1) {
+    int b = 0;
+    for (int kc = 0; kc < K; kc += 16, b ^= 1) step(kc, b, S0{});
+  } else {
+    for (int kc = 0; kc < K; kc += 32) {
+      step(kc, 0, S1{});  // the next chunk sits in set 1
+      if (kc + 16 < K) step(kc + 16, 1, S0{});
+    }
   }
 }
 
@@ -1122,8 +1138,10 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
   if constexpr (TS == 64) read_c(0);
   const T* base = A + F.off + (i64)kbase * F.N;
   if constexpr (!SPLIT) {
+    // FP32 fronts fetch two chunks ahead (FP64 tiles would lose occupancy to the registers)
+    constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
     __shared__ TileSmemT<T, TS> sm;
-    tile_gemm_nt_t<T, TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+    tile_gemm_nt_t<T, TS, false, DEPTH>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   } else {
     __shared__ TileSmemBF<TS> sm;
     tile_gemm_nt_bf<TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
