@@ -566,9 +566,10 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 //   registers, 4 per lane; each step's column of D and row of the inverse part V move between
 //   lanes by readlane, DPP and ds_bpermute, with no LDS round trip or barrier per pivot), then
 //   scales: L_kk = D' diag(r), Linv_kk = diag(r) V with r_j = 1/sqrt(pivot j); panel
-//   L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave; then
-//   the off-diagonal blocks of the inverse, Linv_ij = -Linv_ii sum_k L_ik Linv_kj, by distance
-//   i - j.
+//   L_ik = A_ik Linv_kk^T and trailing A_ij -= L_ik L_jk^T on MFMA, one 16-block per wave, and
+//   row k of the inverse's off-diagonal blocks, Linv_kj = -Linv_kk sum_i L_ki Linv_ij.  Wave 0
+//   updates only the next diagonal block and starts its elimination at once; the other waves do
+//   the rest of step k (other trailing blocks, L stores, the inverse row) meanwhile.
 // Ts[c*LD + r] holds L(r, c) for r >= c; the inverse is stored row-major in the strict upper
 // triangle: Linv(r, c) at U[r*LD + c], U = Ts + LD, for r >= c.
 #define LD (NB + 1)
@@ -704,32 +705,28 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
         if (bi < 0) break;
         trail(bi, bj);
       }
+      // row k of the inverse's off-diagonal blocks, one per wave: Linv_kj = -Linv_kk S with
+      // S = sum_{kk=j}^{k-1} L_k,kk Linv_kk,j (earlier rows are complete, Linv_kk came from (a));
+      // S is staged in the destination block
+      const int j = w - 1;
+      if (j < k) {
+        T* Skj = U + 16 * k * LD + 16 * j;  // S(i', j') at Skj[i'*LD + j']
+        V4<T> acc = vzero<T>();  // P(i', c) = L(16k+i', 16kk+c), Q(j', c) = Linv(16kk+c, 16j+j')
+        mma16<0, 2>(acc, Ts + 16 * j * LD + 16 * k, 1, LD, U + 16 * j * LD + 16 * j, 1, LD);
+        for (int kk = j + 1; kk < k; kk++)
+          mma16(acc, Ts + 16 * kk * LD + 16 * k, 1, LD, U + 16 * kk * LD + 16 * j, 1, LD);
+#pragma unroll
+        for (int q = 0; q < 4; q++) Skj[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = acc[q];
+        wave_lds_sync();
+        V4<T> a2 = vzero<T>();  // P(i', c) = Linv_kk(i', c), Q(j', c) = S(c, j')
+        mma16<1, 0>(a2, U + 16 * k * LD + 16 * k, LD, 1, Skj, 1, LD);
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < 4; q++) Skj[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
+      }
     }
   }
   __syncthreads();
-  // inverse off-diagonal blocks, by distance; S = sum_kk L_i,kk Linv_kk,j is staged in the
-  // (not yet computed) destination block Linv_ij.
-  for (int dist = 1; dist < 4; dist++) {
-    const int i = w + dist, j = w;
-    T* Sij = U + 16 * i * LD + 16 * j;  // S(i', j') at Sij[i'*LD + j']
-    V4<T> acc = vzero<T>();
-    if (i < 4) {  // P(i', k) = L(16i+i', 16kk+k), Q(j', k) = Linv(16kk+k, 16j+j'), lower for kk = j
-      mma16<0, 2>(acc, Ts + 16 * j * LD + 16 * i, 1, LD, U + 16 * j * LD + 16 * j, 1, LD);
-      for (int kk = j + 1; kk < i; kk++) mma16(acc, Ts + 16 * kk * LD + 16 * i, 1, LD, U + 16 * kk * LD + 16 * j, 1, LD);
-    }
-    if (i < 4)
-#pragma unroll
-      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = acc[q];
-    __syncthreads();
-    V4<T> a2 = vzero<T>();
-    if (i < 4)  // P(i', k) = Linv_ii(i', k), Q(j', k) = S(k, j')
-      mma16<1, 0>(a2, U + 16 * i * LD + 16 * i, LD, 1, Sij, 1, LD);
-    __syncthreads();
-    if (i < 4)
-#pragma unroll
-      for (int q = 0; q < 4; q++) Sij[(lane & 15) * LD + mrow<T>(lane >> 4, q)] = -a2[q];
-    __syncthreads();
-  }
   if (store)
 #pragma unroll
     for (int e = t; e < NB * NB; e += 256) {
