@@ -218,3 +218,30 @@ def test_large_fronts_match_oracle():
     X = G.Solve(B)
     for j in range(3):
         assert relerr(X[:, j], O.solve(B[:, j])) <= SOLVE_TOL
+
+
+@pytest.mark.parametrize("dims,maxs", [((127, 127), 8), ((23, 23, 23), 5)])
+def test_non_spd_reports_first_failing_column(dims, maxs):
+    """One interior diagonal entry made strongly negative: the leading principal minors before
+    its pivot position p are untouched, so the first non-positive pivot of that front is exactly
+    column p, and no other front pivots on that DOF.  Positions inside later 16-column
+    sub-blocks and 64-column blocks pin the in-block order of the report, on the leaf level
+    (left-looking steps) and on the root (fused right-looking steps)."""
+    import umf
+    co = coeff_field(dims, "lognormal", seed=5)
+    base = laplacian(dims, co)
+    G0 = (umf.MF2D if len(dims) == 2 else umf.MF3D)(base, *dims, width=1, maxSideLength=maxs,
+                                                   analyze_only=True)
+    rows = np.repeat(np.arange(base.n), np.diff(base.rp))
+    for ell in (G0.levels - 1, 0):
+        dofs = G0.box_dofs(ell, 0, 0)
+        n = len(dofs)
+        for p in sorted({min(n - 1, 37), min(n - 1, 100)}):
+            A = laplacian(dims, co)
+            i = int(dofs[p])
+            A.val = A.val.copy()
+            A.val[np.flatnonzero((rows == i) & (A.ci == i))] = -1e6
+            G = (umf.MF2D if len(dims) == 2 else umf.MF3D)(A, *dims, width=1, maxSideLength=maxs)
+            print(dims, "level", ell, "n", n, "column", p)
+            with pytest.raises(ArithmeticError, match=f"level {ell} box 0 pivot column {p} "):
+                G.Factor()
