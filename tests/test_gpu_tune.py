@@ -62,7 +62,7 @@ def test_autotune_picks_a_candidate_and_solves():
     O.factor()
     res = G.autotune(reps=1)
     print("autotune", res)
-    assert res["gw2"] in (1024, 2048) and res["tile"] in (64, 128)  # FP64: the size rule is 64
+    assert res["gw2"] in (1024, 2048) and res["tile"] in (0, 64, 128)
     assert res["factor_ms"] > 0
     b = np.random.default_rng(3).standard_normal(A.n)
     assert relerr(G.Solve(b), O.solve(b)) <= 1e-10  # the plan holds the chosen settings' factors

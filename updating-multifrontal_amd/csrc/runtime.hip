@@ -381,9 +381,9 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
       }
     }
     const int GW = 256;
-    // FP32: 128 x 128 tiles for a wide update (half the operand traffic) when it still yields
-    // 2048+ tiles; in FP64 the 128-tile kernel holds 2 waves per SIMD and measured slower
-    const bool f32 = sizeof(T) == 4;
+    // 128 x 128 tiles for a wide update (half the operand traffic) when it still yields 2048+
+    // tiles (3D 95^3 FP64 trailing update 319 -> 302 ms; the 128-tile FP64 kernel holds two waves
+    // per SIMD, and smaller stages stay on 64 x 64)
     // The trailing update is blocked on two levels: a finished group of GW columns updates only
     // the rest of its outer group of GW2 columns (K = GW), and the finished outer group updates
     // the trailing matrix once with K = GW2.  At K = GW the tile GEMMs are bound by operand and
@@ -392,7 +392,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     // GW2 = 2048 measured the same as 1024.
     const int GW2 = p->tune_gw2;
     // 128 x 128 GEMM tiles for a stage of `tiles128` such tiles in all
-    auto wide = [&](i64 tiles128) { return p->tune_tile == 128 || (p->tune_tile == 0 && f32 && tiles128 >= 2048); };
+    auto wide = [&](i64 tiles128) { return p->tune_tile == 128 || (p->tune_tile == 0 && tiles128 >= 2048); };
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     bool xcd = L.f_cnt >= 64;
@@ -1808,7 +1808,6 @@ int umf_autotune(umf_plan* p, int reps, double* best_ms) {
   double best = 1e300;
   for (int g : {1024, 2048})
     for (int t : {0, 64, 128}) {
-      if (t == 0 && p->prec == UMF_PREC_FP64) continue;  // the rule picks 64 for FP64 fronts
       p->tune_gw2 = g;
       p->tune_tile = t;
       double ms = 1e300;
