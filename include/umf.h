@@ -223,7 +223,8 @@ int umf_set_graph(umf_plan* plan, int on);
 /* Tuning of the factorization's GEMM stages (no reference counterpart; the reference's BLAS
  * picks its own blocking): gw2 = width of the outer column groups of the two-level trailing
  * update (256..8192, a multiple of 256; default 1024), tile = 64 or 128 for the trailing and
- * Schur GEMM tiles, or 0 (default) for 128 when a stage still has 2048 such tiles, else 64.  The tile size does not change results (each entry accumulates the same
+ * Schur GEMM tiles, or 0 (default) for 128 when a stage's largest front has 16 such tiles and
+ * its fronts 2048 in all, else 64.  The tile size does not change results (each entry accumulates the same
  * products in the same order); gw2 changes the order of the trailing updates. */
 int umf_set_tune(umf_plan* plan, int gw2, int tile);
 int umf_get_tune(const umf_plan* plan, int* gw2, int* tile);

@@ -381,9 +381,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
       }
     }
     const int GW = 256;
-    // 128 x 128 tiles for a wide update (half the operand traffic) when it still yields 2048+
-    // tiles (3D 95^3 FP64 trailing update 319 -> 302 ms; the 128-tile FP64 kernel holds two waves
-    // per SIMD, and smaller stages stay on 64 x 64)
+    // 128 x 128 tiles (half the operand traffic) for a stage whose largest front has 16+ of them
+    // and whose fronts have 2048+ in all: 3D 95^3 FP64 trailing update 319 -> 302 ms; fronts of a
+    // few 128-tiles waste the partial tiles (256-problem batch FP64 Schur 13.5 -> 27.3 ms), and
+    // the 2D stages stay below 2048 tiles
     // The trailing update is blocked on two levels: a finished group of GW columns updates only
     // the rest of its outer group of GW2 columns (K = GW), and the finished outer group updates
     // the trailing matrix once with K = GW2.  At K = GW the tile GEMMs are bound by operand and
@@ -391,8 +392,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     // the FP32 / BF16-split tiles of a 16k-front microbenchmark from 76 / 103 to 126 / 247;
     // GW2 = 2048 measured the same as 1024.
     const int GW2 = p->tune_gw2;
-    // 128 x 128 GEMM tiles for a stage of `tiles128` such tiles in all
-    auto wide = [&](i64 tiles128) { return p->tune_tile == 128 || (p->tune_tile == 0 && tiles128 >= 2048); };
+    // 128 x 128 GEMM tiles for a stage over `fronts` fronts of at most `per_front` such tiles
+    auto wide = [&](i64 per_front, i64 fronts) {
+      return p->tune_tile == 128 || (p->tune_tile == 0 && per_front >= 16 && per_front * fronts >= 2048);
+    };
     // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
     // diagonal-block columns); levels of few fronts spread each front over the whole chip
     bool xcd = L.f_cnt >= 64;
@@ -523,7 +526,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
       auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
-        if (wide((i64)t128 * act))
+        if (wide(t128, act))
           launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, act), ids + L.f_off,
                  (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
         else
@@ -563,7 +566,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         const bool xc = c >= 64;
         auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
         if (tiles > 0) {
-          if (wide((i64)tiles128 * c))
+          if (wide(tiles128, c))
             launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split), grid(tiles128), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
           else
