@@ -1445,9 +1445,13 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
 // A workgroup only waits for blocks owned by workgroups with smaller ids, which the hardware
 // dispatches first, and every wait is bounded: past the limit err[3] is set and the
 // workgroup goes on, so the launch always drains.
-// The flag is polled with relaxed agent-scope loads (no cache maintenance per poll); one
-// acquire fence follows the successful poll.  Publication: every wave's stores are released at
-// agent scope before the barrier, then one relaxed store sets the flag.
+// The rows a block publishes to the other blocks (X) are written and read with relaxed
+// agent-scope atomics, which go through to memory instead of a CU's L1 or an XCD's L2, so no
+// cache maintenance is needed: the flag is polled with relaxed agent-scope loads, and the
+// publisher waits for its stores to complete (s_waitcnt) before the barrier, after which one
+// relaxed store sets the flag.  (An agent-scope release / acquire fence instead writes back and
+// invalidates the XCD's L2 at every block, which also evicts the factor rows the other
+// workgroups are streaming.)
 __device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err) {
   if (threadIdx.x == 0) {
     int spins = 0;
@@ -1458,14 +1462,22 @@ __device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err)
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
 __device__ __forceinline__ void post_block(int* flag, int epoch) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's published stores have completed
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// published rows: relaxed agent-scope atomic accesses
+template <class T>
+__device__ __forceinline__ T ld_pub(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_pub(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Forward sweep of one front by blocks (blockIdx.x = block, blockIdx.y = front): interior block
@@ -1514,7 +1526,8 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      bv[k4] = ld_or0(X + (i64)(jb + min(k, kb - 1)) * RB + c16, k < kb);
+      const T v = ld_pub(X + (i64)(jb + min(k, kb - 1)) * RB + c16);
+      bv[k4] = k < kb ? v : T(0);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1541,7 +1554,7 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
   for (int r = 0; r < 4; r++) {
     const int i = r0 + 16 * w + mrow<T>(q, r);
     if (i < rend) {
-      X[(i64)i * RB + c16] = a2[r];
+      st_pub(X + (i64)i * RB + c16, a2[r]);
       W[(i64)di[i] * RB + c16] = a2[r];
     }
   }
@@ -1608,7 +1621,8 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
-      bv[k4] = ld_or0(X + (i64)(k0 + min(k, kn - 1)) * RB + c16, k < kn);
+      const T v = ld_pub(X + (i64)(k0 + min(k, kn - 1)) * RB + c16);
+      bv[k4] = k < kn ? v : T(0);
     }
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
@@ -1626,7 +1640,7 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
   for (int r = 0; r < 4; r++) {
     const int i = 16 * w + mrow<T>(q, r);
     if (i < kb) {
-      X[(i64)(jb + i) * RB + c16] = a2[r];
+      st_pub(X + (i64)(jb + i) * RB + c16, a2[r]);
       W[(i64)di[jb + i] * RB + c16] = a2[r];
     }
   }
