@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: factor + local update + 16-RHS solve of the 2D variable-coefficient operator.
+"""Benchmark: factor + local update + 16-RHS solve of a variable-coefficient elliptic operator,
+in the FP32-MFMA + FP64-refinement form against the plain FP64 form.
 
-Workload (the 2D 1023^2 configuration): 5-point -div(a grad u) on the unit square, 1023 x 1023
-interior grid (n = 1,046,529), a = lognormal(0,1) iid per padded node (numpy PCG64, seed 2),
-h = 1/1024, nested dissection with maxSideLength 16 (31 x 31 leaf boxes, 6 levels), FP64
-plain form.  One step = Factor() + Update() (a multiplied by 100 on a random 32 x 32 block of
+Headline workload (the default): 3D 7-point -div(a grad u) on the unit cube, 127^3 interior grid
+(n = 2,048,383), a = lognormal(0,1) iid per padded node (numpy PCG64, seed 3), h = 1/128,
+nested dissection with maxSideLength 8 (15^3 leaf boxes, 4 levels, root front of 48,007).
+One step = Factor() + Update() (a multiplied by one lognormal(0,2) draw on a random 8^3 box of
 nodes, modified DOFs by the reference's modDOF rule) + Solve() of 16 N(0,1) right-hand sides.
 All inputs (CSR values of both operators, right-hand sides) are resident in HBM before timing;
-the update's modified-DOF list is host data, as in the reference API.
+the update's modified-DOF list is host data, as in the reference API.  The two operators
+alternate, so every step really refactors.
 
-value = algorithmic TFLOP/s of the whole step: (F_factor + F_update + F_solve) / time, with
-F = sum over fronts of n^3/3 + n^2 m + n m^2 (factor, dirty fronts for update) and
-nrhs * sum (2 n^2 + 4 n m) (solve), n / m = pivot / boundary size of each front.
+value = the FP32 form's algorithmic TFLOP/s over the timed steps: fronts stored and factored in
+FP32 on FP32 MFMA, the solve refined in FP64 (preconditioned CG on the FP64 operator) until every
+column's infinity-norm backward error ||b - Ax|| / (||A|| ||x|| + ||b||) is <= 1e-10.  The plain
+FP64 form and the BF16-split form run the same steps beside it (objects "fp64" and "bf16x3_ir");
+every form reports its backward error and its forward error against the FP64 form's solution.
+TFLOP/s numerator (the same for every form and for the CPU baseline): F = sum over fronts of
+n^3/3 + n^2 m + n m^2 (factor; dirty fronts for the update) + nrhs * sum (2 n^2 + 4 n m) (solve),
+n / m = pivot / boundary size of each front.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -26,54 +34,109 @@ PKG = os.path.join(ROOT, "updating-multifrontal_amd")
 sys.path.insert(0, PKG)
 
 METRIC = "factor+update+solve s & TFLOP/s; FP32-MFMA+IR vs FP64 at 1e-10 backward err"
-PEAK_FP64_TFLOPS = 78.6
+# published MI355X dense peaks: FP64 / FP32 MFMA, BF16 MFMA; HBM3E
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PEAK_HBM_GBS = 8000.0
+FORM_DTYPE = {
+    "fp64": "f64",
+    "fp32": "f32 fronts (FP32 MFMA) + f64 residual and refinement",
+    "bf16x3": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 2 bf16 pieces (3 products, f32 accumulate) "
+              "+ f64 residual and refinement",
+}
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("kernels.hip", "runtime.hip", "plan.cpp")]
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
-    summary of this workload (profiles/<kernel>_traffic_<workload>.md, calibrated on a plain
-    copy), or None when there is none: counters cannot be read inside the timed run."""
+def source_hash():
+    """First 16 hex digits of the SHA-256 of the library sources that decide what a kernel
+    launch reads and writes."""
+    h = hashlib.sha256()
+    for f in SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(kernel, workload, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    summary of this workload and form (profiles/<kernel>_traffic_<workload>_<form>.md, calibrated
+    on a plain copy), used only when the summary was measured on these exact sources (its
+    'source hash' line); otherwise None.  Counters cannot be read inside the timed run."""
     import re
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}_{prec}.md")
     try:
         with open(path) as f:
-            m = re.search(r"^bytes per launch:\s*([0-9.eE+-]+)\s*$", f.read(), re.M)
-        return float(m.group(1)) if m else None
+            txt = f.read()
     except OSError:
         return None
+    m = re.search(r"^bytes per launch:\s*([0-9.eE+-]+)\s*$", txt, re.M)
+    hs = re.search(r"^source hash:\s*([0-9a-f]+)\s*$", txt, re.M)
+    if not m or not hs or hs.group(1) != source_hash():
+        return None
+    return float(m.group(1))
 
 
-# workloads: name -> (dims, maxSideLength, update box side, update scale or None for one LN(0,2)
-# factor, seed); the first is the headline, the 3D ones are scale runs of the same step
+# workloads: dims, maxSideLength, the update (a box of `box` nodes per axis at `lo` (random when
+# None), multiplied by `scale` (one LN(0,2) draw when None) or set to `setval`), coefficient
+# field, right-hand sides, seed, the form whose rate is `value`, and the CPU baseline's sample
 WORKLOADS = {
-    "2d1023": ((1023, 1023), 16, 32, 100.0, 2),
-    "3d95": ((95, 95, 95), 8, 8, None, 3),
-    "3d127": ((127, 127, 127), 8, 8, None, 3),
+    "3d127": dict(dims=(127, 127, 127), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d47"),
+    "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d47"),
+    "2d1023": dict(dims=(1023, 1023), max_side=16, box=32, scale=100.0, seed=2, value_form="fp32", cpu="2d511"),
+    "2d127": dict(dims=(127, 127), max_side=8, box=8, lo=(60, 60), setval=10.0, coeff="uniform", rhs="uniform",
+                  nrhs=1, seed=1, value_form="fp64", cpu="2d127"),
     "batch256": None,  # make_batch
     "serve95": None,   # run_serve
 }
+# bounded samples of the same generators for the reference's CPU run (about 10-30 s of CPU work)
+CPU_SAMPLES = {
+    "3d47": dict(dims=(47, 47, 47), max_side=8, box=8, scale=None, seed=3),
+    "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3),
+    "2d511": dict(dims=(511, 511), max_side=16, box=32, scale=100.0, seed=2),
+    "2d127": WORKLOADS["2d127"],
+}
 
 
-def make_problem(N, seed, box=32, scale=100.0, nrhs=16, dims=None):
-    """Lognormal(0,1) nodal coefficients on the padded grid, the update multiplies a random
-    box of nodes by `scale` (or by one LN(0,2) draw when scale is None)."""
+def make_problem(dims, seed, box=32, scale=100.0, nrhs=16, coeff="lognormal", lo=None, setval=None, rhs="normal",
+                 **_):
+    """Nodal coefficients on the padded grid (lognormal(0,1) iid, or 1), and the updated field:
+    a box of `box` nodes per axis at `lo` (random) multiplied by `scale` (one LN(0,2) draw when
+    None) or set to `setval`.  Returns both operators' CSR values, the modified DOFs (the
+    reference's modDOF rule, test_src/DE/testMF2D.cpp:143-150), right-hand sides (nrhs x n: row r
+    = right-hand side r, i.e. column-major n x nrhs) and the box corner."""
     import umf
-    dims = dims or (N, N)
     d = len(dims)
     rng = np.random.default_rng(seed)
-    co = np.exp(rng.standard_normal(tuple(n + 2 for n in dims)))
+    shape = tuple(n + 2 for n in dims)
+    co = np.exp(rng.standard_normal(shape)) if coeff == "lognormal" else np.ones(shape)
     co2 = co.copy()
-    lo = [int(v) for v in rng.integers(0, [n + 2 - box for n in dims])]
-    if scale is None:
-        scale = float(np.exp(2.0 * rng.standard_normal()))
-    co2[tuple(slice(l, l + box) for l in lo)] *= scale
+    if lo is None:
+        lo = [int(v) for v in rng.integers(0, [n + 2 - box for n in dims])]
+    sl = tuple(slice(l, l + box) for l in lo)
+    if setval is not None:
+        co2[sl] = setval
+    else:
+        if scale is None:
+            scale = float(np.exp(2.0 * rng.standard_normal()))
+        co2[sl] *= scale
     h = 1.0 / (dims[0] + 1)
     rp, ci, v1 = umf.build_laplacian(dims, co, 1, h)
     _, _, v2 = umf.build_laplacian(dims, co2, 1, h)
-    mod = np.unique(ci[v1 != v2]).astype(np.int32)  # modDOF rule (testMF2D.cpp:143-150)
-    B = rng.standard_normal((nrhs, int(np.prod(dims))))  # row r = right-hand side r: column-major n x nrhs
+    mod = np.unique(ci[v1 != v2]).astype(np.int32)
+    n = int(np.prod(dims))
+    B = rng.standard_normal((nrhs, n)) if rhs == "normal" else rng.uniform(-1.0, 1.0, (nrhs, n))
     return rp, ci, v1, v2, mod, B, tuple(lo)
+
+
+def describe(spec, nrhs, levels=None, leaf=None):
+    dims = spec["dims"]
+    d = len(dims)
+    upd = (f"a = {spec['setval']:g}" if spec.get("setval") is not None else
+           f"a x {spec['scale']:g}" if spec.get("scale") is not None else "a x LN(0,2)")
+    geo = f" ({levels} levels, {leaf}^{d} leaves)" if levels else ""
+    return (f"{d}D {'5' if d == 2 else '7'}-point -div(a grad u), {'x'.join(map(str, dims))} grid "
+            f"(n={int(np.prod(dims))}), a~{'lognormal(0,1)' if spec.get('coeff', 'lognormal') == 'lognormal' else '1'}"
+            f" per node, maxSideLength {spec['max_side']}{geo}; Factor + Update ({upd} on a {spec['box']}^{d} node box) "
+            f"+ {nrhs}-RHS Solve")
 
 
 def make_batch(seed=4, nprob=256, nrhs=4, box=16, scale=100.0):
@@ -139,20 +202,72 @@ def cpu_threads():
     return max(1, min(n, 16))
 
 
-def cpu_baseline_threaded(nrhs=16):
-    """The same sample on the reference built against threaded MKL (oracle/_ref/libumfref_mt.so)
-    on all of this job's host cores.  It runs in a child process started before this one touches
-    the GPU: the threaded and sequential MKL layers cannot share a process.  None when that build
-    is absent or the child fails."""
+def host_info():
+    info = {"cpu_threads_available": cpu_threads()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    info["host_ram_gb"] = round(int(line.split()[1]) / 1e6, 1)
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_sample(name, nrhs):
+    """The reference itself (oracle/_ref, built from its sources) on the bounded sample `name`:
+    Factor + Update + nrhs single-vector Solve calls, as the reference API requires; the
+    algorithmic numerator is the one the GPU line uses."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    import umf
+    kind = "reference" if oracle_py.have_ref() else "port"
+    spec = CPU_SAMPLES[name]
+    dims = spec["dims"]
+    nrhs = spec.get("nrhs", nrhs)
+    rp, ci, v1, v2, mod, B, _ = make_problem(**dict(spec, nrhs=nrhs))
+    P = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, spec["max_side"], analyze_only=True)
+    f_factor = P.stats()["flops_factor"]
+    P.analyze_update(mod)
+    f_update = P.stats()["flops_update"]
+    f_solve = algorithmic_flops(P, nrhs)
+    F = oracle_py.Factorization(oracle_py.Csr(rp, ci, v1), dims, 1, spec["max_side"],
+                                "ref" if kind == "reference" else "orc")
+    t0 = time.perf_counter()
+    F.factor(trace=False)
+    t1 = time.perf_counter()
+    F.update(oracle_py.Csr(rp, ci, v2), mod, trace=False)
+    t2 = time.perf_counter()
+    for r in range(nrhs):
+        F.solve(B[r])
+    t3 = time.perf_counter()
+    tot = t3 - t0
+    return {"value": (f_factor + f_update + f_solve) / tot / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
+            "sample": describe(spec, nrhs, P.levels, P.leaf_width(0)) + "; the reference's LU form, MKL sequential",
+            "t_factor_s": t1 - t0, "t_update_s": t2 - t1, "t_solve_s": t3 - t2, "seconds": tot,
+            "flops": f_factor + f_update + f_solve}
+
+
+def cpu_sample_threaded(name, nrhs, threads=None):
+    """The sample on the reference built against threaded MKL (oracle/_ref/libumfref_mt.so) on
+    all of this job's host cores, in a child process started before this one touches the GPU
+    (the threaded and sequential MKL layers cannot share a process).  None when that build is
+    absent or the child fails."""
     import subprocess
     lib = os.path.join(ROOT, "oracle", "_ref", "libumfref_mt.so")
     if not os.path.exists(lib):
         return None
-    t = cpu_threads()
+    t = threads or cpu_threads()
     env = dict(os.environ, UMF_REF_SO=lib, MKL_NUM_THREADS=str(t), OMP_NUM_THREADS=str(t))
     try:
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", "--nrhs", str(nrhs)],
-                           env=env, capture_output=True, text=True, timeout=300)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", name, "--nrhs", str(nrhs)],
+                           env=env, capture_output=True, text=True, timeout=1800)
     except subprocess.TimeoutExpired:
         return None
     if r.returncode != 0 or not r.stdout.strip():
@@ -164,93 +279,90 @@ def cpu_baseline_threaded(nrhs=16):
     return out
 
 
-def cpu_baseline(N=511, seed=2, nrhs=16, max_side=16):
-    """The reference itself (oracle/_ref, MKL sequential, 1 thread) on a bounded sample of the
-    same workload: the same generator at 511^2 (5 levels, the same 31^2 leaf fronts),
-    Factor + Update + nrhs single-vector Solve calls as the reference API requires."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py
-    import umf
-    kind = "reference" if oracle_py.have_ref() else "port"
-    rp, ci, v1, v2, mod, B, _ = make_problem(N, seed, nrhs=nrhs)
-    A1, A2 = Csr(rp, ci, v1), Csr(rp, ci, v2)
-    P = umf.MF2D(A1, N, N, 1, max_side, analyze_only=True)
-    f_factor = P.stats()["flops_factor"]
-    P.analyze_update(mod)
-    f_update = P.stats()["flops_update"]
-    f_solve = algorithmic_flops(P, nrhs)
-    Ac1 = oracle_py.Csr(rp, ci, v1)
-    Ac2 = oracle_py.Csr(rp, ci, v2)
-    F = oracle_py.Factorization(Ac1, (N, N), 1, max_side, "ref" if kind == "reference" else "orc")
-    t0 = time.perf_counter()
-    F.factor(trace=False)
-    t1 = time.perf_counter()
-    F.update(Ac2, mod, trace=False)
-    t2 = time.perf_counter()
-    for r in range(nrhs):
-        F.solve(B[r])
-    t3 = time.perf_counter()
-    tot = t3 - t0
-    return {
-        "value": (f_factor + f_update + f_solve) / tot / 1e12,
-        "unit": "TFLOP/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": f"2D {N}x{N} lognormal(0,1), maxSideLength {max_side}: Factor + Update (x100 on 32x32) + "
-                  f"{nrhs} single-RHS Solve calls; reference LU form via MKL sequential",
-        "t_factor_s": t1 - t0,
-        "t_update_s": t2 - t1,
-        "t_solve_s": t3 - t2,
-        "seconds": tot,
-    }
+def fold(prof_lv):
+    """Per kernel, summed over tree levels (names carry '@L<level>')."""
+    prof = {}
+    for k, v in prof_lv.items():
+        d = prof.setdefault(k.split("@")[0], {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
+        for f in d:
+            d[f] += v[f]
+    return prof
 
 
-def backward_error(plan, d_B, d_X, d_R, vals, B, nrhs):
-    """max_j ||b_j - A x_j|| / (||A||_F ||x_j|| + ||b_j||), residual by the FP64 SpMV."""
-    plan.spmv_device(d_X.data_ptr(), d_R.data_ptr(), nrhs)
-    R = (d_B - d_R).cpu().numpy()
-    X = d_X.cpu().numpy()
-    normA = float(np.linalg.norm(vals))
-    return max(np.linalg.norm(R[j]) / (normA * np.linalg.norm(X[j]) + np.linalg.norm(B[j])) for j in range(nrhs))
+class Run:
+    """One plan and its inputs on the device; step() = Factor + Update + Solve."""
+
+    def __init__(self, torch, plan, v1, v2, mod, B, nrhs, rp):
+        self.torch, self.plan, self.mod, self.nrhs = torch, plan, mod, nrhs
+        self.d_v = [torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()]
+        self.d_B = torch.from_numpy(B).cuda()
+        self.d_X = torch.empty_like(self.d_B)
+        self.d_R = torch.empty_like(self.d_B)
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        self.norm_inf = []
+        for v in (v1, v2):  # ||A||_inf of each operator
+            rs = np.zeros(len(rp) - 1)
+            np.add.at(rs, rows, np.abs(v))
+            self.norm_inf.append(float(rs.max()))
+        self.cur = 0  # operator the plan holds
+        self.X64 = [None, None]
+
+    def step(self):
+        self.plan.Factor()
+        nxt = 1 - self.cur
+        self.plan.update_device(self.d_v[nxt].data_ptr(), self.mod)
+        self.cur = nxt
+        self.plan.solve_device(self.d_B.data_ptr(), self.d_X.data_ptr(), self.nrhs)
+
+    def errors(self):
+        """Infinity-norm backward error of the last solve (FP64 residual by the device SpMV) and
+        its forward error against the FP64 form's solution of the same operator, max over
+        columns."""
+        torch = self.torch
+        self.plan.spmv_device(self.d_X.data_ptr(), self.d_R.data_ptr(), self.nrhs)
+        R = self.d_B - self.d_R
+        den = self.norm_inf[self.cur] * self.d_X.abs().amax(dim=1) + self.d_B.abs().amax(dim=1)
+        eta = float((R.abs().amax(dim=1) / den).max())
+        fwd = None
+        ref = self.X64[self.cur]
+        if ref is not None:
+            fwd = float((torch.linalg.norm(self.d_X - ref, dim=1) / torch.linalg.norm(ref, dim=1)).max())
+        return eta, fwd
 
 
-FORM_DTYPE = {
-    "fp32": "f32 fronts (FP32 MFMA) + f64 refinement",
-    "bf16x3": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 2 bf16 pieces (3 products, f32 accumulate) "
-              "+ f64 refinement",
-}
-
-
-def low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
-    """The same step in a lower-precision form (FP32 fronts; GEMMs on FP32 MFMA or on BF16
-    pieces) with the Solve refined in FP64 to backward error <= --ir-tol; same algorithmic flop
-    numerator as the FP64 form."""
-    import torch
+def run_form(run, prec, steps, warmup, args, keep_x64=False):
+    """Time `steps` steps of one precision form after `warmup` untimed ones and one untimed
+    profiling step (every kernel bracketed by events).  In the timed steps only the form's
+    dominant kernel is bracketed by events (umf_profile_select), which gives its roofline."""
+    torch, plan = run.torch, run.plan
     plan.SetPrecision(prec, tol=args.ir_tol, max_iter=100)
     plan.profile(False)
-    for _ in range(max(args.warmup, 1)):
-        step()
-    kern = {}
-    if not args.no_profile:  # untimed step with per-kernel events
+    for i in range(warmup):
+        run.step()
+        progress(f"{prec}: warmup step {i} done")
+    prof_lv = {}
+    if not args.no_profile:
         plan.profile(True)
         plan.profile_select(None)
         plan.profile_reset()
-        step()
+        run.step()
         torch.cuda.synchronize()
-        for k, v in plan.profile_read().items():
-            d = kern.setdefault(k.split("@")[0], {"ms": 0.0, "launches": 0, "flops": 0.0})
-            for f in d:
-                d[f] += v[f]
+        prof_lv = plan.profile_read()
         plan.profile_reset()
         plan.profile(False)
+    prof = fold(prof_lv)
+    dom = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
+    if dom:
+        plan.profile(True)
+        plan.profile_select(dom)
     tf = tu = ts = flops = 0.0
     iters = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
+    for i in range(steps):
+        run.step()
         st = plan.stats()
-        progress(f"step {i}: factor {st['t_factor_ms']:.1f} ms, update {st['t_update_ms']:.1f} ms, "
+        progress(f"{prec} step {i}: factor {st['t_factor_ms']:.1f} ms, update {st['t_update_ms']:.1f} ms, "
                  f"solve {st['t_solve_ms']:.1f} ms")
         tf += st["t_factor_ms"]
         tu += st["t_update_ms"]
@@ -259,21 +371,45 @@ def low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, vals, B, args, nrhs):
         flops += st["flops_factor"] + st["flops_update"] + st["flops_solve"]
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
-    K = args.steps
+    plan.profile(False)
     st = plan.stats()
-    be = backward_error(plan, d_B, d_X, d_R, vals[state["cur"]], B, nrhs)
-    plan.SetPrecision("fp64")
-    progress(f"{prec} form: {secs / K * 1e3:.1f} ms per step, refinement iterations {iters}, berr {be:.1e}")
-    return {"value": flops / secs / 1e12, "unit": "TFLOP/s", "ms_per_step": secs / K * 1e3,
-            "t_factor_ms": tf / K, "t_update_ms": tu / K, "t_solve_ms": ts / K,
-            "ir_iters": iters, "ir_tol": args.ir_tol, "backward_error": be, "ir_berr": st["ir_berr"],
-            "front_pool_gb": st["bytes_front_pool"] / 1e9, "dtype": FORM_DTYPE[prec],
-            "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in sorted(kern.items())},
-            "kernels_tflops": {k: round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 2)
-                               for k, v in sorted(kern.items()) if v["ms"] > 0 and v["flops"]}}
+    roof = None
+    if dom:
+        d = fold(plan.profile_read()).get(dom, {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
+        plan.profile_reset()
+        launches = max(d["launches"], 1)
+        if d["flops"] > 0 and d["ms"] > 0:
+            ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
+            peak = PEAK_TFLOPS[prec]
+            roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
+        else:
+            ach = d["bytes"] / (max(d["ms"], 1e-9) * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS}
+        roof.update({"traffic": measured_traffic(dom, args.workload, prec), "kernel": dom,
+                     "launches": d["launches"], "avg_launch_ms": d["ms"] / launches,
+                     "algorithmic_flops_per_launch": d["flops"] / launches,
+                     "algorithmic_bytes_per_launch": d["bytes"] / launches,
+                     "share_of_kernel_time": d["ms"] / max(steps, 1) / max(sum(v["ms"] for v in prof.values()), 1e-9)})
+    eta, fwd = run.errors()
+    if keep_x64:  # the FP64 solutions of both operators, for the other forms' forward errors
+        run.X64[run.cur] = run.d_X.clone()
+        run.step()
+        run.X64[run.cur] = run.d_X.clone()
+    progress(f"{prec}: {secs / steps * 1e3:.1f} ms per step, iterations {iters}, eta_inf {eta:.2e}, fwd {fwd}")
+    return {"value": flops / secs / 1e12, "unit": "TFLOP/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": secs / steps * 1e3, "t_factor_ms": tf / steps, "t_update_ms": tu / steps,
+            "t_solve_ms": ts / steps, "flops_per_step": flops / steps, "ir_iters": iters if prec != "fp64" else None,
+            "ir_tol": args.ir_tol if prec != "fp64" else None, "backward_error_inf": eta,
+            "forward_error_vs_fp64": fwd, "front_pool_gb": st["bytes_front_pool"] / 1e9, "dtype": FORM_DTYPE[prec],
+            "roofline": roof,
+            "kernels": {k: {"ms_per_step": v["ms"], "launches_per_step": v["launches"],
+                            "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] else None,
+                            "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 and v["bytes"] else None}
+                        for k, v in sorted(prof.items())},
+            "kernels_by_level": {k: round(v["ms"], 4) for k, v in sorted(prof_lv.items())}}
 
 
-def factor_graph_timing(plan, torch, reps=5):
+def factor_graph_timing(plan, torch, reps=2):
     """Factor() alone, direct launches against the captured HIP graph (umf_set_graph): mean ms
     of `reps` calls each after one untimed call (the graph's capture)."""
     plan.profile(False)
@@ -290,6 +426,197 @@ def factor_graph_timing(plan, torch, reps=5):
     plan.use_graph(False)
     progress(f"Factor alone: direct {res['direct_ms']:.2f} ms, graph {res['graph_ms']:.2f} ms")
     return res
+
+
+def build_run(torch, name, nrhs=None):
+    import umf
+    spec = WORKLOADS[name]
+    dims = spec["dims"]
+    nrhs = nrhs or spec.get("nrhs", 16)
+    t0 = time.perf_counter()
+    rp, ci, v1, v2, mod, B, corner = make_problem(**dict(spec, nrhs=nrhs))
+    plan = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, spec["max_side"])
+    run = Run(torch, plan, v1, v2, mod, B, nrhs, rp)
+    desc = describe(spec, nrhs, plan.levels, plan.leaf_width(0)) + f" (update box at {corner})"
+    progress(f"workload {name}: n={int(np.prod(dims))}, {plan.levels} levels, front pool "
+             f"{plan.stats()['bytes_front_pool'] / 1e9:.1f} GB, set up in {time.perf_counter() - t0:.1f} s")
+    return run, desc, nrhs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps of the value form")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side-steps", type=int, default=8,
+                    help="timed steps of the forms beside the value form (at most --steps)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d127")
+    ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (16; 4 for the batch)")
+    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp64", "fp32", "bf16x3"],
+                    help="precision forms to time (the value form is always timed)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 2D secondary workloads")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the refined forms refine to")
+    ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)  # child of the CPU baseline
+    ap.add_argument("--cpu-only", default=None, choices=sorted(CPU_SAMPLES),
+                    help="time only the reference on the host (threaded MKL) on this sample and exit")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 1 GiB once on the GPU and exit (the HBM counters' calibration run)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the plan's autotuner (GEMM tile size, outer group width) before timing")
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
+                    help="serve95: request arrival rates (per second)")
+    ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
+    args = ap.parse_args()
+    if args.cpu_sample:  # the threaded CPU baseline's child process: no GPU work
+        print(json.dumps(cpu_sample(args.cpu_sample, args.nrhs or 16)), flush=True)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the sharded path is design-only)", file=sys.stderr)
+        sys.exit(2)
+    if args.cpu_only:
+        out = cpu_sample_threaded(args.cpu_only, args.nrhs or 16)
+        print(json.dumps({"cpu_only": out, "host": host_info()}), flush=True)
+        return
+    if args.copy_probe:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b = a.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8, "sum": float(b[:16].sum())}), flush=True)
+        return
+    if args.workload == "serve95":
+        return run_serve(args)
+    if args.workload == "batch256":
+        return run_batch(args)
+
+    spec = WORKLOADS[args.workload]
+    value_form = spec["value_form"]
+    forms = [f for f in ("fp64", "fp32", "bf16x3") if f in args.forms or f == value_form]
+    cpu_mt = None
+    if not args.no_cpu_baseline:
+        progress(f"CPU baseline: reference on sample {spec['cpu']} ({cpu_threads()} threads)")
+        cpu_mt = cpu_sample_threaded(spec["cpu"], args.nrhs or spec.get("nrhs", 16))
+    import torch
+    torch.cuda.set_device(0)
+    run, wdesc, nrhs = build_run(torch, args.workload, args.nrhs)
+    plan = run.plan
+    tuned = None
+    if args.autotune:
+        plan.SetPrecision(value_form, tol=args.ir_tol, max_iter=100)
+        plan.Factor()
+        tuned = plan.autotune(reps=2)
+        progress(f"autotune: {tuned}")
+    side = max(1, min(args.side_steps, args.steps))
+    res = {}
+    # FP64 first: its solutions of both operators are the other forms' forward-error reference
+    for f in forms:
+        is_value = f == value_form
+        res[f] = run_form(run, f, args.steps if is_value else side, args.warmup if is_value else 1, args,
+                          keep_x64=(f == "fp64"))
+    main_ = res[value_form]
+    plan.SetPrecision(value_form, tol=args.ir_tol, max_iter=100)
+    fgraph = factor_graph_timing(plan, torch)
+    out = {
+        "metric": METRIC,
+        "value": main_["value"],
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-refinement" if value_form != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": wdesc, "nrhs": nrhs, "precision": value_form, "ir_tol": args.ir_tol,
+                   "tune": plan.tune(), "autotuned": tuned is not None},
+        "t_factor_ms": main_["t_factor_ms"],
+        "t_update_ms": main_["t_update_ms"],
+        "t_solve_ms": main_["t_solve_ms"],
+        "flops_per_step": main_["flops_per_step"],
+        "backward_error_inf": main_["backward_error_inf"],
+        "forward_error_vs_fp64": main_["forward_error_vs_fp64"],
+        "roofline": main_["roofline"],
+        "cpu_baseline": None,
+        "factor_graph": fgraph,
+    }
+    for f in forms:
+        if f == value_form:
+            out[f] = {k: main_[k] for k in ("kernels", "kernels_by_level", "ir_iters", "front_pool_gb")}
+            continue
+        key = f if f == "fp64" else f + "_ir"
+        out[key] = res[f]
+    if "fp64" in res:  # wall time of the FP64 form over each form's (same work)
+        out["speedup_vs_fp64"] = {f: res["fp64"]["ms_per_step"] / res[f]["ms_per_step"] for f in res}
+    out["host"] = host_info()
+    if cpu_mt is not None:
+        out["cpu_baseline"] = cpu_mt
+        out["gpu_over_cpu"] = out["value"] / cpu_mt["value"]
+    if args.workload == "3d127" and not args.no_secondary:
+        del run, plan
+        torch.cuda.empty_cache()
+        out["secondary"] = secondary(torch, args)
+    print(json.dumps(out), flush=True)
+
+
+def secondary(torch, args):
+    """The 2D configurations beside the 3D headline: 2D 1023^2 (FP64 and FP32 + refinement) and
+    2D 127^2 (the plain FP64 form against the reference's own CPU path, one core, MKL
+    sequential, at full size)."""
+    out = {}
+    run, desc, nrhs = build_run(torch, "2d1023")
+    a = argparse.Namespace(**vars(args))
+    a.workload = "2d1023"
+    r64 = run_form(run, "fp64", 10, 2, a, keep_x64=True)
+    r32 = run_form(run, "fp32", 10, 2, a)
+    out["2d1023"] = {"workload": desc, "fp64": {k: r64[k] for k in ("value", "ms_per_step", "backward_error_inf")},
+                     "fp32_ir": {k: r32[k] for k in ("value", "ms_per_step", "backward_error_inf",
+                                                      "forward_error_vs_fp64", "ir_iters")},
+                     "speedup_fp32_vs_fp64": r64["ms_per_step"] / r32["ms_per_step"]}
+    del run
+    run, desc, nrhs = build_run(torch, "2d127")
+    a.workload = "2d127"
+    r64 = run_form(run, "fp64", 20, 3, a)
+    ent = {"workload": desc, "fp64_gpu": {k: r64[k] for k in ("value", "ms_per_step", "t_factor_ms", "t_update_ms",
+                                                                "t_solve_ms", "backward_error_inf")}}
+    del run
+    if not args.no_cpu_baseline:
+        cpu1 = cpu_sample("2d127", nrhs)
+        ent["reference_cpu_1core"] = cpu1
+        ent["gpu_over_cpu_1core"] = r64["value"] / cpu1["value"]
+    out["2d127"] = ent
+    return out
+
+
+def run_batch(args):
+    """The batched workload (make_batch): every form on one plan of 256 problems."""
+    import torch
+    torch.cuda.set_device(0)
+    nrhs = args.nrhs or 4
+    plan, rp, ci, v1, v2, mod, B, wdesc = make_batch(nrhs=nrhs)
+    run = Run(torch, plan, v1, v2, mod, B, nrhs, rp)
+    tuned = None
+    if args.autotune:
+        tuned = plan.autotune(reps=2)
+    res = {}
+    for f in ("fp64", "fp32", "bf16x3"):
+        if f in args.forms:
+            res[f] = run_form(run, f, args.steps, args.warmup, args, keep_x64=(f == "fp64"))
+    main_ = res.get("fp32") or next(iter(res.values()))
+    out = {"metric": METRIC, "value": main_["value"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": main_["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64-refinement", "data": "synthetic",
+           "config": {"workload": wdesc, "nrhs": nrhs, "tune": plan.tune(), "autotuned": tuned is not None},
+           "forms": res, "cpu_baseline": None}
+    if "fp64" in res:
+        out["speedup_vs_fp64"] = {f: res["fp64"]["ms_per_step"] / res[f]["ms_per_step"] for f in res}
+    print(json.dumps(out), flush=True)
 
 
 def run_serve(args):
@@ -363,190 +690,6 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d1023")
-    ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (16; 4 for the batch)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the lower-precision + FP64-refinement forms")
-    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp32", "bf16x3"],
-                    help="lower-precision forms measured beside FP64 (each refined in FP64)")
-    ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the FP32 form refines to")
-    ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)  # child of the CPU baseline
-    ap.add_argument("--autotune", action="store_true",
-                    help="run the plan's autotuner (GEMM tile size, outer group width) before timing")
-    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
-                    help="serve95: request arrival rates (per second)")
-    ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
-    args = ap.parse_args()
-    if args.cpu_sample:  # the threaded CPU baseline's child process: no GPU work
-        print(json.dumps(cpu_baseline(nrhs=args.nrhs or 16)), flush=True)
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the sharded path is design-only)", file=sys.stderr)
-        sys.exit(2)
-
-    if args.workload == "serve95":
-        return run_serve(args)
-    cpu_mt = None
-    if not args.no_cpu_baseline and args.workload == "2d1023":
-        progress(f"threaded CPU baseline ({cpu_threads()} threads)")
-        cpu_mt = cpu_baseline_threaded(nrhs=args.nrhs or 16)
-    import torch
-    import umf
-    torch.cuda.set_device(0)
-    if args.workload == "batch256":
-        nrhs = args.nrhs or 4
-        plan, rp, ci, v1, v2, mod, B, wdesc = make_batch(nrhs=nrhs)
-        n = plan.n
-    else:
-        dims, max_side, box, scale, seed = WORKLOADS[args.workload]
-        nrhs = args.nrhs or 16
-        rp, ci, v1, v2, mod, B, corner = make_problem(dims[0], seed, box=box, scale=scale, nrhs=nrhs, dims=dims)
-        n = int(np.prod(dims))
-        plan = (umf.MF2D if len(dims) == 2 else umf.MF3D)(Csr(rp, ci, v1), *dims, 1, max_side)
-        wdesc = (f"{len(dims)}D {'5' if len(dims) == 2 else '7'}-point -div(a grad u), "
-                 f"{'x'.join(map(str, dims))} grid (n={n}), a~lognormal(0,1) per node, "
-                 f"maxSideLength {max_side} ({plan.levels} levels, {plan.leaf_width(0)}^{len(dims)} leaves), "
-                 f"Factor + Update (a x{'100' if scale else 'LN(0,2)'} on {box}^{len(dims)} nodes at {corner}) "
-                 f"+ {nrhs}-RHS Solve, FP64")
-    d_v = [torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()]
-    d_B = torch.from_numpy(B).cuda()
-    d_X = torch.empty_like(d_B)
-    d_R = torch.empty_like(d_B)
-
-    state = {"cur": 0}  # index of the operator the plan currently holds
-
-    def step():
-        plan.Factor()
-        nxt = 1 - state["cur"]
-        plan.update_device(d_v[nxt].data_ptr(), mod)
-        state["cur"] = nxt
-        plan.solve_device(d_B.data_ptr(), d_X.data_ptr(), nrhs)
-
-    progress(f"workload {args.workload}: n={n}, {plan.levels} levels, front pool "
-             f"{plan.stats()['bytes_front_pool'] / 1e9:.1f} GB")
-    tuned = None
-    if args.autotune:
-        tuned = plan.autotune(reps=2)
-        progress(f"autotune: {tuned}")
-    for i in range(args.warmup):
-        step()
-        progress(f"warmup step {i} done")
-    # untimed profiling step: every kernel bracketed by events -> per-kernel table, dominant kernel
-    prof_lv = {}
-    if not args.no_profile:
-        plan.profile(True)
-        plan.profile_select(None)
-        plan.profile_reset()
-        step()
-        torch.cuda.synchronize()
-        prof_lv = plan.profile_read()
-        plan.profile_reset()
-    prof = {}  # per kernel, summed over tree levels (names carry "@L<level>")
-    for k, v in prof_lv.items():
-        d = prof.setdefault(k.split("@")[0], {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
-        for f in d:
-            d[f] += v[f]
-    dom = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
-    if dom:
-        plan.profile_select(dom)  # the timed region carries events around this kernel only
-    tf = tu = ts = 0.0
-    flops = 0.0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        st = plan.stats()
-        progress(f"step {i}: factor {st['t_factor_ms']:.1f} ms, update {st['t_update_ms']:.1f} ms, "
-                 f"solve {st['t_solve_ms']:.1f} ms")
-        tf += st["t_factor_ms"]
-        tu += st["t_update_ms"]
-        ts += st["t_solve_ms"]
-        flops += st["flops_factor"] + st["flops_update"] + st["flops_solve"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    plan.profile(False)
-    secs = t1 - t0
-    K = args.steps
-
-    # accuracy of the last step: normwise backward error max_j ||b - A x|| / (||A||_F ||x|| + ||b||)
-    berr = backward_error(plan, d_B, d_X, d_R, (v1, v2)[state["cur"]], B, nrhs)
-
-    roof = None
-    if dom:
-        d = {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0}  # the dominant kernel in the timed region
-        for k, v in plan.profile_read().items():
-            if k.split("@")[0] == dom:
-                for f in d:
-                    d[f] += v[f]
-        avg_ms = d["ms"] / max(d["launches"], 1)
-        if d["flops"] > 0:
-            ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
-            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS,
-                    "traffic": measured_traffic(dom, args.workload)}
-        else:
-            ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None}
-        roof.update({"kernel": dom, "launches": d["launches"], "avg_launch_ms": avg_ms,
-                     "share_of_kernel_time": d["ms"] / K / sum(v["ms"] for v in prof.values())})
-
-    st = plan.stats()
-    out = {
-        "metric": METRIC,
-        "value": flops / secs / 1e12,
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": secs / K * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": wdesc,
-            "nrhs": nrhs,
-            "precision": "fp64",
-            "tune": plan.tune(),
-            "autotuned": tuned is not None,
-        },
-        "t_factor_ms": tf / K,
-        "t_update_ms": tu / K,
-        "t_solve_ms": ts / K,
-        "flops_per_step": flops / K,
-        "fronts": st["n_fronts"],
-        "fronts_update": st["n_fronts_update"],
-        "front_pool_gb": st["bytes_front_pool"] / 1e9,
-        "backward_error": berr,
-        "roofline": roof,
-        "kernels": {k: {"ms_per_step": v["ms"], "launches_per_step": v["launches"],
-                        "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] else None,
-                        "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 and v["bytes"] else None}
-                    for k, v in prof.items()},
-        "kernels_by_level": {k: round(v["ms"], 4) for k, v in sorted(prof_lv.items())},
-        "cpu_baseline": None,
-    }
-    out["factor_graph"] = factor_graph_timing(plan, torch)
-    if not args.no_fp32:
-        for prec in args.forms:
-            out[prec + "_ir"] = low_form(prec, plan, step, state, d_v, d_B, d_X, d_R, (v1, v2), B, args, nrhs)
-    if not args.no_cpu_baseline and args.workload == "2d1023":
-        one = cpu_baseline(nrhs=nrhs)
-        if cpu_mt is not None:  # all host cores (threaded MKL), with the 1-thread run beside it
-            cpu_mt["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "kind", "seconds")}
-            out["cpu_baseline"] = cpu_mt
-        else:
-            out["cpu_baseline"] = one
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -175,7 +175,7 @@ typedef struct umf_stats {
   int n_fronts;
   int n_fronts_update;
   int ir_iters;         /* FP32 form: refinement iterations of the last solve (max over columns) */
-  double ir_berr;       /* FP32 form: final normwise backward error (max over columns) */
+  double ir_berr;       /* FP32 form: final normwise backward error in the infinity norm (max over columns) */
 } umf_stats;
 int umf_get_stats(const umf_plan* plan, umf_stats* out);
 
@@ -184,7 +184,7 @@ int umf_get_stats(const umf_plan* plan, umf_stats* out);
  *   UMF_PREC_FP32: fronts stored and factored in FP32 on v_mfma_f32_16x16x4 (half the front
  *     memory); umf_solve then runs preconditioned conjugate gradients in FP64 (FP64 operator
  *     and residuals, the FP32 multifrontal solve as preconditioner) until every column reaches
- *     the normwise backward error ||b - A x|| / (||A||_F ||x|| + ||b||) <= tol, at most max_iter
+ *     the normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol, at most max_iter
  *     iterations (UMF_ENOCONV when a column does not); umf_apply applies the FP32 factors.
  *   UMF_PREC_BF16X3: the FP32 form with the factorization's Schur and trailing-update GEMMs
  *     on v_mfma_f32_16x16x32_bf16: each FP32 operand is split into two bfloat16 pieces,
@@ -208,6 +208,13 @@ int umf_profile_select(umf_plan* plan, const char* kernel);
 int umf_profile_reset(umf_plan* plan);
 int umf_profile_read(const umf_plan* plan, int i, char* name, int name_len, double* ms_total, int* launches,
                      double* flops, double* bytes);
+
+/* Publication between the workgroups of the blocked solve sweeps (no reference counterpart):
+ * fenced = 0 (default) relies on gfx950's write-through agent-scope atomics, with no cache
+ * maintenance; fenced = 1 adds agent-scope release / acquire fences around the block flags,
+ * which the HIP memory model orders formally (slower: L2 write-back / invalidate per block).
+ * Results are identical. */
+int umf_set_sweep_sync(umf_plan* plan, int fenced);
 
 /* hipStream_t of the plan (for callers ordering their own work), returned as void*. */
 void* umf_stream(umf_plan* plan);

@@ -262,4 +262,73 @@ int ref_laplacian(int dim, int nx, int ny, int nz, const double* coeff, int w, d
   return nnz;
 }
 
+// ---- the reference's LinAlg free functions (include/LinAlg/dense.hpp:167-227, :281-310) on
+// column-major arrays, for checking the drop-in header's versions.
+static Dense<double> to_dense(int m, int n, const double* a) {
+  Dense<double> A(m, n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) A(i, j) = a[(size_t)j * m + i];
+  return A;
+}
+static void from_dense(const Dense<double>& A, double* a) {
+  for (int j = 0; j < A.n_; j++)
+    for (int i = 0; i < A.m_; i++) a[(size_t)j * A.m_ + i] = A.data_[(size_t)j * A.m_ + i];
+}
+// LU(A) in place (n x n), ipiv (1-based) out.
+void ref_dense_lu(int n, double* a, int* ipiv) {
+  Dense<double> A = to_dense(n, n, a);
+  LU(A);
+  from_dense(A, a);
+  for (int i = 0; i < n; i++) ipiv[i] = A.ipiv_[i];
+}
+// After ref_dense_lu: mode 0 LUSolve (b: n x nrhs), 1 LUSolveT (b: nrhs x n, solves X A = B),
+// 2 LUSolveVec (b: n).
+void ref_dense_lusolve(int n, const double* lu, const int* ipiv, int nrhs, double* b, int mode) {
+  Dense<double> A = to_dense(n, n, lu);
+  A.ipiv_.assign(ipiv, ipiv + n);
+  if (mode == 2) {
+    Vec<double> v(n);
+    for (int i = 0; i < n; i++) v(i) = b[i];
+    LUSolveVec(A, v);
+    for (int i = 0; i < n; i++) b[i] = v(i);
+    return;
+  }
+  Dense<double> B = mode == 0 ? to_dense(n, nrhs, b) : to_dense(nrhs, n, b);
+  if (mode == 0) LUSolve(A, B);
+  else LUSolveT(A, B);
+  from_dense(B, b);
+}
+// C := alpha op(A) op(B) + beta C (GEMM); c is cm x cn, updated in place.
+void ref_dense_gemm(double alpha, int am, int an, const double* a, char ta, int bm, int bn, const double* b, char tb,
+                    double beta, int cm, int cn, double* c) {
+  Dense<double> A = to_dense(am, an, a), B = to_dense(bm, bn, b), C = to_dense(cm, cn, c);
+  GEMM(alpha, A, ta, B, tb, beta, C);
+  from_dense(C, c);
+}
+// transpose(A, AT): at is n x m.
+void ref_dense_transpose(int m, int n, const double* a, double* at) {
+  Dense<double> A = to_dense(m, n, a), AT;
+  transpose(A, AT);
+  from_dense(AT, at);
+}
+// The reference's CSR text reader, Sparse(filename) (src/LinAlg/sparse.cpp:5-44): returns nnz
+// and writes m, n; with rp non-null also the arrays (the caller sizes them from a first call).
+// Returns -1 when the reader throws (its message in ref_last_error).
+int ref_read_csr(const char* path, int* m, int* n, int* rp, int* ci, double* val) {
+  try {
+    Sparse<double> A{std::string(path)};
+    *m = A.m();
+    *n = A.n();
+    const int nnz = A.nnz();
+    if (rp) {
+      for (int i = 0; i <= *m; i++) rp[i] = A.row_ptr(i);
+      for (int k = 0; k < nnz; k++) { ci[k] = A.col_ind(k); val[k] = A.val(k); }
+    }
+    return nnz;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 }  // extern "C"

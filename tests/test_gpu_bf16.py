@@ -15,6 +15,7 @@ stay on FP32 MFMA) yet within the FP32 form's block bound times ten.
 import numpy as np
 import pytest
 
+from metrics import berr
 from oracle_py import Factorization, coeff_field, laplacian, moddofs
 
 pytestmark = pytest.mark.gpu
@@ -47,14 +48,6 @@ def _mk(dims, maxs, kind, seed=11):
 def relerr(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
-
-
-def berr(A, X, B):
-    X = X.reshape(A.n, -1)
-    B = B.reshape(A.n, -1)
-    na = np.linalg.norm(A.val)
-    return max(np.linalg.norm(B[:, j] - A.matvec(X[:, j])) / (na * np.linalg.norm(X[:, j]) + np.linalg.norm(B[:, j]))
-               for j in range(X.shape[1]))
 
 
 def inv_norm(A, O, iters=30):

@@ -48,10 +48,10 @@ template <> __device__ __forceinline__ int mrow<float>(int q, int r) { return MR
 #define SPIN_LIMIT (1 << 22)  // polls of a bounded wait between workgroups of one launch
 
 // ----------------------------------------------------------------------------- assembly
-// Zero n ints (the factorization's error flags at its start: a kernel node, so that a captured
-// factorization holds no memset node).
+// Zero n ints (the factorization's error flags and arrival counters at its start: a kernel
+// node, so that a captured factorization holds no memset node).
 __global__ void k_zero_ints(int* __restrict__ p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // Zero the lower triangle (row >= col) of each listed matrix.
@@ -1452,6 +1452,15 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
 // relaxed store sets the flag.  (An agent-scope release / acquire fence instead writes back and
 // invalidates the XCD's L2 at every block, which also evicts the factor rows the other
 // workgroups are streaming.)
+// This relies on how gfx950 code is generated for these operations, not on the HIP memory
+// model, which orders nothing between the relaxed data stores and the relaxed flag store: the
+// agent-scope atomic stores are written through (sc1) and complete in order before the
+// s_waitcnt, and the loads after the flag are agent-scope atomic loads that bypass the
+// non-coherent caches.  FENCED = true adds the agent-scope release fence before the flag store
+// and the acquire fence after the flag is seen (formally ordered under the memory model, at
+// the cost of the L2 write-back / invalidate); the runtime selects it per plan
+// (umf_set_sweep_sync) and the parity tests compare the two.
+template <bool FENCED>
 __device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err) {
   if (threadIdx.x == 0) {
     int spins = 0;
@@ -1462,13 +1471,18 @@ __device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err)
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    if (FENCED) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
+template <bool FENCED>
 __device__ __forceinline__ void post_block(int* flag, int epoch) {
   __builtin_amdgcn_s_waitcnt(0);  // this wave's published stores have completed
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    if (FENCED) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 // published rows: relaxed agent-scope atomic accesses
 template <class T>
@@ -1484,7 +1498,7 @@ __device__ __forceinline__ void st_pub(T* p, T v) {
 // b waits for the blocks j < b, forms v = b_b - sum_j L_bj x_j and publishes x_b = Linv_b v
 // (to X and to W[int]); boundary block b accumulates over every interior block and leaves
 // X[rows] = -sum_j L_bj x_j, the contribution gathered by k_sep_acc.
-template <class T>
+template <class T, bool FENCED>
 __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                  const T* __restrict__ A, const int* __restrict__ dofs,
                                                  T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
@@ -1522,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
       const int k = 4 * k4 + q;
       av[k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + rowc, rok && k < kb);
     }
-    wait_block(fl + j, epoch, err);
+    wait_block<FENCED>(fl + j, epoch, err);
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
@@ -1558,13 +1572,13 @@ __global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, co
       W[(i64)di[i] * RB + c16] = a2[r];
     }
   }
-  post_block(fl + b, epoch);
+  post_block<FENCED>(fl + b, epoch);
 }
 
 // Backward sweep of one front by blocks, last block first (workgroup b owns block
 // nbi-1-b): z = y_i - L(rows below, i)^T x(rows below), boundary rows (read from W[bd]) first,
 // then the interior blocks below as they are published; x_i = Linv_i^T z goes to X and W[int].
-template <class T>
+template <class T, bool FENCED>
 __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                  const T* __restrict__ A, const int* __restrict__ dofs,
                                                  T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
@@ -1617,7 +1631,7 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
       const int k = 4 * k4 + q;
       av[k4] = ld_or0(Lc + k0 + min(k, kn - 1), k < kn && cin);
     }
-    wait_block(fl + kbk, epoch, err);
+    wait_block<FENCED>(fl + kbk, epoch, err);
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) {
       const int k = 4 * k4 + q;
@@ -1644,7 +1658,7 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
       W[(i64)di[jb + i] * RB + c16] = a2[r];
     }
   }
-  post_block(fl + ib, epoch);
+  post_block<FENCED>(fl + ib, epoch);
 }
 
 // Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.
@@ -1797,21 +1811,6 @@ __global__ __launch_bounds__(256) void k_dots(int n, int npair, const double* __
     part[(i64)blockIdx.x * 32 + threadIdx.x] = t;
   }
 }
-// Sum of squares of a plain vector, first pass (part[g], fixed order).
-__global__ __launch_bounds__(256) void k_sumsq(i64 n, const double* __restrict__ v, double* __restrict__ part) {
-  const i64 chunk = (n + gridDim.x - 1) / gridDim.x;
-  const i64 i0 = (i64)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-  double s = 0.0;
-  for (i64 i = i0 + threadIdx.x; i < i1; i += 256) s += v[i] * v[i];
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x * 32] = red[0];
-}
 // Second pass: out[c] = sum_g part[g*32 + c], c < 32, in a fixed order (8 interleaved partial
 // sums per column, then combined).
 __global__ __launch_bounds__(256) void k_dots_final(int nblk, const double* __restrict__ part,
@@ -1825,6 +1824,68 @@ __global__ __launch_bounds__(256) void k_dots_final(int nblk, const double* __re
   if (threadIdx.x < 32) {
     double t = 0.0;
     for (int q = 0; q < 8; q++) t += red[q * 32 + c];
+    out[c] = t;
+  }
+}
+// Max that keeps a NaN (a NaN residual must not read as converged).
+__device__ __forceinline__ double nanmax(double m, double v) { return (v > m || v != v) ? v : m; }
+// Per-column max |a| of one or two row-major n x 16 blocks, first pass: block g covers rows
+// [g*chunk, (g+1)*chunk) into part[g*32 + 16*pair + r].  (Max is order-independent: the result
+// does not depend on the grid.)
+__global__ __launch_bounds__(256) void k_absmax(int n, int npair, const double* __restrict__ a0,
+                                                const double* __restrict__ a1, double* __restrict__ part) {
+  const int r = threadIdx.x & 15, rl = threadIdx.x >> 4;
+  const int chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double m0 = 0.0, m1 = 0.0;
+  for (int i = i0 + rl; i < i1; i += 16) {
+    const i64 e = (i64)i * RB + r;
+    m0 = nanmax(m0, fabs(a0[e]));
+    if (npair > 1) m1 = nanmax(m1, fabs(a1[e]));
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = m0;
+  red[1][threadIdx.x] = m1;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int pr = threadIdx.x >> 4, c = threadIdx.x & 15;
+    double t = 0.0;
+    for (int q = 0; q < 16; q++) t = nanmax(t, red[pr][q * 16 + c]);
+    part[(i64)blockIdx.x * 32 + threadIdx.x] = t;
+  }
+}
+// ||A||_inf of a CSR operator, first pass: part[g*32] = max over block g's rows of sum_j |a_ij|.
+__global__ __launch_bounds__(256) void k_rowabs(int n, const int* __restrict__ rp, const double* __restrict__ v,
+                                                double* __restrict__ part) {
+  const int chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double m = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; k++) s += fabs(v[k]);
+    m = nanmax(m, s);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 32) part[blockIdx.x * 32 + threadIdx.x] = threadIdx.x == 0 ? red[0] : 0.0;
+}
+// Second pass of the maxima: out[c] = max_g part[g*32 + c], c < 32.
+__global__ __launch_bounds__(256) void k_max_final(int nblk, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  const int c = threadIdx.x & 31, s8 = threadIdx.x >> 5;
+  double m = 0.0;
+  for (int g = s8; g < nblk; g += 8) m = nanmax(m, part[(i64)g * 32 + c]);
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    double t = 0.0;
+    for (int q = 0; q < 8; q++) t = nanmax(t, red[q * 32 + c]);
     out[c] = t;
   }
 }

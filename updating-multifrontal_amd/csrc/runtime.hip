@@ -103,6 +103,7 @@ struct umf_plan {
   int blist_cap = 0;
   int bepoch = 0;
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
+  bool fenced = false;     // blocked sweeps publish with agent-scope release / acquire fences
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
   // tuning of the factorization's GEMM stages (umf_set_tune / umf_autotune): the outer column
@@ -130,7 +131,7 @@ struct umf_plan {
   double* d_cg = nullptr;    // 5 row-major n x RB blocks: B, X, R, P, Q/Z
   double* d_part = nullptr;  // reduction partials (DOT_BLOCKS x 32)
   double* d_dots = nullptr;  // 4 x 32 reduction results
-  double norm_a = 0;         // ||A||_F of the current operator
+  double norm_a = 0;         // ||A||_inf of the current operator
   double* d_stage = nullptr;  // device copy of host right-hand sides
   int stage_cols = 0;
   i64 xs_elems = 0;
@@ -308,8 +309,12 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
                             p->stream));
     ids = p->d_ids;
   }
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
   const int NF = P.nfronts();
+  // error flags, and the fused diagonal kernel's arrival counters: a launch whose wait timed out
+  // may leave a count behind, which must not reach the next factorization
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
+  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)std::min((NF + 255) / 256, 1024)), dim3(256), 0, p->stream,
+                     p->d_arrive, NF);
   auto lowerN = [](double N) { return N * (N + 1) / 2; };
   for (int ell = P.L - 1; ell >= 0; ell--) {
     const Sched::Lvl& L = S.lv[ell];
@@ -349,7 +354,13 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     //    GW: each block updates only the rest of its group (K = 64), and the group then updates
     //    the trailing matrix once (K = GW), so the trailing read-modify-write happens once per GW
     //    columns.
-    const bool left = L.f_cnt >= 32;
+    // The choice of algorithm (left- or right-looking, fused or lazy column groups) fixes the
+    // order of every front's floating-point operations, so it is made from the plan's whole
+    // level, never from the fronts this pass happens to factor: an Update then computes every
+    // dirty front exactly as a full factorization of the new operator would, bit for bit, and a
+    // front's arithmetic does not depend on what else shares its launches.
+    const std::vector<int>& level_fronts = P.box_gid[ell];
+    const bool left = level_fronts.size() >= 32;
     // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
     // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
     bool fly = left && !P.rest_off.empty();
@@ -437,10 +448,23 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         if (k0 < F.n) rows = std::max(rows, F.n + F.m - k0 - std::min(NB, F.n - k0));
       }
       // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
-      // repeats the diagonal factorization in each) run diagonal block and panel in one launch
+      // repeats the diagonal factorization in each) run diagonal block and panel in one launch;
+      // decided over the whole level (above), whose fronts bound this pass's, so a fused launch
+      // of this pass fits one round too
       const int ptd = std::max(1, (rows + NB - 1) / NB);
-      if (k0 == g0) lazy = !left && (i64)ptd * act <= 512;
-      const bool fused = !left && (lazy || (i64)ptd * act <= 512);
+      i64 fit = 0;  // workgroups of the step over the whole level
+      if (!left) {
+        int rows_all = 0, act_all = 0;
+        for (int f : level_fronts) {
+          const FrontInfo& F = P.fronts[f];
+          if (k0 >= F.n) continue;
+          act_all++;
+          rows_all = std::max(rows_all, F.n + F.m - k0 - std::min(NB, F.n - k0));
+        }
+        fit = (i64)std::max(1, (rows_all + NB - 1) / NB) * act_all;
+      }
+      if (k0 == g0) lazy = !left && fit <= 512;
+      const bool fused = !left && (lazy || fit <= 512);
       const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
       const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
       const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
@@ -747,9 +771,14 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
     }
   };
   double fl, by;
-  // Levels of few (<= 16), large fronts are swept by 64-row blocks, one workgroup each
-  // (k_fwd_blk / k_bwd_blk); levels of many fronts keep one workgroup per front.
-  auto blocked = [&](int ell) { return A.f_cnt[ell] <= 16; };
+  // Levels of few (<= 16) fronts, or of at most 128 fronts of 2048+ rows, are swept by 64-row
+  // blocks, one workgroup each (k_fwd_blk / k_bwd_blk): one workgroup per front would leave
+  // most of the 256 CUs idle while it streams its factor (3D 127^3: 64 fronts of 8557 rows at
+  // 0.7 TB/s).  Levels of many fronts keep one workgroup per front.
+  std::vector<int> maxN(L, 0);
+  for (int ell = 0; ell < L; ell++)
+    for (int f : P.box_gid[ell]) maxN[ell] = std::max(maxN[ell], P.fronts[f].n + P.fronts[f].m);
+  auto blocked = [&](int ell) { return A.f_cnt[ell] <= 16 || (A.f_cnt[ell] <= 128 && maxN[ell] >= 2048); };
   auto blocks = [&](int ell, int& nbi, int& nbt) {
     nbi = nbt = 1;
     for (int f : P.box_gid[ell]) {
@@ -798,7 +827,8 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
     } else {
       int nbi, nbt;
       blocks(ell, nbi, nbt);
-      launch(p, "k_fwd_blk", fl, by, k_fwd_blk<T>, dim3(nbt, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+      launch(p, "k_fwd_blk", fl, by, p->fenced ? k_fwd_blk<T, true> : k_fwd_blk<T, false>, dim3(nbt, A.f_cnt[ell]),
+             ids + A.f_off[ell], fr,
              (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
     }
   };
@@ -818,7 +848,8 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
       } else {
         int nbi, nbt;
         blocks(ell, nbi, nbt);
-        launch(p, "k_bwd_blk", fl, by, k_bwd_blk<T>, dim3(nbi, A.f_cnt[ell]), ids + A.f_off[ell], fr,
+        launch(p, "k_bwd_blk", fl, by, p->fenced ? k_bwd_blk<T, true> : k_bwd_blk<T, false>, dim3(nbi, A.f_cnt[ell]),
+               ids + A.f_off[ell], fr,
                (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
       }
     }
@@ -847,6 +878,14 @@ void dots(umf_plan* p, const double* a0, const double* b0, const double* a1, con
   launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, out);
 }
 
+// Per-column max |.| of one or two row-major n x RB blocks into out[0:16] (and out[16:32]).
+void absmax(umf_plan* p, const double* a0, const double* a1, double* out) {
+  const int n = p->P.n;
+  launch(p, "k_absmax", 0, 8.0 * RB * n * (a1 ? 2 : 1), k_absmax, dim3(DOT_BLOCKS), n, a1 ? 2 : 1, a0, a1 ? a1 : a0,
+         p->d_part);
+  launch(p, "k_max_final", 0, 0, k_max_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, out);
+}
+
 // FP32 form: z = M^{-1} r with M the FP32 multifrontal factorization (r, z FP64 row-major).
 int precondition(umf_plan* p, const double* r, double* z) {
   const i64 n16 = (i64)p->P.n * RB;
@@ -860,8 +899,9 @@ int precondition(umf_plan* p, const double* r, double* z) {
 
 // FP32 form of Solve for nr <= RB columns: preconditioned conjugate gradients in FP64 (FP64
 // operator and residuals, FP32 factor as preconditioner), x0 = 0, each column stopped once its
-// normwise backward error ||r|| / (||A||_F ||x|| + ||b||) <= ir_tol.  Returns the iteration count
-// and the largest final backward error through it / berr.
+// normwise backward error in the infinity norm, ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf),
+// is <= ir_tol.  Returns the iteration count and the largest final backward error through
+// it / berr.
 int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out, double& berr_out) {
   const int n = p->P.n;
   const i64 n16 = (i64)n * RB;
@@ -872,17 +912,18 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   double* Pd = R + n16;
   double* QZ = Pd + n16;
   double* S1 = p->d_dots + 32;  // p.q
-  double* S2 = p->d_dots + 64;  // r.r | x.x
+  double* S2 = p->d_dots + 64;  // max |r| | max |x| per column
   double* rz = p->d_dots;       // r.z, current
   double* rzn = p->d_dots + 96; // r.z, next
-  // ||A||_F of the operator the plan holds
+  // ||A||_inf of the operator the plan holds
   const i64 nnz = (i64)p->P.ci.size();
-  launch(p, "k_sumsq", 0, 8.0 * nnz, k_sumsq, dim3(DOT_BLOCKS), nnz, (const double*)p->d_val, p->d_part);
-  launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
+  launch(p, "k_rowabs", 0, 12.0 * nnz, k_rowabs, dim3(DOT_BLOCKS), n, (const int*)p->d_rp, (const double*)p->d_val,
+         p->d_part);
+  launch(p, "k_max_final", 0, 0, k_max_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
   launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3((n + 63) / 64), n, in, ld, nr, Bv);
   HIPCHK(hipMemsetAsync(X, 0, sizeof(double) * n16, p->stream));
   HIPCHK(hipMemcpyAsync(R, Bv, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
-  dots(p, Bv, Bv, nullptr, nullptr, S2);
+  absmax(p, Bv, nullptr, S2);
   double h[32], na;
   HIPCHK(hipMemcpyAsync(&na, p->d_dots + 128, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
@@ -891,9 +932,9 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   HIPCHK(hipMemcpyAsync(Pd, QZ, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
   dots(p, R, QZ, nullptr, nullptr, rz);
   HIPCHK(hipStreamSynchronize(p->stream));
-  p->norm_a = std::sqrt(na);
+  p->norm_a = na;
   double nb[RB];
-  for (int c = 0; c < RB; c++) nb[c] = std::sqrt(h[c]);
+  for (int c = 0; c < RB; c++) nb[c] = h[c];
   int active = (1 << nr) - 1;
   double berr[RB] = {0};
   int it = 0;
@@ -904,13 +945,13 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
     dots(p, Pd, QZ, nullptr, nullptr, S1);
     launch(p, "k_cg_xr", 0, 0, k_cg_xr, dim3(g16), n16, (const double*)rz, (const double*)S1, active,
            (const double*)Pd, (const double*)QZ, X, R);
-    dots(p, R, R, X, X, S2);
+    absmax(p, R, X, S2);
     HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     for (int c = 0; c < nr; c++) {
       if (!((active >> c) & 1)) continue;
-      const double den = p->norm_a * std::sqrt(h[16 + c]) + nb[c];
-      berr[c] = den > 0 ? std::sqrt(h[c]) / den : 0.0;
+      const double den = p->norm_a * h[16 + c] + nb[c];
+      berr[c] = den > 0 ? h[c] / den : (h[c] > 0 ? INFINITY : 0.0);
       if (berr[c] <= p->ir_tol) active &= ~(1 << c);
     }
     if (!active) break;
@@ -1377,7 +1418,7 @@ int umf_plan_load(const char* path, umf_plan** out) {
     return bad("not a plan file");
   auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_BF16X3; };
   if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0 ||
-      !prec_ok(h.prec) || !prec_ok(h.fact_prec) ||
+      !prec_ok(h.prec) || !prec_ok(h.fact_prec) || !(h.ir_tol > 0) || !std::isfinite(h.ir_tol) || h.ir_max < 1 ||
       (h.elem_size != 0 && (h.elem_size == 8) != (h.fact_prec == UMF_PREC_FP64)))
     return bad("corrupt header");
   std::vector<int> rp((size_t)h.n + 1), ci((size_t)h.nnz);
@@ -1842,6 +1883,12 @@ int umf_set_graph(umf_plan* p, int on) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
   p->use_graph = on != 0;
+  return UMF_OK;
+}
+
+int umf_set_sweep_sync(umf_plan* p, int fenced) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  p->fenced = fenced != 0;
   return UMF_OK;
 }
 

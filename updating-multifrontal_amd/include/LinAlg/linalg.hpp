@@ -9,6 +9,7 @@
 #ifndef UMF_LINALG_HPP
 #define UMF_LINALG_HPP 1
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -48,10 +49,6 @@ class Vec {
     double s = 0;
     for (const Scalar& x : data_) s += x * x;
     return std::sqrt(s);
-  }
-  void print() const {
-    for (const Scalar& x : data_) std::cout << x << ' ';
-    std::cout << std::endl;
   }
   Scalar* data() { return data_.data(); }
   const Scalar* data() const { return data_.data(); }
@@ -109,18 +106,16 @@ class Dense {
     for (const Scalar& x : data_) s += x * x;
     return std::sqrt(s);
   }
-  void print() const {
-    for (int i = 0; i < m_; i++) {
-      for (int j = 0; j < n_; j++) std::cout << (*this)(i, j) << ' ';
-      std::cout << std::endl;
-    }
-  }
   Scalar* data() { return data_.data(); }
   const Scalar* data() const { return data_.data(); }
+  // row interchanges of the last LU() of this matrix, 1-based (LAPACK's ipiv)
+  const IdxVec& ipiv() const { return ipiv_; }
+  IdxVec& ipiv() { return ipiv_; }
 
  private:
   int m_, n_;
   std::vector<Scalar> data_;  // column-major, ld = m_
+  IdxVec ipiv_;
 };
 
 template <class Scalar>
@@ -195,33 +190,141 @@ class Sparse {
   const int* row_ptr_data() const { return row_ptr_.data(); }
   const int* col_ind_data() const { return col_ind_.data(); }
   const Scalar* val_data() const { return val_.data(); }
-  // Printing (sparse.cpp:180-215): triples, the dense matrix, and 1-based "i,j,a," triples to
-  // data.csv for MATLAB.
-  void print() const {
-    std::cout << m_ << " " << n_ << std::endl;
-    for (int i = 0; i < m_; i++)
-      for (int k = row_ptr_[i]; k < row_ptr_[i + 1]; k++)
-        std::cout << "(i,j,a) = (" << i << "," << col_ind_[k] << "," << val_[k] << ")" << std::endl;
-    std::cout << std::flush;
-  }
-  void printDense() const {
-    std::cout << m_ << " " << n_ << std::endl;
-    for (int i = 0; i < m_; i++) {
-      for (int j = 0; j < n_; j++) std::cout << (*this)(i, j) << ' ';
-      std::cout << std::endl;
-    }
-  }
-  void printMatlab() const {
-    std::ofstream f("data.csv");
-    for (int i = 0; i < m_; i++)
-      for (int k = row_ptr_[i]; k < row_ptr_[i + 1]; k++) f << i + 1 << "," << col_ind_[k] + 1 << "," << val_[k] << ",";
-  }
-
  private:
   int m_, n_, nnz_;
   std::vector<Scalar> val_;
   IdxVec col_ind_, row_ptr_;
 };
+
+// ---- free functions of the reference's LinAlg (include/LinAlg/dense.hpp:167-227, :281-310),
+// host-side, Scalar = double.  Inline (the reference defines them non-inline in a header, so two
+// translation units that include it collide).  A numerical failure the reference meets with
+// assert(info == 0) -- an exactly singular pivot -- throws std::runtime_error here.
+
+// AT = A^T, AT resized to n x m (dense.hpp:167-176).
+inline void transpose(Dense<double>& A, Dense<double>& AT) {
+  AT.resize(A.n(), A.m());
+  for (int j = 0; j < A.n(); j++)
+    for (int i = 0; i < A.m(); i++) AT(j, i) = A(i, j);
+}
+
+// In-place LU with partial pivoting, P A = L U (L unit lower, U upper, both stored in A; row
+// interchanges in A.ipiv(), 1-based) -- the reference's LU, LAPACK dgetrf (dense.hpp:180-188).
+// The pivot of column k is the first row of largest |a_ik| at or below the diagonal.
+inline void LU(Dense<double>& A) {
+  const int m = A.m(), n = A.n();
+  if (m != n) throw std::domain_error("LU: matrix is not square");
+  IdxVec& ip = A.ipiv();
+  ip.assign(n, 0);
+  int singular = 0;
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double best = std::fabs(A(k, k));
+    for (int i = k + 1; i < n; i++)
+      if (std::fabs(A(i, k)) > best) {
+        best = std::fabs(A(i, k));
+        p = i;
+      }
+    ip[k] = p + 1;
+    if (p != k)
+      for (int j = 0; j < n; j++) std::swap(A(k, j), A(p, j));
+    const double d = A(k, k);
+    if (d == 0.0) {
+      if (!singular) singular = k + 1;
+      continue;  // LAPACK goes on with the column left as it is
+    }
+    for (int i = k + 1; i < n; i++) A(i, k) /= d;
+    for (int j = k + 1; j < n; j++) {
+      const double u = A(k, j);
+      if (u != 0.0)
+        for (int i = k + 1; i < n; i++) A(i, j) -= A(i, k) * u;
+    }
+  }
+  if (singular) throw std::runtime_error("LU: U(" + std::to_string(singular) + "," + std::to_string(singular) +
+                                         ") is exactly zero (singular matrix)");
+}
+
+namespace detail {
+// B (n x nrhs, column-major, ld) <- A^{-1} B with A = P^T L U from LU().
+inline void lu_solve_cols(const Dense<double>& A, double* b, int ld, int nrhs) {
+  const int n = A.m();
+  const IdxVec& ip = A.ipiv();
+  if ((int)ip.size() != n) throw std::runtime_error("LUSolve: the matrix was not factored by LU()");
+  for (int c = 0; c < nrhs; c++) {
+    double* x = b + (size_t)c * ld;
+    for (int k = 0; k < n; k++)
+      if (ip[k] - 1 != k) std::swap(x[k], x[ip[k] - 1]);
+    for (int j = 0; j < n; j++) {
+      const double v = x[j];
+      if (v != 0.0)
+        for (int i = j + 1; i < n; i++) x[i] -= A(i, j) * v;
+    }
+    for (int j = n - 1; j >= 0; j--) {
+      x[j] /= A(j, j);
+      const double v = x[j];
+      if (v != 0.0)
+        for (int i = 0; i < j; i++) x[i] -= A(i, j) * v;
+    }
+  }
+}
+// B (n x nrhs) <- A^{-T} B.
+inline void lu_solve_cols_t(const Dense<double>& A, double* b, int ld, int nrhs) {
+  const int n = A.m();
+  const IdxVec& ip = A.ipiv();
+  if ((int)ip.size() != n) throw std::runtime_error("LUSolveT: the matrix was not factored by LU()");
+  for (int c = 0; c < nrhs; c++) {
+    double* x = b + (size_t)c * ld;
+    for (int j = 0; j < n; j++) {  // U^T y = b
+      double s = x[j];
+      for (int i = 0; i < j; i++) s -= A(i, j) * x[i];
+      x[j] = s / A(j, j);
+    }
+    for (int j = n - 1; j >= 0; j--) {  // L^T z = y
+      double s = x[j];
+      for (int i = j + 1; i < n; i++) s -= A(i, j) * x[i];
+      x[j] = s;
+    }
+    for (int k = n - 1; k >= 0; k--)
+      if (ip[k] - 1 != k) std::swap(x[k], x[ip[k] - 1]);
+  }
+}
+}  // namespace detail
+
+// B <- A^{-1} B after LU(A) (dgetrs 'N', dense.hpp:190-200).
+inline void LUSolve(Dense<double>& A, Dense<double>& B) {
+  if (A.m() != A.n() || B.m() != A.m()) throw std::domain_error("LUSolve: dimension mismatch");
+  detail::lu_solve_cols(A, B.data(), B.m(), B.n());
+}
+// b <- A^{-1} b after LU(A) (dense.hpp:202-212).
+inline void LUSolveVec(Dense<double>& A, Vec<double>& b) {
+  if (A.m() != A.n() || b.n() != A.m()) throw std::domain_error("LUSolveVec: dimension mismatch");
+  detail::lu_solve_cols(A, b.data(), b.n(), 1);
+}
+// Solves X A = B for X after LU(A), B (m x n) overwritten by X (dgetrs 'T' on B^T,
+// dense.hpp:214-227).
+inline void LUSolveT(Dense<double>& A, Dense<double>& B) {
+  if (A.m() != A.n() || B.n() != A.m()) throw std::domain_error("LUSolveT: dimension mismatch");
+  Dense<double> X;
+  transpose(B, X);
+  detail::lu_solve_cols_t(A, X.data(), X.m(), X.n());
+  transpose(X, B);
+}
+
+// C := alpha op(A) op(B) + beta C, op(X) = X or X^T for trans 'N'/'n' or otherwise; C keeps its
+// size (it must be m x n); nothing happens when m, n or k is 0 (dense.hpp:281-310).
+inline void GEMM(double alpha, const Dense<double>& A, char transa, const Dense<double>& B, char transb, double beta,
+                 Dense<double>& C) {
+  const bool ta = !(transa == 'N' || transa == 'n'), tb = !(transb == 'N' || transb == 'n');
+  const int m = ta ? A.n() : A.m(), k = ta ? A.m() : A.n(), n = tb ? B.m() : B.n();
+  if (m == 0 || n == 0 || k == 0) return;
+  if ((tb ? B.n() : B.m()) != k || C.m() != m || C.n() != n) throw std::domain_error("GEMM: dimension mismatch");
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++) s += (ta ? A(l, i) : A(i, l)) * (tb ? B(j, l) : B(l, j));
+      C(i, j) = (beta == 0.0 ? 0.0 : beta * C(i, j)) + alpha * s;
+    }
+}
 
 }  // namespace LinAlg
 

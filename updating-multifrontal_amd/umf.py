@@ -84,6 +84,7 @@ _sep_block = _sig("umf_get_sep_block", C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip
 _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 _set_prec = _sig("umf_set_precision", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
 _set_graph = _sig("umf_set_graph", C.c_int, [_vp, C.c_int])
+_set_sweep_sync = _sig("umf_set_sweep_sync", C.c_int, [_vp, C.c_int])
 _set_tune = _sig("umf_set_tune", C.c_int, [_vp, C.c_int, C.c_int])
 _get_tune = _sig("umf_get_tune", C.c_int, [_vp, _ip, _ip])
 _autotune = _sig("umf_autotune", C.c_int, [_vp, C.c_int, _dp])
@@ -96,7 +97,7 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision", "umf_set_graph",
-            "umf_set_tune", "umf_get_tune", "umf_autotune",
+            "umf_set_tune", "umf_get_tune", "umf_autotune", "umf_set_sweep_sync",
             "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load"]
 
 
@@ -179,6 +180,12 @@ class _MF:
         """Run Factor() as one captured HIP graph (captured at the next Factor, then replayed;
         bit-identical to the direct launches)."""
         _check(_set_graph(self._h, 1 if on else 0))
+
+    def sweep_sync(self, fenced):
+        """Blocked solve sweeps: publish blocks with agent-scope release / acquire fences
+        (fenced=True) or through write-through atomics alone (False, the default); the results
+        are identical."""
+        _check(_set_sweep_sync(self._h, 1 if fenced else 0))
 
     def set_tune(self, gw2=1024, tile=0):
         """Outer column-group width of the trailing update and GEMM tile size (0: by size)."""
