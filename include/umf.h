@@ -112,6 +112,17 @@ int umf_apply(umf_plan* plan, const double* x, double* y, int nrhs, int ld);
 int umf_solve_device(umf_plan* plan, const double* d_b, double* d_x, int nrhs, int ld);
 int umf_apply_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, int ld);
 
+/* umf_solve_device that allocates nothing (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Its work block, per-front right-hand-side scratch and id list live in
+ * one file-scope device array of 64 MiB (each GPU the library runs on holds its own until the
+ * process ends); it calls no hipMalloc / hipFree.  Limits: FP64 factors only (UMF_ESTATE
+ * otherwise), and plans whose solve fits the array -- about n <= 230k unknowns in 2D -- larger
+ * ones return UMF_EINVAL naming the size needed.  Every call on one GPU shares the array: calls
+ * that overlap on one GPU (other host threads, other plans' streams) race and must be serialized
+ * by the caller.  Results are bit-identical to umf_solve_device. */
+int umf_solve_device_noalloc(umf_plan* plan, const double* d_b, double* d_x, int nrhs, int ld);
+
 /* CSR SpMV y = A x with the plan's current operator, device pointers (the reference's
  * Sparse::operator*(Vec), src/LinAlg/sparse.cpp:63-77), for residuals. */
 int umf_spmv_device(umf_plan* plan, const double* d_x, double* d_y, int nrhs, int ld);

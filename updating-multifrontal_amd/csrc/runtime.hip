@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,6 +105,7 @@ struct umf_plan {
   int bepoch = 0;
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   bool fenced = false;     // blocked sweeps publish with agent-scope release / acquire fences
+  int device = 0;          // HIP device the plan's memory and stream live on
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
   // tuning of the factorization's GEMM stages (umf_set_tune / umf_autotune): the outer column
@@ -748,17 +750,22 @@ int next_epoch(umf_plan* p) {
 // Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the row-major work block w
 // (n x RB), fronts of type T in `arena`, per-front scratch xs.
 template <class T>
-int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs) {
+// ids_dev: device buffer of at least all_lists(P).ids.size() ints for the id list (the no-alloc
+// solve's scratch), or null for the plan's own (grown on demand).
+int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = nullptr) {
   const Plan& P = p->P;
   int L = P.L;
   AllLists A = all_lists(P);
-  int rc = ensure_ids(p, A.ids.size());
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(p->stream));
+  if (!ids_dev) {
+    int rc = ensure_ids(p, A.ids.size());
+    if (rc) return rc;
+    ids_dev = p->d_ids;
+  }
+  HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
   p->ids_h = A.ids;
-  HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+  HIPCHK(hipMemcpyAsync(ids_dev, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
                         p->stream));
-  const int* ids = p->d_ids;
+  const int* ids = ids_dev;
   int n = P.n;
   const FrontDev* fr = p->d_fronts;
   // per level: solve flops nr*(2n^2 + 4nm) split evenly over the two sweeps; bytes = factor read
@@ -1056,6 +1063,98 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   return UMF_OK;
 }
 
+// ---- the solve that allocates nothing (umf_solve_device_noalloc): its work block, per-front
+// right-hand-side scratch and id list live in one file-scope device array.  Each GPU holds its
+// own copy; host code reaches it only through the address hipGetSymbolAddress returns on that
+// device (the array's name in host code is a host-side stand-in, not a device address).
+#define UMF_SCRATCH_BYTES (size_t(64) << 20)
+static __device__ __align__(256) unsigned char scratch[UMF_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};  // per device index
+
+int scratch_ptr(int dev, void** out) {
+  if (dev < 0 || dev >= 64) return fail(UMF_EDEVICE, "device index out of range for the no-alloc scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr)
+      return fail(UMF_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return UMF_OK;
+}
+
+// Restores the plan's per-kernel timing on scope exit (the no-alloc solve runs untimed: timing
+// grows an event pool).
+struct ProfOff {
+  Prof& pr;
+  bool was;
+  explicit ProfOff(Prof& p) : pr(p), was(p.on) { pr.on = false; }
+  ~ProfOff() { pr.on = was; }
+};
+
+int solve_noalloc(umf_plan* p, const double* in, double* out, int nrhs, int ld) {
+  if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  if (!p->factored) return fail(UMF_ESTATE, "factor before solve/apply");
+  if (p->fact_prec != UMF_PREC_FP64)
+    return fail(UMF_ESTATE, "the no-alloc solve takes FP64 factors (the refined forms need work vectors of their own)");
+  const int n = p->P.n;
+  if (nrhs < 0 || ld < n) return fail(UMF_EINVAL, "bad nrhs or ld");
+  if (nrhs == 0) return UMF_OK;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n_ids = (size_t)p->P.nfronts() + (size_t)p->P.nseps_total();
+  const size_t off_x = up(sizeof(double) * (size_t)n * RB);
+  const size_t off_i = up(off_x + sizeof(double) * (size_t)std::max<i64>(p->xs_elems, 1));
+  const size_t need = off_i + sizeof(int) * n_ids;
+  if (need > UMF_SCRATCH_BYTES) {
+    char buf[200];
+    snprintf(buf, sizeof buf, "the no-alloc solve needs %.1f MiB of scratch for this plan (n=%d), above its %zu MiB",
+             need / 1048576.0, n, UMF_SCRATCH_BYTES >> 20);
+    return fail(UMF_EINVAL, buf);
+  }
+  void* base = nullptr;
+  int rc = scratch_ptr(p->device, &base);
+  if (rc) return rc;
+  double* w = (double*)base;
+  double* xs = (double*)((char*)base + off_x);
+  int* ids = (int*)((char*)base + off_i);
+  ProfOff off(p->prof);
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  const unsigned gb = (unsigned)((n + 63) / 64);
+  for (int c0 = 0; c0 < nrhs; c0 += RB) {
+    const int nr = std::min(RB, nrhs - c0);
+    const double* ci = in + (size_t)c0 * ld;
+    double* co = out + (size_t)c0 * ld;
+    launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3(gb), n, ci, ld, nr, w);
+    rc = sweep<double>(p, 0, nr, p->arena, w, xs, ids);
+    if (rc) return rc;
+    launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3(gb), n, (const double*)w, co, ld, nr);
+  }
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  int serr = 0;
+  HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
+  if (serr) {
+    HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
+    return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
+  }
+  p->stats.t_solve_ms = ms;
+  double fl = 0;
+  for (const FrontInfo& F : p->P.fronts) fl += 2.0 * F.n * F.n + 4.0 * F.n * F.m;
+  p->stats.flops_solve = fl * nrhs;
+  p->stats.ir_iters = 0;
+  p->stats.ir_berr = 0.0;
+  return UMF_OK;
+}
+
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
@@ -1199,6 +1298,7 @@ int setup_device(umf_plan* p, umf_plan** out) {
 #define UP(d, h)                                                         \
   if ((he = upload(&(d), h)) != hipSuccess)                              \
     return bail(he == hipErrorOutOfMemory ? UMF_ENOMEM : UMF_EDEVICE, std::string("upload: ") + hipGetErrorString(he));
+  if ((he = hipGetDevice(&p->device)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipStreamCreate(&p->stream)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipEventCreate(&p->ev0)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipEventCreate(&p->ev1)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
@@ -1562,6 +1662,10 @@ int umf_apply(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
 int umf_solve_device(umf_plan* p, const double* b, double* x, int nrhs, int ld) {
   if (!b || !x) return fail(UMF_EINVAL, "null argument");
   return solve_apply(p, 0, b, x, nrhs, ld, true);
+}
+int umf_solve_device_noalloc(umf_plan* p, const double* b, double* x, int nrhs, int ld) {
+  if (!p || !b || !x) return fail(UMF_EINVAL, "null argument");
+  return solve_noalloc(p, b, x, nrhs, ld);
 }
 int umf_apply_device(umf_plan* p, const double* x, double* y, int nrhs, int ld) {
   if (!x || !y) return fail(UMF_EINVAL, "null argument");

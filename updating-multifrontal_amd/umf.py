@@ -65,6 +65,7 @@ _prof_read = _sig("umf_profile_read", C.c_int, [_vp, C.c_int, C.c_char_p, C.c_in
 _solve = _sig("umf_solve", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
 _apply = _sig("umf_apply", C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int])
 _solve_dev = _sig("umf_solve_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
+_solve_noalloc = _sig("umf_solve_device_noalloc", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _apply_dev = _sig("umf_apply_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _spmv_dev = _sig("umf_spmv_device", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int])
 _save = _sig("umf_plan_save", C.c_int, [_vp, C.c_char_p])
@@ -94,7 +95,7 @@ _batch_info = _sig("umf_batch_info", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip]
 
 EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
             "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
-            "umf_solve_device", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
+            "umf_solve_device", "umf_solve_device_noalloc", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision", "umf_set_graph",
             "umf_set_tune", "umf_get_tune", "umf_autotune", "umf_set_sweep_sync",
@@ -236,6 +237,11 @@ class _MF:
 
     def solve_device(self, d_b, d_x, nrhs, ld=None):
         _check(_solve_dev(self._h, d_b, d_x, nrhs, ld or self.n))
+
+    def solve_device_noalloc(self, d_b, d_x, nrhs, ld=None):
+        """solve_device through the entry point that allocates nothing (FP64 factors; the plan's
+        solve must fit the library's 64 MiB per-GPU scratch array)."""
+        _check(_solve_noalloc(self._h, d_b, d_x, nrhs, ld or self.n))
 
     def apply_device(self, d_x, d_y, nrhs, ld=None):
         _check(_apply_dev(self._h, d_x, d_y, nrhs, ld or self.n))
