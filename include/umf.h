@@ -28,6 +28,7 @@ extern "C" {
 #define UMF_EUPDATEMAP (-6) /* a modified DOF maps outside the leaf box grid */
 #define UMF_ESTATE (-7)     /* call out of order (e.g. solve before factor) */
 #define UMF_ENOCONV (-8)    /* FP32 form: refinement did not reach the requested backward error */
+#define UMF_EBUSY (-9)      /* serving: the slot pool has no free range for the request now */
 
 typedef struct umf_plan umf_plan;
 
@@ -250,6 +251,44 @@ int umf_get_tune(const umf_plan* plan, int* gw2, int* tile);
  * (gw2 in {1024, 2048} x tile in {0, 64, 128}), `reps` times each, keeps the fastest by the
  * factorization's event time (*best_ms, may be NULL) and refactors with it. */
 int umf_autotune(umf_plan* plan, int reps, double* best_ms);
+
+/* ---- serving (no reference counterpart; the reference's use case is Update after a local
+ * coefficient change followed by Solve, test_src/DE/testMF3D.cpp:137-215, on one object).
+ * A server answers requests against an immutable factored base plan: request r = the base
+ * operator with a local change (its values, its modified DOFs by the reference's rule) and
+ * right-hand sides; its answer is A_r^{-1} b -- what Update(A_r) + Solve on the base would give,
+ * bit for bit -- while the base stays unchanged and requests never see each other's changes.
+ * A request runs in a slot (its own stream, descriptors, operator values and scratch); its dirty
+ * fronts and separators (the reference's Update recurrence) are factored into a range of the
+ * server's slot pool, every other front is read from the base.  Its footprint is the bytes of
+ * those fronts (with their inverted diagonal blocks) and separators.  Slots run concurrently:
+ * distinct slots may be driven by distinct host threads; one slot by one thread at a time.
+ * While a server lives, umf_factor / umf_update / umf_set_precision / umf_autotune /
+ * umf_plan_destroy on its base return UMF_ESTATE. */
+typedef struct umf_server umf_server;
+/* slots: 1..32 execution contexts; pool_bytes: device memory for the requests' dirty fronts. */
+int umf_server_create(umf_plan* base, int slots, double pool_bytes, umf_server** server);
+int umf_server_destroy(umf_server* server);
+/* Slot-pool bytes a request with these modified DOFs needs. */
+int umf_server_footprint(umf_server* server, int n_mod, const int* mod_dofs, double* bytes);
+/* Admission: reserves the request's footprint in the pool (first fit) and returns a ticket;
+ * UMF_EBUSY when no free range holds it now (retry after a release), UMF_EINVAL when the whole
+ * pool could not. */
+int umf_server_admit(umf_server* server, int n_mod, const int* mod_dofs, int* ticket);
+int umf_server_release(umf_server* server, int ticket);
+int umf_server_pool(umf_server* server, double* total_bytes, double* used_bytes);
+/* The request's operator in slot `slot`: full CSR values (device pointer, the plan's pattern),
+ * or a coefficient box change on the base operator, as umf_box_scale_device (the base field
+ * d_coeff is read, not changed; the modified DOFs come back in mod_dofs). */
+int umf_server_set_values(umf_server* server, int slot, const double* d_val);
+int umf_server_box_scale(umf_server* server, int slot, const double* d_coeff, double h, const int* lo, int side,
+                         double factor, int* mod_dofs, int cap, int* n_mod);
+/* Runs an admitted request in `slot` with the operator set there: factors its dirty fronts into
+ * its pool range, then solves nrhs columns (device pointers, as umf_solve_device; the refined
+ * forms refine against the request's operator).  Synchronous on the slot's stream. */
+int umf_server_run(umf_server* server, int slot, int ticket, const double* d_b, double* d_x, int nrhs, int ld);
+/* Timing and refinement statistics of the slot's last request (t_update_ms = its factor pass). */
+int umf_server_stats(const umf_server* server, int slot, umf_stats* out);
 
 #ifdef __cplusplus
 }

@@ -214,22 +214,26 @@ def test_batch_errors_name_the_problem():
         umf.Batch([(A, d, 1, 3)], analyze_only=True).box_dofs(0, 0, 0)
 
 
-@pytest.mark.parametrize("dims", [(21, 18, 19), (40, 37)])
-def test_serve_box_change_matches_rebuild(dims):
-    """A request's windowed operator change equals a full rebuild of the changed field (bit
-    for bit) and its modified DOFs follow the reference's modDOF rule."""
+@pytest.mark.parametrize("dims,w", [((21, 18, 19), 1), ((40, 37), 1), ((30, 27), 2), ((14, 13, 12), 3)])
+def test_serve_box_mods_match_rebuild(dims, w):
+    """The serving loop's host-side modified DOFs of a box change (used for admission before a
+    slot runs it) equal the reference's modDOF rule applied to a full rebuild of the changed
+    field, for every stencil width, boxes at the corners, the far corner and inside, and leave
+    the base field and operator untouched."""
     import umf
-    from serve import box_change
-    rng = np.random.default_rng(7)
+    from serve import box_mods
+    rng = np.random.default_rng(7 + w)
     h = 1.0 / (dims[0] + 1)
-    co = np.exp(rng.standard_normal(tuple(n + 2 for n in dims)))
-    rp, ci, v = umf.build_laplacian(dims, co, 1, h)
-    vals = v.copy()
-    cf = co.copy()
-    for side, lo in ((4, [0] * len(dims)), (6, [d + 2 - 6 for d in dims]), (5, [3] * len(dims))):
-        before = vals.copy()
-        pos, val, mod = box_change(dims, cf, vals, lo, side, 7.5, rp, ci, h)
-        _, _, full = umf.build_laplacian(dims, cf, 1, h)
-        assert np.array_equal(vals, full)
-        assert np.array_equal(np.sort(mod), np.unique(ci[before != full]))
-        assert np.array_equal(np.sort(pos), np.nonzero(before != full)[0])
+    co = np.exp(rng.standard_normal(tuple(n + 2 * w for n in dims)))
+    rp, ci, v = umf.build_laplacian(dims, co, w, h)
+    co0, v0 = co.copy(), v.copy()
+    boxes = [(4, [0] * len(dims)), (6, [d + 2 * w - 6 for d in dims]), (5, [3] * len(dims)), (1, [w] * len(dims))]
+    boxes += [(int(s), [int(rng.integers(0, d + 2 * w - s + 1)) for d in dims]) for s in rng.integers(1, 9, size=6)]
+    for side, lo in boxes:
+        f = float(np.exp(rng.standard_normal()))
+        c2 = co.copy()
+        c2[tuple(slice(l, l + side) for l in lo)] *= f
+        _, _, full = umf.build_laplacian(dims, c2, w, h)
+        mod = box_mods(dims, co, v, lo, side, f, rp, ci, h, w)
+        assert np.array_equal(mod, np.unique(ci[v != full])), (side, lo)
+    assert np.array_equal(co, co0) and np.array_equal(v, v0)

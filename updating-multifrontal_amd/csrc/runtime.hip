@@ -11,6 +11,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -72,7 +74,17 @@ struct Prof {
 };
 
 struct umf_plan {
-  Plan P;
+  // the planner's output; serving slots share their base plan's (umf_server_create)
+  std::shared_ptr<Plan> Pp = std::make_shared<Plan>();
+  Plan& P = *Pp;
+  umf_plan() = default;
+  explicit umf_plan(std::shared_ptr<Plan> shared) : Pp(std::move(shared)) {}
+  umf_plan(const umf_plan&) = delete;
+  umf_plan& operator=(const umf_plan&) = delete;
+  // a serving slot: its device front pool, programs and CSR pattern are the base plan's
+  // (borrowed, not freed with the slot); descriptors, operator values, stream and scratch are its own
+  bool is_slot = false;
+  int servers = 0;  // live servers answering requests against this plan's factors (it stays immutable)
   int args[6] = {0, 0, 0, 0, 0, 0};  // dim, nx, ny, nz, width, max_side_length it was made with
   Prof prof;
   hipStream_t stream = nullptr;
@@ -139,6 +151,7 @@ struct umf_plan {
   i64 xs_elems = 0;
   std::vector<FrontDev> fronts_h;
   std::vector<SepDev> seps_h;
+  std::vector<MatDev> mats_h;
   std::vector<double> val_h;  // current operator values
   bool factored = false;
   bool host_only = false;  // analyzed plan without device memory
@@ -680,7 +693,10 @@ int read_pool(umf_plan* p, i64 off, size_t cnt, std::vector<double>& h) {
 
 int check_err(umf_plan* p) {
   int h[4];
-  HIPCHK(hipMemcpy(h, p->d_err, sizeof(h), hipMemcpyDeviceToHost));
+  // stream-ordered (a serving slot's stream must not wait on other slots, as a null-stream
+  // copy would)
+  HIPCHK(hipMemcpyAsync(h, p->d_err, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   if (h[0]) {
     const FrontInfo& F = p->P.fronts[h[1]];
     char buf[256];
@@ -1041,9 +1057,11 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
   int serr = 0;
-  HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   if (serr) {
-    HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
+    HIPCHK(hipMemsetAsync(p->d_err + 3, 0, sizeof(int), p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
   }
   if (mode == 0) {
@@ -1141,9 +1159,11 @@ int solve_noalloc(umf_plan* p, const double* in, double* out, int nrhs, int ld) 
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
   int serr = 0;
-  HIPCHK(hipMemcpy(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&serr, p->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   if (serr) {
-    HIPCHK(hipMemset(p->d_err + 3, 0, sizeof(int)));
+    HIPCHK(hipMemsetAsync(p->d_err + 3, 0, sizeof(int), p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return fail(UMF_EDEVICE, "blocked solve sweep: a block wait exceeded its spin limit (results invalid)");
   }
   p->stats.t_solve_ms = ms;
@@ -1251,7 +1271,8 @@ namespace {
 int setup_device(umf_plan* p, umf_plan** out) {
   Plan& P = p->P;
   // device descriptors
-  std::vector<MatDev> mats;
+  std::vector<MatDev>& mats = p->mats_h;
+  mats.clear();
   for (const MatDesc& M : P.mats) mats.push_back(MatDev{M.off, M.rows, M.cols});
   i64 xo = 0;
   for (const FrontInfo& F : P.fronts) {
@@ -1445,6 +1466,7 @@ int umf_analyze_update(umf_plan* p, int n_mod, const int* mod_dofs) {
 
 int umf_plan_destroy(umf_plan* p) {
   if (!p) return UMF_OK;
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server still answers requests against this plan");
   free_plan(p);
   delete p;
   return UMF_OK;
@@ -1568,6 +1590,7 @@ int umf_plan_load(const char* path, umf_plan** out) {
 int umf_factor(umf_plan* p) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
   Plan::Dirty D = p->P.all_dirty();
   p->factored = false;
   p->fact_prec = p->prec;
@@ -1592,6 +1615,7 @@ namespace {
 // Shared tail of the update entry points: d_val already holds (or is about to receive, in
 // stream order) the new values.
 int do_update(umf_plan* p, const double* src_val, bool src_device, int n_mod, const int* mod_dofs) {
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
   const Plan& P = p->P;
   Plan::Dirty D;
   std::string e;
@@ -1911,6 +1935,7 @@ int umf_get_stats(const umf_plan* p, umf_stats* out) {
 
 int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
   if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3)
     return fail(UMF_EINVAL, "unknown precision");
   if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
@@ -1944,6 +1969,7 @@ int umf_get_tune(const umf_plan* p, int* gw2, int* tile) {
 
 int umf_autotune(umf_plan* p, int reps, double* best_ms) {
   if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
   reps = std::max(reps, 1);
   const bool graph = p->use_graph, prof = p->prof.on;
@@ -2028,6 +2054,431 @@ int umf_profile_read(const umf_plan* p, int i, char* name, int name_len, double*
     if (bytes) *bytes = st[i].bytes;
   }
   return (int)st.size();
+}
+
+}  // extern "C"
+
+// ============================================================================== serving
+// A server answers requests against an immutable base factorization.  Request r is a local
+// change of the operator (the base operator with some values changed, and the modified DOFs of
+// the reference's modDOF rule) plus right-hand sides; its answer is x = A_r^{-1} b, as if the
+// base plan were Updated with A_r (the reference's use case, test_src/DE/testMF3D.cpp:137-215)
+// -- but the base stays untouched and requests do not see each other's changes.
+//
+// A request runs in a slot: an execution context with its own stream, descriptors, operator
+// values, error flags and solve scratch, sharing the base plan's planner output, programs and
+// front pool.  Its dirty fronts and separators (the reference's Update recurrence) are factored
+// into memory reserved for the request in the server's slot pool; the slot's descriptors point
+// there for those, and at the base's factors for every other front.  The footprint of a request
+// is the bytes of its dirty fronts (with their inverted diagonal blocks) and separators; a
+// request is admitted when the pool has a free range that holds it.  Slots on their own streams
+// run concurrently; each request's arithmetic depends on its own dirty set only (the schedule's
+// algorithm choices are fixed per plan level), so its result is the same alone or among others,
+// and the same as a fresh factorization of its operator.
+struct umf_server {
+  umf_plan* base = nullptr;
+  std::vector<umf_plan*> slots;
+  std::vector<double*> coeff;       // per slot: the request's coefficient field (box requests)
+  std::vector<size_t> coeff_elems;
+  std::vector<char> busy;           // per slot: a request is running in it
+  char* pool = nullptr;
+  size_t pool_bytes = 0, used = 0;
+  int elem = 8;                     // bytes per front entry (8 FP64 factors, 4 FP32)
+  std::mutex mu;
+  std::map<size_t, size_t> free_;   // free byte ranges of the pool, start -> end
+  struct Ticket {
+    Plan::Dirty D;
+    size_t off = 0, bytes = 0;
+    std::vector<std::pair<int, i64>> fr, dinv, sep;  // id -> element offset inside the range
+    bool running = false;
+  };
+  std::map<int, Ticket> tickets;
+  int next_ticket = 1;
+};
+
+namespace {
+// Layout of a request's dirty matrices inside one pool range (256-byte aligned pieces).
+size_t ticket_layout(const Plan& P, const Plan::Dirty& D, int elem, umf_server::Ticket& T) {
+  size_t b = 0;
+  auto put = [&](i64 elems) {
+    const size_t at = b;
+    b += ((size_t)elems * elem + 255) / 256 * 256;
+    return (i64)(at / elem);
+  };
+  T.fr.clear();
+  T.dinv.clear();
+  T.sep.clear();
+  for (int ell = 0; ell < P.L && ell < (int)D.boxes.size(); ell++)
+    for (int bx : D.boxes[ell]) {
+      const int f = P.box_gid[ell][bx];
+      const FrontInfo& F = P.fronts[f];
+      const i64 N = (i64)F.n + F.m;
+      T.fr.push_back({f, put(N * N)});
+      T.dinv.push_back({f, put((i64)((F.n + NB - 1) / NB) * NB * NB)});
+    }
+  for (int ell = 1; ell < P.L && ell < (int)D.seps.size(); ell++)
+    for (int e : D.seps[ell]) {
+      const int sid = P.sep_gid[ell][e];
+      T.sep.push_back({sid, put((i64)P.seps[sid].k * P.seps[sid].k)});
+    }
+  return b;
+}
+
+int check_slot(umf_server* s, int slot) {
+  if (!s || !s->base) return fail(UMF_EINVAL, "null server");
+  if (slot < 0 || slot >= (int)s->slots.size()) return fail(UMF_EINVAL, "bad slot index");
+  return UMF_OK;
+}
+
+// Frees a slot's own device memory (its borrowed pointers are cleared first).
+void free_slot(umf_plan* q) {
+  q->arena = nullptr;
+  q->arena32 = nullptr;
+  q->d_dofs = nullptr;
+  q->d_tri = nullptr;
+  q->d_tri_off = nullptr;
+  q->d_ops = nullptr;
+  q->d_tiles = nullptr;
+  q->d_tile_off = nullptr;
+  q->d_rp = nullptr;
+  q->d_ci = nullptr;
+  q->d_fly_tab = nullptr;
+  q->d_fly_tri = nullptr;
+  q->d_rest_tri = nullptr;
+  q->d_rest_off = nullptr;
+  free_plan(q);
+  delete q;
+}
+
+// A slot context over base plan b: everything a factor pass and a solve touch, with the
+// plan's data borrowed and the per-call buffers its own, allocated here once.
+int make_slot(umf_plan* b, umf_plan** out) {
+  umf_plan* q = new umf_plan(b->Pp);
+  q->is_slot = true;
+  std::copy(b->args, b->args + 6, q->args);
+  q->prec = q->fact_prec = b->fact_prec;
+  q->ir_tol = b->ir_tol;
+  q->ir_max = b->ir_max;
+  q->tune_gw2 = b->tune_gw2;
+  q->tune_tile = b->tune_tile;
+  q->fenced = b->fenced;
+  q->device = b->device;
+  q->xs_elems = b->xs_elems;
+  q->n_flags = b->n_flags;
+  q->max_tiles_per_mat = b->max_tiles_per_mat;
+  q->fronts_h = b->fronts_h;
+  q->seps_h = b->seps_h;
+  q->mats_h = b->mats_h;
+  q->stats = b->stats;
+  q->arena = b->arena;
+  q->arena32 = b->arena32;
+  q->d_dofs = b->d_dofs;
+  q->d_tri = b->d_tri;
+  q->d_tri_off = b->d_tri_off;
+  q->d_ops = b->d_ops;
+  q->d_tiles = b->d_tiles;
+  q->d_tile_off = b->d_tile_off;
+  q->d_rp = b->d_rp;
+  q->d_ci = b->d_ci;
+  q->d_fly_tab = b->d_fly_tab;
+  q->d_fly_tri = b->d_fly_tri;
+  q->d_rest_tri = b->d_rest_tri;
+  q->d_rest_off = b->d_rest_off;
+  const Plan& P = *b->Pp;
+  const size_t nnz = P.ci.size(), NF = P.nfronts(), NS = P.nseps_total();
+  auto bail = [&](hipError_t e) {
+    free_slot(q);
+    return fail(e == hipErrorOutOfMemory ? UMF_ENOMEM : UMF_EDEVICE,
+                std::string("serving slot: ") + hipGetErrorString(e));
+  };
+  hipError_t e;
+#define SLOTCHK(x) if ((e = (x)) != hipSuccess) return bail(e)
+  // non-blocking: a slot's work never waits on the null stream's, so slots overlap
+  SLOTCHK(hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking));
+  SLOTCHK(hipEventCreate(&q->ev0));
+  SLOTCHK(hipEventCreate(&q->ev1));
+  SLOTCHK(hipMalloc((void**)&q->d_mats, sizeof(MatDev) * std::max<size_t>(q->mats_h.size(), 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_fronts, sizeof(FrontDev) * std::max<size_t>(NF, 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_seps, sizeof(SepDev) * std::max<size_t>(NS, 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_val, sizeof(double) * std::max<size_t>(nnz, 1)));
+  SLOTCHK(hipMemcpy(q->d_val, b->d_val, sizeof(double) * nnz, hipMemcpyDeviceToDevice));
+  SLOTCHK(hipMalloc((void**)&q->d_err, sizeof(int) * 4));
+  SLOTCHK(hipMemset(q->d_err, 0, sizeof(int) * 4));
+  SLOTCHK(hipMalloc((void**)&q->d_flags, sizeof(int) * std::max(q->n_flags, 1)));
+  SLOTCHK(hipMemset(q->d_flags, 0, sizeof(int) * std::max(q->n_flags, 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_arrive, sizeof(int) * std::max<size_t>(NF, 1)));
+  SLOTCHK(hipMemset(q->d_arrive, 0, sizeof(int) * std::max<size_t>(NF, 1)));
+  q->ids_cap = NF + 2 * NS;
+  SLOTCHK(hipMalloc((void**)&q->d_ids, sizeof(int) * std::max<size_t>(q->ids_cap, 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_bflag, sizeof(int) * std::max(P.n, 1)));
+  SLOTCHK(hipMemset(q->d_bflag, 0, sizeof(int) * std::max(P.n, 1)));
+  q->blist_cap = P.n;
+  SLOTCHK(hipMalloc((void**)&q->d_blist, sizeof(int) * (P.n + 1)));
+#undef SLOTCHK
+  q->factored = true;
+  const int rc = ensure_solve_scratch(q);
+  if (rc) {
+    free_slot(q);
+    return rc;
+  }
+  *out = q;
+  return UMF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int umf_server_create(umf_plan* base, int nslots, double pool_bytes, umf_server** out) {
+  if (!base || !out) return fail(UMF_EINVAL, "null argument");
+  *out = nullptr;
+  if (base->host_only || !base->factored) return fail(UMF_ESTATE, "the base plan must be factored");
+  if (base->P.forest) return fail(UMF_EINVAL, "serving takes a single-problem plan");
+  if (nslots < 1 || nslots > 32) return fail(UMF_EINVAL, "slots must be 1..32");
+  if (!(pool_bytes >= 0)) return fail(UMF_EINVAL, "bad pool size");
+  HIPCHK(hipStreamSynchronize(base->stream));
+  umf_server* s = new umf_server();
+  s->elem = base->fact_prec == UMF_PREC_FP64 ? 8 : 4;
+  s->pool_bytes = (size_t)pool_bytes / 256 * 256;
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != base->device) HIPCHK(hipSetDevice(base->device));
+  int rc = UMF_OK;
+  if (s->pool_bytes) {
+    const hipError_t e = hipMalloc((void**)&s->pool, s->pool_bytes);
+    if (e != hipSuccess)
+      rc = fail(UMF_ENOMEM, "slot pool of " + std::to_string(pool_bytes / 1e9) + " GB: " + hipGetErrorString(e));
+  }
+  for (int i = 0; rc == UMF_OK && i < nslots; i++) {
+    umf_plan* q = nullptr;
+    rc = make_slot(base, &q);
+    if (rc == UMF_OK) s->slots.push_back(q);
+  }
+  if (cur != base->device) (void)hipSetDevice(cur);
+  if (rc) {
+    for (umf_plan* q : s->slots) free_slot(q);
+    if (s->pool) (void)hipFree(s->pool);
+    delete s;
+    return rc;
+  }
+  s->base = base;
+  s->coeff.assign(nslots, nullptr);
+  s->coeff_elems.assign(nslots, 0);
+  s->busy.assign(nslots, 0);
+  if (s->pool_bytes) s->free_[0] = s->pool_bytes;
+  base->servers++;
+  *out = s;
+  return UMF_OK;
+}
+
+int umf_server_destroy(umf_server* s) {
+  if (!s) return UMF_OK;
+  for (umf_plan* q : s->slots) {
+    if (q->stream) (void)hipStreamSynchronize(q->stream);
+    free_slot(q);
+  }
+  for (double* c : s->coeff)
+    if (c) (void)hipFree(c);
+  if (s->pool) (void)hipFree(s->pool);
+  if (s->base) s->base->servers--;
+  delete s;
+  return UMF_OK;
+}
+
+int umf_server_footprint(umf_server* s, int n_mod, const int* mod_dofs, double* bytes) {
+  if (!s || !s->base || !bytes || (n_mod > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  const Plan& P = s->base->P;
+  std::vector<int> mod(mod_dofs, mod_dofs + std::max(n_mod, 0));
+  Plan::Dirty D;
+  std::string e;
+  if (!P.dirty_from_dofs(mod, D, e)) return fail(UMF_EUPDATEMAP, e);
+  umf_server::Ticket T;
+  *bytes = (double)ticket_layout(P, D, s->elem, T);
+  return UMF_OK;
+}
+
+int umf_server_admit(umf_server* s, int n_mod, const int* mod_dofs, int* ticket) {
+  if (!s || !s->base || !ticket || (n_mod > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  const Plan& P = s->base->P;
+  std::vector<int> mod(mod_dofs, mod_dofs + std::max(n_mod, 0));
+  umf_server::Ticket T;
+  std::string e;
+  if (!P.dirty_from_dofs(mod, T.D, e)) return fail(UMF_EUPDATEMAP, e);
+  T.bytes = ticket_layout(P, T.D, s->elem, T);
+  if (T.bytes > s->pool_bytes) {
+    char buf[200];
+    snprintf(buf, sizeof buf, "request needs %.3f GB of slot pool, the pool holds %.3f GB", T.bytes / 1e9,
+             s->pool_bytes / 1e9);
+    return fail(UMF_EINVAL, buf);
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (T.bytes > 0) {
+    auto it = s->free_.begin();
+    for (; it != s->free_.end(); ++it)
+      if (it->second - it->first >= T.bytes) break;
+    if (it == s->free_.end()) return fail(UMF_EBUSY, "slot pool: no free range holds the request now");
+    T.off = it->first;
+    const size_t end = it->second;
+    s->free_.erase(it);
+    if (T.off + T.bytes < end) s->free_[T.off + T.bytes] = end;
+    s->used += T.bytes;
+  }
+  *ticket = s->next_ticket++;
+  s->tickets[*ticket] = std::move(T);
+  return UMF_OK;
+}
+
+int umf_server_release(umf_server* s, int ticket) {
+  if (!s || !s->base) return fail(UMF_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->tickets.find(ticket);
+  if (it == s->tickets.end()) return fail(UMF_EINVAL, "unknown ticket");
+  if (it->second.running) return fail(UMF_ESTATE, "the ticket's request is still running");
+  const size_t a = it->second.off, bb = a + it->second.bytes;
+  if (it->second.bytes > 0) {
+    size_t lo = a, hi = bb;
+    auto nx = s->free_.lower_bound(a);  // merge with the neighbouring free ranges
+    if (nx != s->free_.end() && nx->first == bb) {
+      hi = nx->second;
+      s->free_.erase(nx);
+    }
+    auto pv = s->free_.lower_bound(a);
+    if (pv != s->free_.begin()) {
+      --pv;
+      if (pv->second == a) {
+        lo = pv->first;
+        s->free_.erase(pv);
+      }
+    }
+    s->free_[lo] = hi;
+    s->used -= it->second.bytes;
+  }
+  s->tickets.erase(it);
+  return UMF_OK;
+}
+
+int umf_server_pool(umf_server* s, double* total, double* used) {
+  if (!s || !s->base) return fail(UMF_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (total) *total = (double)s->pool_bytes;
+  if (used) *used = (double)s->used;
+  return UMF_OK;
+}
+
+int umf_server_set_values(umf_server* s, int slot, const double* d_val) {
+  int rc = check_slot(s, slot);
+  if (rc) return rc;
+  if (!d_val) return fail(UMF_EINVAL, "null argument");
+  umf_plan* q = s->slots[slot];
+  HIPCHK(hipMemcpyAsync(q->d_val, d_val, sizeof(double) * q->P.ci.size(), hipMemcpyDeviceToDevice, q->stream));
+  HIPCHK(hipStreamSynchronize(q->stream));
+  return UMF_OK;
+}
+
+int umf_server_box_scale(umf_server* s, int slot, const double* d_coeff, double h, const int* lo, int side,
+                         double factor, int* mod_dofs, int cap, int* n_mod) {
+  int rc = check_slot(s, slot);
+  if (rc) return rc;
+  if (!d_coeff || !lo || !n_mod || (cap > 0 && !mod_dofs)) return fail(UMF_EINVAL, "null argument");
+  umf_plan* q = s->slots[slot];
+  const Grid& g = q->P.g;
+  size_t ce = 1;
+  for (int a = 0; a < g.dim; a++) ce *= (size_t)(g.N[a] + 2 * g.w);
+  if (s->coeff_elems[slot] < ce) {
+    if (s->coeff[slot]) HIPCHK(hipFree(s->coeff[slot]));
+    s->coeff[slot] = nullptr;
+    HIPCHK(hipMalloc((void**)&s->coeff[slot], sizeof(double) * ce));
+    s->coeff_elems[slot] = ce;
+  }
+  // the request's field and operator start from the base's
+  HIPCHK(hipMemcpyAsync(s->coeff[slot], d_coeff, sizeof(double) * ce, hipMemcpyDeviceToDevice, q->stream));
+  HIPCHK(hipMemcpyAsync(q->d_val, s->base->d_val, sizeof(double) * q->P.ci.size(), hipMemcpyDeviceToDevice,
+                        q->stream));
+  return umf_box_scale_device(q, s->coeff[slot], h, lo, side, factor, q->d_val, mod_dofs, cap, n_mod);
+}
+
+int umf_server_run(umf_server* s, int slot, int ticket, const double* d_b, double* d_x, int nrhs, int ld) {
+  int rc = check_slot(s, slot);
+  if (rc) return rc;
+  if (!d_b || !d_x) return fail(UMF_EINVAL, "null argument");
+  umf_plan* q = s->slots[slot];
+  const Plan& P = q->P;
+  umf_server::Ticket* T = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(UMF_EINVAL, "unknown ticket");
+    if (it->second.running) return fail(UMF_ESTATE, "the ticket's request is already running");
+    if (s->busy[slot]) return fail(UMF_ESTATE, "the slot is running another request");
+    it->second.running = true;
+    s->busy[slot] = 1;
+    T = &it->second;  // map nodes stay put while other tickets come and go
+  }
+  struct Done {
+    umf_server* s;
+    umf_server::Ticket* T;
+    int slot;
+    ~Done() {
+      std::lock_guard<std::mutex> lk(s->mu);
+      T->running = false;
+      s->busy[slot] = 0;
+    }
+  } done{s, T, slot};
+  // descriptors: the request's dirty matrices in its pool range, every other one the base's
+  q->mats_h = s->base->mats_h;
+  q->fronts_h = s->base->fronts_h;
+  q->seps_h = s->base->seps_h;
+  const char* abase = s->elem == 8 ? (const char*)s->base->arena : (const char*)s->base->arena32;
+  const i64 shift = (i64)(((intptr_t)(s->pool + T->off) - (intptr_t)abase) / s->elem);
+  const int NF = P.nfronts();
+  for (auto& fo : T->fr) {
+    q->fronts_h[fo.first].off = shift + fo.second;
+    q->mats_h[fo.first].off = shift + fo.second;
+  }
+  for (auto& fo : T->dinv) q->fronts_h[fo.first].dinv_off = shift + fo.second;
+  for (auto& so : T->sep) {
+    q->seps_h[so.first].off = shift + so.second;
+    q->mats_h[NF + so.first].off = shift + so.second;
+  }
+  HIPCHK(hipMemcpyAsync(q->d_mats, q->mats_h.data(), sizeof(MatDev) * q->mats_h.size(), hipMemcpyHostToDevice,
+                        q->stream));
+  HIPCHK(hipMemcpyAsync(q->d_fronts, q->fronts_h.data(), sizeof(FrontDev) * q->fronts_h.size(),
+                        hipMemcpyHostToDevice, q->stream));
+  if (!q->seps_h.empty())
+    HIPCHK(hipMemcpyAsync(q->d_seps, q->seps_h.data(), sizeof(SepDev) * q->seps_h.size(), hipMemcpyHostToDevice,
+                          q->stream));
+  // factor the request's dirty fronts (the reference's Update over its operator)
+  HIPCHK(hipEventRecord(q->ev0, q->stream));
+  double fl = 0;
+  int nf = 0;
+  if (T->D.any) {
+    rc = factor_any(q, T->D);
+    if (rc) return rc;
+    for (int ell = 0; ell < P.L; ell++)
+      for (int bx : T->D.boxes[ell]) {
+        fl += P.front_flops(P.box_gid[ell][bx]);
+        nf++;
+      }
+  }
+  HIPCHK(hipEventRecord(q->ev1, q->stream));
+  HIPCHK(hipStreamSynchronize(q->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, q->ev0, q->ev1));
+  q->stats.t_update_ms = ms;
+  q->stats.flops_update = fl;
+  q->stats.n_fronts_update = nf;
+  q->last = T->D;
+  rc = check_err(q);
+  if (rc) return rc;
+  q->factored = true;
+  return solve_apply(q, 0, d_b, d_x, nrhs, ld, true);
+}
+
+int umf_server_stats(const umf_server* s, int slot, umf_stats* out) {
+  if (!s || !s->base || !out) return fail(UMF_EINVAL, "null argument");
+  if (slot < 0 || slot >= (int)s->slots.size()) return fail(UMF_EINVAL, "bad slot index");
+  *out = s->slots[slot]->stats;
+  out->bytes_front_pool = (double)s->pool_bytes;
+  return UMF_OK;
 }
 
 }  // extern "C"

@@ -23,7 +23,8 @@ if not os.path.exists(LIB_PATH):
 _lib = C.CDLL(LIB_PATH)
 
 UMF_OK = 0
-UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE, UMF_ENOCONV = range(-1, -9, -1)
+UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE, UMF_ENOCONV, UMF_EBUSY = \
+    range(-1, -10, -1)
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
@@ -92,6 +93,17 @@ _autotune = _sig("umf_autotune", C.c_int, [_vp, C.c_int, _dp])
 _batch_create = _sig("umf_batch_create", C.c_int, [C.c_int, _ip, C.POINTER(_ip), C.POINTER(_ip), C.POINTER(_dp),
                                                    C.c_int, C.POINTER(_vp)])
 _batch_info = _sig("umf_batch_info", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip])
+_srv_create = _sig("umf_server_create", C.c_int, [_vp, C.c_int, C.c_double, C.POINTER(_vp)])
+_srv_destroy = _sig("umf_server_destroy", C.c_int, [_vp])
+_srv_footprint = _sig("umf_server_footprint", C.c_int, [_vp, C.c_int, _ip, _dp])
+_srv_admit = _sig("umf_server_admit", C.c_int, [_vp, C.c_int, _ip, _ip])
+_srv_release = _sig("umf_server_release", C.c_int, [_vp, C.c_int])
+_srv_pool = _sig("umf_server_pool", C.c_int, [_vp, _dp, _dp])
+_srv_set_values = _sig("umf_server_set_values", C.c_int, [_vp, C.c_int, _vp])
+_srv_box_scale = _sig("umf_server_box_scale", C.c_int, [_vp, C.c_int, _vp, C.c_double, _ip, C.c_int, C.c_double, _ip,
+                                                        C.c_int, _ip])
+_srv_run = _sig("umf_server_run", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int])
+_srv_stats = _sig("umf_server_stats", C.c_int, [_vp, C.c_int, C.c_void_p])
 
 EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_plan_destroy", "umf_factor", "umf_update",
             "umf_update_device", "umf_profile_enable", "umf_profile_select", "umf_profile_reset", "umf_profile_read", "umf_solve", "umf_apply",
@@ -99,7 +111,10 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision", "umf_set_graph",
             "umf_set_tune", "umf_get_tune", "umf_autotune", "umf_set_sweep_sync",
-            "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load"]
+            "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load",
+            "umf_server_create", "umf_server_destroy", "umf_server_footprint", "umf_server_admit",
+            "umf_server_release", "umf_server_pool", "umf_server_set_values", "umf_server_box_scale",
+            "umf_server_run", "umf_server_stats"]
 
 
 def _i(a):
@@ -526,3 +541,81 @@ class Batch(_MF):
     def split(self, X):
         """The per-problem row blocks of a stacked n_total (x nrhs) array."""
         return [X[o:o + n] for o, n in zip(self.offsets, self.sizes)]
+
+
+class Server:
+    """Requests against an immutable factored plan (umf_server_*): each request is the base
+    operator with a local change plus right-hand sides, answered as Update + Solve on the base
+    would answer it, without changing the base.  `slots` execution contexts run requests
+    concurrently (one host thread per slot); a request's dirty fronts live in a slot pool of
+    `pool_bytes`, where it is admitted by footprint.  While the server lives the base plan cannot
+    be refactored or updated; close() the server first."""
+
+    def __init__(self, plan, slots=4, pool_bytes=8e9):
+        h = _vp()
+        _check(_srv_create(plan._h, int(slots), float(pool_bytes), C.byref(h)))
+        self._h, self.plan, self.slots = h, plan, int(slots)
+
+    @staticmethod
+    def _mods(mods):
+        m = np.ascontiguousarray(sorted(set(int(d) for d in mods)), dtype=np.int32)
+        return m, (_i(m) if len(m) else None)
+
+    def footprint(self, mods):
+        """Slot-pool bytes a request with these modified DOFs needs."""
+        m, p = self._mods(mods)
+        b = C.c_double()
+        _check(_srv_footprint(self._h, len(m), p, C.byref(b)))
+        return b.value
+
+    def admit(self, mods):
+        """Reserve the request's footprint; a ticket, or None while the pool has no room."""
+        m, p = self._mods(mods)
+        t = C.c_int()
+        rc = _srv_admit(self._h, len(m), p, C.byref(t))
+        if rc == UMF_EBUSY:
+            return None
+        _check(rc)
+        return t.value
+
+    def release(self, ticket):
+        _check(_srv_release(self._h, int(ticket)))
+
+    def pool(self):
+        """(total, used) bytes of the slot pool."""
+        a, b = C.c_double(), C.c_double()
+        _check(_srv_pool(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_values(self, slot, d_val):
+        """The request's full CSR values (device pointer) as the operator of `slot`."""
+        _check(_srv_set_values(self._h, int(slot), d_val))
+
+    def box_scale(self, slot, d_coeff, h, lo, side, factor):
+        """The base operator with the coefficient box change (lo, side, factor) as the operator
+        of `slot`, computed on the device; returns the modified DOFs."""
+        lo = np.ascontiguousarray(lo, dtype=np.int32)
+        n = self.plan.n
+        out = np.empty(max(n, 1), np.int32)
+        cnt = C.c_int()
+        _check(_srv_box_scale(self._h, int(slot), d_coeff, float(h), _i(lo), int(side), float(factor), _i(out), n,
+                              C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    def run(self, slot, ticket, d_b, d_x, nrhs, ld=None):
+        """Factor the ticket's dirty fronts with the slot's operator and solve (device pointers,
+        column-major n x nrhs).  Releases the GIL while it runs (ctypes)."""
+        _check(_srv_run(self._h, int(slot), int(ticket), d_b, d_x, int(nrhs), int(ld or self.plan.n)))
+
+    def stats(self, slot):
+        st = Stats()
+        _check(_srv_stats(self._h, int(slot), C.byref(st)))
+        return {f: getattr(st, f) for f, _ in Stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _srv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
