@@ -468,6 +468,9 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0],
                     help="serve95: request arrival rates (per second)")
     ap.add_argument("--requests", type=int, default=24, help="serve95: requests per rate")
+    ap.add_argument("--slots", type=int, default=4, help="serve95: request slots (concurrent requests)")
+    ap.add_argument("--pool-gb", type=float, default=120.0,
+                    help="serve95: slot pool for the requests' dirty fronts, FP64 (half for the FP32 forms)")
     args = ap.parse_args()
     if args.cpu_sample:  # the threaded CPU baseline's child process: no GPU work
         print(json.dumps(cpu_sample(args.cpu_sample, args.nrhs or 16)), flush=True)
@@ -620,15 +623,14 @@ def run_batch(args):
 
 
 def run_serve(args):
-    """Serving: Poisson arrivals of coefficient-change + solve requests on one 3D 95^3
-    factorization (a ~ lognormal(0,1), maxSideLength 8, seed 5); a request multiplies a box of
-    side U{4..16} at a uniform position by one LN(0,2) factor and brings U{1..32} N(0,1)
-    right-hand sides.  The server batches every waiting request into one Update + one Solve
-    (updating-multifrontal_amd/serve.py); latency = completion - arrival on a virtual clock
-    advanced by the measured GPU service time."""
+    """Serving on a resident 3D 95^3 factorization (a ~ lognormal(0,1), maxSideLength 8, seed 5):
+    Poisson arrivals of requests, each a coefficient change on a random box of side U{4..16} by
+    one LN(0,2) factor plus U{1..32} N(0,1) right-hand sides, answered against the unchanged base
+    (updating-multifrontal_amd/serve.py: slots on their own streams, admission by the dirty
+    fronts' footprint in the slot pool, the real clock).  Latency = completion - arrival."""
     import torch
     import umf
-    from serve import Request, UpdateServer
+    from serve import Request, SlotServer
     torch.cuda.set_device(0)
     N, ms, seed = 95, 8, 5
     dims = (N, N, N)
@@ -638,58 +640,71 @@ def run_serve(args):
     coeff = np.exp(rng.standard_normal((N + 2,) * 3))
     rp, ci, v = umf.build_laplacian(dims, coeff, 1, h)
     plan = umf.MF3D(Csr(rp, ci, v), *dims, 1, ms)
-    out = {"metric": "update+solve request latency, continuous batching", "unit": "ms", "n_gpus": 1,
-           "higher_is_better": False, "data": "synthetic",
-           "config": {"workload": f"3D 7-point -div(a grad u), 95^3 grid (n={n}), a~lognormal(0,1), maxSideLength {ms}; "
-                                  f"requests: a x LN(0,2) on a U{{4..16}}-side node box (applied on the device) + "
-                                  f"U{{1..32}} N(0,1) RHS, Poisson arrivals", "requests_per_rate": args.requests},
+    d_coeff = torch.from_numpy(coeff).cuda()
+    out = {"metric": "update+solve request latency (p50/p99) and requests/s, continuous batching", "unit": "ms",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": f"3D 7-point -div(a grad u), 95^3 grid (n={n}), a~lognormal(0,1), maxSideLength "
+                                  f"{ms}; requests: a x LN(0,2) on a U{{4..16}}-side node box + U{{1..32}} N(0,1) "
+                                  f"right-hand sides, Poisson arrivals, each answered against the unchanged base",
+                      "requests_per_rate": args.requests, "slots": args.slots},
            "forms": []}
-    # one operator state throughout: the FP32 pass continues from the FP64 pass's last round;
-    # the coefficient field and the operator values live on the device, where each request's
-    # box change is applied inside its round
-    d_val = torch.from_numpy(v).cuda()
-    d_coeff = torch.from_numpy(coeff.copy()).cuda()
     for prec in ("fp64", "fp32", "bf16x3"):
-        plan.SetPrecision(prec, tol=args.ir_tol)
+        if prec not in args.forms:
+            continue
+        plan.SetPrecision(prec, tol=args.ir_tol, max_iter=100)
         plan.Factor()
-        srv = UpdateServer(plan, d_val, torch, d_coeff=d_coeff, h=h)
-        form = {"precision": prec, "rates": []}
-        for rate in args.rates:
-            reqs = []
-            t = 0.0
-            for _ in range(args.requests):
-                t += float(rng.exponential(1.0 / rate))
+        pool = args.pool_gb * 1e9 * (1.0 if prec == "fp64" else 0.5)
+        S = SlotServer(plan, torch, coeff, d_coeff, v, rp, ci, h, slots=args.slots, pool_bytes=pool)
+
+        def make(count, rate):
+            reqs, t = [], 0.0
+            for _ in range(count):
+                t += float(rng.exponential(1.0 / rate)) if rate else 0.0
                 side = int(rng.integers(4, 17))
                 lo = [int(x) for x in rng.integers(0, N + 2 - side, size=3)]
-                fac = float(np.exp(2.0 * rng.standard_normal()))
-                k = int(rng.integers(1, 33))
-                reqs.append(Request(None, None, None, rng.standard_normal((k, n)), t, box=(lo, side, fac)))
-            srv.rounds = []
-            lat = srv.run(reqs) * 1e3
-            # accuracy of the last round: every right-hand side of its last request
-            r = reqs[-1]
-            X = r.x.contiguous()
-            Bd = r.rhs_d
-            R = torch.empty_like(X)
-            plan.spmv_device(X.data_ptr(), R.data_ptr(), X.shape[0])
-            na = float(torch.linalg.norm(d_val))
-            be = max(float(torch.linalg.norm(Bd[j] - R[j]) / (na * torch.linalg.norm(X[j]) + torch.linalg.norm(Bd[j])))
-                     for j in range(X.shape[0]))
-            rounds = srv.rounds
-            span = max(q.done for q in reqs) - min(q.arrival for q in reqs)
+                reqs.append(Request(rng.standard_normal((int(rng.integers(1, 33)), n)), t,
+                                    box=(lo, side, float(np.exp(2.0 * rng.standard_normal())))))
+            return reqs
+
+        S.run(make(args.slots, 0), realtime=False)  # warm-up
+        form = {"precision": prec, "pool_gb": pool / 1e9, "rates": []}
+        for rate in args.rates:
+            reqs = make(args.requests, rate)
+            lat = S.run(reqs) * 1e3
+            span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+            tr = S.inflight_trace
             ent = {"rate_per_s": rate, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                   "mean_ms": float(lat.mean()), "rounds": len(rounds),
-                   "requests_per_round": float(np.mean([x[0] for x in rounds])),
-                   "rhs_per_round": float(np.mean([x[1] for x in rounds])),
-                   "round_ms_mean": float(np.mean([x[2] for x in rounds]) * 1e3),
-                   "throughput_per_s": len(reqs) / span, "backward_error_last": be}
+                   "mean_ms": float(lat.mean()), "throughput_per_s": len(reqs) / span,
+                   "queue_wait_ms_mean": float(np.mean([(r.admitted - r.arrival) * 1e3 for r in reqs])),
+                   "gpu_update_ms_mean": float(np.mean([r.t_update_ms for r in reqs])),
+                   "gpu_solve_ms_mean": float(np.mean([r.t_solve_ms for r in reqs])),
+                   "max_in_flight": int(max(c for _, c in tr)),
+                   "footprint_gb_mean": float(np.mean([r.footprint for r in reqs]) / 1e9),
+                   "rhs_total": int(sum(len(r.rhs) for r in reqs))}
+            if prec != "fp64":
+                ent["ir_iters_max"] = int(max(r.ir_iters for r in reqs))
             progress(f"{prec} rate {rate}/s: p50 {ent['p50_ms']:.0f} ms, p99 {ent['p99_ms']:.0f} ms, "
-                     f"{len(rounds)} rounds, berr {be:.1e}")
+                     f"{ent['throughput_per_s']:.2f} req/s, in flight <= {ent['max_in_flight']}")
             form["rates"].append(ent)
+        # accuracy of the last request: infinity-norm backward error against its own operator
+        r = reqs[-1]
+        lo, side, f = r.box
+        c2 = coeff.copy()
+        c2[tuple(slice(l, l + side) for l in lo)] *= f
+        _, _, v2 = umf.build_laplacian(dims, c2, 1, h)
+        X = r.x.cpu().numpy()
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        rs = np.zeros(n)
+        np.add.at(rs, rows, np.abs(v2))
+        import scipy.sparse as sp
+        A2 = sp.csr_matrix((v2, ci, rp), shape=(n, n))
+        R = r.rhs - (A2 @ X.T).T
+        form["backward_error_inf_last"] = float(max(np.abs(R[j]).max() / (rs.max() * np.abs(X[j]).max() +
+                                                                        np.abs(r.rhs[j]).max())
+                                                    for j in range(len(R))))
+        S.close()
         out["forms"].append(form)
     print(json.dumps(out), flush=True)
-
-
 
 
 if __name__ == "__main__":
