@@ -48,8 +48,9 @@ typedef struct umf_plan umf_plan;
 /* Replaces the constructors MF2D(A, Nx, Ny, width, maxSideLength) (include/DE/mf.hpp:67) and
  * MF3D(A, Nx, Ny, Nz, width, maxSideLength) (include/DE/mf.hpp:173): plans the elimination
  * tree from the grid, uploads the CSR operator (int32 indices, n x n, DOF id i + j*Nx [+ k*Nx*Ny])
- * and allocates the device front pool.  dim = 2 or 3 (nz ignored for 2D).  The operator must be
- * symmetric positive definite with a symmetric pattern and no duplicate entries. */
+ * and allocates the device front pool.  dim = 2 or 3 (nz ignored for 2D).  The pattern must be
+ * symmetric with no duplicate entries; symmetric values are factored by Cholesky (they must be
+ * positive definite), others by LU fronts (umf_set_lu). */
 int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_length, int n,
                     const int* row_ptr, const int* col_ind, const double* val, umf_plan** plan);
 
@@ -62,7 +63,8 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
  * right-hand side; an update's modified DOFs may touch any problems, each gets its own dirty
  * sets).  Every level of every tree runs in the same batched launches: the trees are aligned at
  * the root.  flags bit 0: plan only (no device memory, val may be NULL).  Per-tree structure
- * and block queries take single-problem plans. */
+ * and block queries take single-problem plans.  Every problem's values must be symmetric
+ * (Cholesky fronts). */
 int umf_batch_create(int nprob, const int* desc, const int* const* row_ptr, const int* const* col_ind,
                      const double* const* val, int flags, umf_plan** plan);
 /* Problem `prob` of a batch: first merged DOF, size, tree levels, level shift (merged level =
@@ -220,6 +222,19 @@ int umf_profile_select(umf_plan* plan, const char* kernel);
 int umf_profile_reset(umf_plan* plan);
 int umf_profile_read(const umf_plan* plan, int i, char* name, int name_len, double* ms_total, int* launches,
                      double* flops, double* bytes);
+
+/* Front factorization (the reference factors every front by LU, include/LinAlg/dense.hpp:180-227):
+ * operators with symmetric values are factored by Cholesky (the default, which requires SPD
+ * input); an operator whose values are not symmetric -- pattern-symmetric, as the reference
+ * requires (include/DE/mf.hpp:9-12), e.g. the harness operator with stencil width 2-3 and a
+ * variable coefficient -- gets LU fronts at umf_plan_create: partial pivoting within each front's
+ * interior rows, FP64, the same blocks (A22, XL, XR, S, A11post; UMF_BLOCK_L = the packed L\U of
+ * the row-permuted A22), Solve, Apply and Update.  umf_set_lu(plan, 1) makes a plan use LU fronts
+ * also for symmetric values (indefinite operators), 0 returns to Cholesky (symmetric values
+ * only); either takes effect at the next umf_factor.  LU fronts are FP64 only and are not served
+ * (umf_server_*) or batched.  umf_get_lu reports the current form. */
+int umf_set_lu(umf_plan* plan, int on);
+int umf_get_lu(const umf_plan* plan);
 
 /* Publication between the workgroups of the blocked solve sweeps (no reference counterpart):
  * fenced = 0 (default) relies on gfx950's write-through agent-scope atomics, with no cache

@@ -22,7 +22,7 @@ CASES = [
     ((23, 57), 3, "lognormal", 1),
     ((127, 127), 8, "uniform", 1),   # the 2D 127^2 configuration's tree (levels 4, 15^2 leaves)
     ((64, 47), 5, "lognormal", 1),
-    ((40, 33), 5, "uniform", 2),  # width > 1 with a variable field is unsymmetric (see below)
+    ((40, 33), 5, "uniform", 2),  # width > 1 with a variable field is unsymmetric (test_gpu_lu.py)
     ((9, 9, 9), 3, "rand", 1),
     ((15, 15, 15), 3, "lognormal", 1),
     ((11, 14, 9), 2, "lognormal", 1),
@@ -212,20 +212,6 @@ def test_dropin_cpp_driver(args):
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
-
-
-def test_unsymmetric_operator_rejected():
-    """buildLaplacian with width 2 and a variable field averages different node pairs for
-    (p, p-2) and (p-2, p) (test_include/DE/buildLaplacian.hpp:64-74 vs :115-124), so the matrix
-    is pattern-symmetric but unsymmetric; the reference factors it with LU.  The Cholesky
-    fronts here need symmetry and refuse it up front."""
-    import umf
-    dims = (40, 33)
-    A = laplacian(dims, coeff_field(dims, "rand", seed=1, width=2), width=2)
-    D = A.to_scipy()
-    assert abs(D - D.T).max() > 0
-    with pytest.raises(ValueError, match="not symmetric"):
-        umf.MF2D(A, *dims, width=2, maxSideLength=5)
 
 
 def test_large_fronts_match_oracle():

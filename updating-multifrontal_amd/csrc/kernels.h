@@ -20,6 +20,12 @@ struct FrontDev {
   int sep_pos[6], sep_len[6];
   int flag_off;  // first of the ceil(n/64) block flags of the blocked solve sweeps
   int fly_off;   // its entries by panel tile in the fly table (leaf fronts), -1 = none
+  // LU fronts (unsymmetric operators): F[0:n,0:n] = L\U of P A22 (L unit lower), F[0:n,n:N] =
+  // U12 = L^{-1} P A21, F[n:N,0:n] = L21 = A12 U^{-1}, F[n:N,n:N] = A11 - L21 U12; dinv holds the
+  // inverted 64x64 diagonal blocks of L, uinv_off those of U (in the LU buffer), ut_off the
+  // front's 64-row slice of transposed U rows used by the trailing update (N x 64)
+  i64 uinv_off;
+  i64 ut_off;
 };
 struct SepDev {
   i64 off;  // arena offset of the k x k A11

@@ -1212,16 +1212,19 @@ __device__ __forceinline__ void gather_rhs(const FrontDev& F, T* __restrict__ X,
 
 // Forward sweep of one front: X[0:n] <- L11^{-1} W[int] block by block (x_j = Linv_j b_j, then
 // b_below -= L(below, j) x_j), X[n:N] <- -L21 X[0:n]; X[0:n] goes back to W[int].
+// perm (LU fronts): the interior rows are gathered in pivot order, b_rd -> P b_rd, from the
+// front's permuted DOF list perm + int_off (L then is unit lower; dinv holds its inverted blocks).
 template <class T>
 __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    const T* __restrict__ A, const int* __restrict__ dofs,
-                                                   T* __restrict__ W, T* __restrict__ xs) {
+                                                   T* __restrict__ W, T* __restrict__ xs,
+                                                   const int* __restrict__ perm) {
   const FrontDev F = fr[ids[blockIdx.x]];
   const T* L = A + F.off;
   T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
-  gather_rhs(F, X, W, di, nullptr);
+  gather_rhs(F, X, W, perm ? perm + F.int_off : di, nullptr);
   __shared__ T xb[NB][RB];
   for (int jb = 0; jb < F.n; jb += NB) {
     const int kb = min(NB, F.n - jb);
@@ -1754,6 +1757,364 @@ __global__ __launch_bounds__(256) void k_apply_down(const int* __restrict__ ids,
       if (r < nr)
         for (int c = 0; c < F.n; c++) s += L[(i64)c * F.N + i] * X[(i64)c * RB + r];
       X[(i64)i * RB + r] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------- LU fronts
+// Unsymmetric (pattern-symmetric) operators: each front is factored by LU with partial pivoting
+// restricted to its interior rows, as the reference factors A22 with dgetrf and forms
+// XL = A12 A22^{-1}, XR = A22^{-1} A21, S = -A12 XR (include/LinAlg/dense.hpp:180-227,
+// src/DE/mf2d.cpp:541-576).  Right-looking over 64-column panels of the whole N x N front:
+//   k_lu_panel: pivot search in column c among rows [c, n), interchange within the panel,
+//               scale rows (c, N) (the boundary rows give L21 = A12 U^{-1}), rank-1 update of
+//               the panel; then the panel's diagonal block's unit-lower and upper inverses;
+//   k_lu_swap:  the panel's interchanges applied to every other column of the front;
+//   k_lu_trsm:  U row block = Linv_kk A(panel rows, later columns), also stored transposed;
+//   k_lu_gemm:  trailing update A(s0.., s0..) -= L(s0.., panel) U(panel, s0..) on MFMA, the
+//               boundary block accumulating S.
+// FP64 only.  A zero pivot records the first failing (front, column) in err[0..2].
+
+// Zero every entry of each listed matrix.
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_full(const int* __restrict__ ids, const MatDev* __restrict__ mats,
+                                                   T* __restrict__ A) {
+  const MatDev M = mats[ids[blockIdx.y]];
+  T* a = A + M.off;
+  for (int c = blockIdx.x; c < M.cols; c += gridDim.x) {
+    T* col = a + (i64)c * M.ld;
+    for (int r = threadIdx.x; r < M.ld; r += 256) col[r] = T(0);
+  }
+}
+
+// Separator Schur accumulation on the full k x k matrix, box 0 first (ScatterAdd_,
+// src/DE/mf2d.cpp:599-606).
+template <class T>
+__global__ __launch_bounds__(256) void k_sep_add_full(const int* __restrict__ ids, const SepDev* __restrict__ seps,
+                                                      const FrontDev* __restrict__ fr, T* __restrict__ A) {
+  const SepDev S = seps[ids[blockIdx.y]];
+  const FrontDev B0 = fr[S.box[0]], B1 = fr[S.box[1]];
+  T* a = A + S.off;
+  const T* s0 = A + B0.off + (i64)(B0.n + S.pos[0]) * B0.N + B0.n + S.pos[0];
+  const T* s1 = A + B1.off + (i64)(B1.n + S.pos[1]) * B1.N + B1.n + S.pos[1];
+  for (int c = blockIdx.x; c < S.k; c += gridDim.x)
+    for (int r = threadIdx.x; r < S.k; r += 256) {
+      T v = a[(i64)c * S.k + r] + s0[(i64)c * B0.N + r];
+      a[(i64)c * S.k + r] = v + s1[(i64)c * B1.N + r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lu_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                  double* __restrict__ A, int k0, int* __restrict__ ipiv,
+                                                  double* __restrict__ uinv, int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  if (k0 >= F.n) return;
+  const int kb = min(NB, F.n - k0), N = F.N, t = threadIdx.x;
+  double* a = A + F.off;
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ double D[NB][NB + 1];
+  for (int j = 0; j < kb; j++) {
+    const int c = k0 + j;
+    double* col = a + (i64)c * N;
+    // pivot: the first row of largest |a_ic| among the interior rows [c, n)
+    double best = -1.0;
+    int bi = F.n;
+    for (int i = c + t; i < F.n; i += 256) {
+      const double v = fabs(col[i]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    rv[t] = best;
+    ri[t] = bi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h && (rv[t + h] > rv[t] || (rv[t + h] == rv[t] && ri[t + h] < ri[t]))) {
+        rv[t] = rv[t + h];
+        ri[t] = ri[t + h];
+      }
+      __syncthreads();
+    }
+    const int p = ri[0] < F.n ? ri[0] : c;  // (a column of NaNs keeps its row)
+    __syncthreads();
+    if (p != c && t < kb) {  // interchange within the panel
+      double* pc = a + (i64)(k0 + t) * N;
+      const double x = pc[c];
+      pc[c] = pc[p];
+      pc[p] = x;
+    }
+    if (t == 0) ipiv[F.int_off + c] = p;
+    __syncthreads();
+    const double d = col[c];
+    if (d == 0.0) {
+      if (t == 0 && atomicCAS(err, 0, 1) == 0) {
+        err[1] = fid;
+        err[2] = c;
+      }
+      continue;  // the column stays as it is (LAPACK goes on; the result is reported invalid)
+    }
+    for (int i = c + 1 + t; i < N; i += 256) col[i] /= d;
+    __syncthreads();
+    for (int cc = c + 1; cc < k0 + kb; cc++) {
+      double* pc = a + (i64)cc * N;
+      const double u = pc[c];
+      if (u != 0.0)
+        for (int i = c + 1 + t; i < N; i += 256) pc[i] -= col[i] * u;
+    }
+    __syncthreads();
+  }
+  // inverses of the panel's diagonal block: unit lower (dinv) and upper (uinv), column-major,
+  // zero outside their triangle and beyond kb
+  for (int e = t; e < NB * NB; e += 256) {
+    const int r = e & (NB - 1), cc = e / NB;
+    D[r][cc] = (r < kb && cc < kb) ? a[(i64)(k0 + cc) * N + k0 + r] : 0.0;
+  }
+  __syncthreads();
+  double* Li = A + F.dinv_off + (i64)(k0 / NB) * NB * NB;
+  double* Ui = uinv + F.uinv_off + (i64)(k0 / NB) * NB * NB;
+  if (t < NB) {  // column t of L^{-1}: forward substitution
+    const int cc = t;
+    for (int i = 0; i < NB; i++) {
+      double x = 0.0;
+      if (cc < kb && i >= cc && i < kb) {
+        x = i == cc ? 1.0 : 0.0;
+        for (int k = cc; k < i; k++) x -= D[i][k] * Li[(i64)cc * NB + k];
+      }
+      Li[(i64)cc * NB + i] = x;
+    }
+  } else if (t < 2 * NB) {  // column t - 64 of U^{-1}: back substitution
+    const int cc = t - NB;
+    for (int i = NB - 1; i >= 0; i--) {
+      double x = 0.0;
+      if (cc < kb && i <= cc) {
+        x = i == cc ? 1.0 : 0.0;
+        for (int k = i + 1; k <= cc; k++) x -= D[i][k] * Ui[(i64)cc * NB + k];
+        x /= D[i][i];
+      }
+      Ui[(i64)cc * NB + i] = x;
+    }
+  }
+}
+
+// The panel's interchanges on the front's other columns [0, k0) and [k0 + kb, N), one column per
+// thread.
+__global__ __launch_bounds__(256) void k_lu_swap(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 double* __restrict__ A, int k0, const int* __restrict__ ipiv) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  if (k0 >= F.n) return;
+  const int kb = min(NB, F.n - k0);
+  const int cc = blockIdx.x * 256 + threadIdx.x;
+  if (cc >= F.N || (cc >= k0 && cc < k0 + kb)) return;
+  double* col = A + F.off + (i64)cc * F.N;
+  for (int j = 0; j < kb; j++) {
+    const int c = k0 + j, p = ipiv[F.int_off + c];
+    if (p != c) {
+      const double x = col[c];
+      col[c] = col[p];
+      col[p] = x;
+    }
+  }
+}
+
+// U row block: A(k0.., cols) <- Linv_kk A(k0.., cols) for 64 columns per workgroup
+// (blockIdx.x) of [k0 + kb, N); the result also goes transposed to ut (ut[c + k N] = U(k0+k, c)).
+__global__ __launch_bounds__(256) void k_lu_trsm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 double* __restrict__ A, int k0, double* __restrict__ ut) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  if (k0 >= F.n) return;
+  const int kb = min(NB, F.n - k0), s0 = k0 + kb, N = F.N;
+  const int c0 = s0 + NB * blockIdx.x;
+  if (c0 >= N) return;
+  const int nc = min(NB, N - c0), t = threadIdx.x;
+  __shared__ double Li[NB][NB + 1], B[NB][NB + 1];
+  const double* Lg = A + F.dinv_off + (i64)(k0 / NB) * NB * NB;
+  double* a = A + F.off;
+  for (int e = t; e < NB * NB; e += 256) {
+    const int r = e & (NB - 1), cc = e / NB;
+    Li[r][cc] = Lg[(i64)cc * NB + r];
+    B[r][cc] = (r < kb && cc < nc) ? a[(i64)(c0 + cc) * N + k0 + r] : 0.0;
+  }
+  __syncthreads();
+  const int cc = t & (NB - 1), r0 = (t >> 6) * 16;
+  if (cc >= nc) return;
+  double* ucol = ut + F.ut_off + c0 + cc;
+  for (int r = r0; r < r0 + 16 && r < kb; r++) {
+    double s = 0.0;
+    for (int k = 0; k <= r; k++) s += Li[r][k] * B[k][cc];
+    a[(i64)(c0 + cc) * N + k0 + r] = s;
+    ucol[(i64)r * N] = s;
+  }
+}
+
+// Trailing update of the full region [s0, N) x [s0, N), s0 = k0 + kb: C -= L(rows, panel)
+// U(panel, cols), one 64 x 64 tile per workgroup (blockIdx.x = tile, row-major over the region's
+// tile grid), the product on FP64 MFMA (tile_gemm_nt_t with P = the panel's L columns and Q = the
+// transposed U rows).
+__global__ __launch_bounds__(256) void k_lu_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 double* __restrict__ A, int k0, const double* __restrict__ ut) {
+  const FrontDev F = fr[ids[blockIdx.y]];
+  if (k0 >= F.n) return;
+  const int kb = min(NB, F.n - k0), s0 = k0 + kb, N = F.N;
+  const int T = (N - s0 + NB - 1) / NB;
+  if ((int)blockIdx.x >= T * T) return;
+  const int row0 = __builtin_amdgcn_readfirstlane(s0 + NB * ((int)blockIdx.x / T));
+  const int col0 = __builtin_amdgcn_readfirstlane(s0 + NB * ((int)blockIdx.x % T));
+  const int nr = min(NB, N - row0), nc = min(NB, N - col0);
+  __shared__ TileSmemT<double, 64> sm;
+  double* a = A + F.off;
+  V4<double> acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<double>();
+  double cv[2][2][4];
+  double* C = a + (i64)col0 * N + row0;
+  tile_read(cv, C, N, nr, nc, true);
+  tile_gemm_nt_t<double, 64>(acc, a + (i64)k0 * N + row0, N, nr, ut + F.ut_off + col0, N, nc, kb, sm);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<double>(bj, r);
+        if (i < nr && j < nc) C[(i64)j * N + i] = cv[bi][bj][r] - acc[bi][bj][r];
+      }
+}
+
+// The front's interior DOFs in pivot order: perm[int_off + j] = dofs[int_off + pi(j)], pi the
+// product of the interchanges (P b)[j] = b[pi(j)].  One workgroup per front; the interchanges
+// are applied in order by one thread, on LDS when the front fits (n <= 12288).
+__global__ __launch_bounds__(256) void k_lu_perm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                 const int* __restrict__ ipiv, const int* __restrict__ dofs,
+                                                 int* __restrict__ perm) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  __shared__ int pi[12288];
+  const int* di = dofs + F.int_off;
+  int* out = perm + F.int_off;
+  const int* pv = ipiv + F.int_off;
+  if (F.n <= 12288) {
+    for (int j = threadIdx.x; j < F.n; j += 256) pi[j] = j;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < F.n; j++) {
+        const int p = pv[j];
+        const int x = pi[j];
+        pi[j] = pi[p];
+        pi[p] = x;
+      }
+    __syncthreads();
+    for (int j = threadIdx.x; j < F.n; j += 256) out[j] = di[pi[j]];
+  } else {
+    for (int j = threadIdx.x; j < F.n; j += 256) out[j] = j;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < F.n; j++) {
+        const int p = pv[j];
+        const int x = out[j];
+        out[j] = out[p];
+        out[p] = x;
+      }
+    __syncthreads();
+    for (int j = threadIdx.x; j < F.n; j += 256) out[j] = di[out[j]];
+  }
+}
+
+// Backward sweep of one LU front: X[0:n] = y (the forward result, W[int]), X[n:N] = W[bd];
+// block by block from the last: z_j = y_j - U(block j, later columns) X(later), x_j = Uinv_j z_j
+// (= U^{-1} (y - U12 x_sk) overall, the reference's A22^{-1} b_rd - XR b_sk); X[0:n] -> W[int].
+__global__ __launch_bounds__(256) void k_lu_bwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                const double* __restrict__ A, const int* __restrict__ dofs,
+                                                double* __restrict__ W, double* __restrict__ xs,
+                                                const double* __restrict__ uinv) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  const double* a = A + F.off;
+  double* X = xs + F.x_off;
+  const int* di = dofs + F.int_off;
+  const int* db = dofs + F.bd_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  gather_rhs(F, X, W, di, db);
+  __shared__ double zb[NB][RB];
+  const int nblk = (F.n + NB - 1) / NB;
+  for (int bk = nblk - 1; bk >= 0; bk--) {
+    const int jb = bk * NB, kb = min(NB, F.n - jb);
+    __syncthreads();
+    // T(i, r) = sum_{c >= jb+kb} U(jb+i, c) X(c, r): wave w -> rows 16w..16w+15
+    const int row = 16 * w + c16;
+    const bool rin = row < kb;
+    const double* ar = a + jb + (rin ? row : 0);
+    V4<double> acc = vzero<double>();
+    for (int c0 = jb + kb; c0 < F.N; c0 += 4 * GU) {
+      double av[GU], bv[GU];
+#pragma unroll
+      for (int u = 0; u < GU; u++) {
+        const int c = c0 + 4 * u + q;
+        const int cc = min(c, F.N - 1);
+        av[u] = ld_or0(ar + (i64)cc * F.N, rin && c < F.N);
+        bv[u] = X[(i64)cc * RB + c16];
+      }
+#pragma unroll
+      for (int u = 0; u < GU; u++) acc = mfma4(av[u], bv[u], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 16 * w + mrow<double>(q, r);
+      zb[i][c16] = ld_or0(X + (i64)(jb + min(i, kb - 1)) * RB + c16, i < kb) - (i < kb ? acc[r] : 0.0);
+    }
+    __syncthreads();
+    {  // x_j = Uinv_j z: wave w -> rows 16w..16w+15
+      const double* Ui = uinv + F.uinv_off + (i64)bk * NB * NB;
+      V4<double> a2 = vzero<double>();
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        const int c = 4 * k4 + q;
+        a2 = mfma4(Ui[(i64)c * NB + 16 * w + c16], zb[c][c16], a2);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + mrow<double>(q, r);
+        if (i < kb) X[(i64)(jb + i) * RB + c16] = a2[r];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+}
+
+// Upward step of Apply for one LU front: u = [U U12] [x_rd; x_sk] (kept in X[0:n] for
+// k_apply_down, whose X[n:N] = L21 u is the reference's XL b_rd), b_rd <- P^T L u written to the
+// permuted interior DOFs (the reference's A22 (b_rd + XR b_sk), src/DE/mf2d.cpp:904-974).
+__global__ __launch_bounds__(256) void k_lu_apply_up(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                     const double* __restrict__ A, const int* __restrict__ dofs,
+                                                     double* __restrict__ W, double* __restrict__ xs, int nr,
+                                                     const int* __restrict__ perm) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  const double* a = A + F.off;
+  double* X = xs + F.x_off;
+  const int* di = dofs + F.int_off;
+  const int* db = dofs + F.bd_off;
+  const int* pm = perm + F.int_off;
+  const int t = threadIdx.x;
+  gather_rhs(F, X, W, di, db);
+  __syncthreads();
+  // u_i = sum_{c >= i} F(i, c) X(c) into W[int] (X still holds x)
+  for (int i = t; i < F.n; i += 256)
+    for (int r = 0; r < nr; r++) {
+      double s = 0.0;
+      for (int c = i; c < F.N; c++) s += a[(i64)c * F.N + i] * X[(i64)c * RB + r];
+      W[(i64)di[i] * RB + r] = s;
+    }
+  __syncthreads();
+  for (int e = t; e < F.n * RB; e += 256) X[e] = (e & 15) < nr ? W[(i64)di[e >> 4] * RB + (e & 15)] : 0.0;
+  __syncthreads();
+  // (L u)_i = u_i + sum_{c < i} F(i, c) u_c to row pi(i) of b_rd
+  for (int i = t; i < F.n; i += 256)
+    for (int r = 0; r < nr; r++) {
+      double s = X[(i64)i * RB + r];
+      for (int c = 0; c < i; c++) s += a[(i64)c * F.N + i] * X[(i64)c * RB + r];
+      W[(i64)pm[i] * RB + r] = s;
     }
 }
 

@@ -447,11 +447,12 @@ struct Builder {
     for (int i = 0; i < (int)b.size(); i++) where[b[i]] = val_or_clear ? n + i : -1;
   }
 
-  // lower-triangle CSR triples of a front: leaf fronts take every entry inside the front except
-  // the trailing diagonal separator blocks (never read downstream); upper fronts take only the
-  // entries with a center point in row or column (AddNewInteractions_ into A22 and
-  // AddNewRowInteractions_ into A12, src/DE/mf2d.cpp:369-401).
-  void front_triples(int f, std::vector<Triple>& out) {
+  // CSR triples of a front, lower triangle (full = false) or both (full = true, unsymmetric
+  // operators): leaf fronts take every entry inside the front except the trailing diagonal
+  // separator blocks (never read downstream); upper fronts take only the entries with a center
+  // point in row or column (AddNewInteractions_ into A22, AddNewRowInteractions_ into A12 and
+  // AddNewColInteractions_ into A21, src/DE/mf2d.cpp:369-401, :836-901).
+  void front_triples(int f, std::vector<Triple>& out, bool full) {
     const FrontInfo& F = P.fronts[f];
     bool leaf = F.level == g.L - 1;
     mark_front(f, 1);
@@ -479,12 +480,14 @@ struct Builder {
       int d = p < n ? a[p] : b[p - n];
       for (int k = P.rp[d]; k < P.rp[d + 1]; k++) {
         int q = where[P.ci[k]];
-        if (q < 0 || q > p) continue;
+        if (q < 0 || (!full && q > p)) continue;
         if (leaf) {
           if (p >= n && q >= n && run[p - n] == run[q - n]) continue;
         } else {
-          // A22 (both < n): a box-center endpoint; A12 (row >= n, col < n): an edge-center row
-          bool take = (p < n && (center[p] == 1 || center[q] == 1)) || (p >= n && q < n && center[p] == 2);
+          // A22 (both < n): a box-center endpoint; A12 (row >= n, col < n): an edge-center row;
+          // A21 (row < n, col >= n): an edge-center column
+          bool take = (p < n && q < n && (center[p] == 1 || center[q] == 1)) || (p >= n && q < n && center[p] == 2) ||
+                      (p < n && q >= n && center[q] == 2);
           if (!take) continue;
         }
         out.push_back(Triple{p, q, k});
@@ -493,9 +496,10 @@ struct Builder {
     mark_front(f, 0);
   }
 
-  // separator A11 triples (lower): leaf = A_(DOFs, DOFs) (src/DE/mf2d.cpp:595); upper = entries
-  // with a center point endpoint (AddNewInteractions_, src/DE/mf2d.cpp:307)
-  void sep_triples(int s, std::vector<Triple>& out) {
+  // separator A11 triples (lower, or both triangles when full): leaf = A_(DOFs, DOFs)
+  // (src/DE/mf2d.cpp:595); upper = entries with a center point endpoint (AddNewInteractions_,
+  // src/DE/mf2d.cpp:307)
+  void sep_triples(int s, std::vector<Triple>& out, bool full) {
     const SepInfo& S = P.seps[s];
     const std::vector<int>& d = sep_dofs[s];
     bool leaf = S.level == g.L - 1;
@@ -510,7 +514,7 @@ struct Builder {
     for (int p = 0; p < S.k; p++)
       for (int k = P.rp[d[p]]; k < P.rp[d[p] + 1]; k++) {
         int q = where[P.ci[k]];
-        if (q < 0 || q > p) continue;
+        if (q < 0 || (!full && q > p)) continue;
         if (!leaf && p < k0 && q < k0) continue;
         out.push_back(Triple{p, q, k});
       }
@@ -530,8 +534,135 @@ struct Builder {
 
 }  // namespace
 
+// The assembly programs of every matrix (CSR triples and block-copy tiles), lower triangles only
+// (Cholesky fronts of symmetric operators) or full matrices (LU fronts of unsymmetric ones), and
+// for the former the fly / rest split of the leaf fronts.
+void build_programs(Plan& P, bool full) {
+  Builder B(P);
+  Grid& g = P.g;
+  const int n = P.n;
+  B.box_int.assign(P.nfronts(), {});
+  B.box_bd.assign(P.nfronts(), {});
+  B.sep_dofs.assign(P.nseps_total(), {});
+  for (int f = 0; f < P.nfronts(); f++) {
+    const FrontInfo& F = P.fronts[f];
+    B.box_int[f].assign(P.dofs.begin() + F.int_off, P.dofs.begin() + F.int_off + F.n);
+    B.box_bd[f].assign(P.dofs.begin() + F.bd_off, P.dofs.begin() + F.bd_off + F.m);
+  }
+  for (int s = 0; s < P.nseps_total(); s++)
+    B.sep_dofs[s].assign(P.dofs.begin() + P.seps[s].dof_off, P.dofs.begin() + P.seps[s].dof_off + P.seps[s].k);
+  B.where.assign(n, -1);
+  int NF = P.nfronts();
+  P.triples.clear();
+  P.ops.clear();
+  P.tiles.clear();
+  P.tri_off.assign(P.mats.size() + 1, 0);
+  P.tile_off.assign(P.mats.size() + 1, 0);
+  for (int mid = 0; mid < (int)P.mats.size(); mid++) {
+    P.tri_off[mid] = (i64)P.triples.size();
+    P.tile_off[mid] = (i64)P.tiles.size();
+    if (mid < NF) {
+      int f = mid;
+      const FrontInfo& F = P.fronts[f];
+      B.front_triples(f, P.triples, full);
+      if (F.level < g.L - 1) {
+        // position of every child separator (level+1) inside this front
+        std::map<int, int> pos;
+        std::vector<int> sc, sh;
+        g.child_seps(F.box, F.level, sc, sh);
+        int p = 0;
+        for (int e : sc) {
+          int sg = P.sep_gid[F.level + 1][e];
+          pos[sg] = p;
+          p += P.seps[sg].k;
+        }
+        for (int s = 0; s < F.nsep; s++) {
+          const SepInfo& S = P.seps[F.sep[s]];
+          int ch[4], k = g.sep_children(S.idx, S.level, ch);
+          int q = F.n + F.sep_pos[s];
+          for (int i = 0; i < k; i++) {
+            int sg = P.sep_gid[S.level + 1][ch[i]];
+            pos[sg] = q;
+            q += P.seps[sg].k;
+          }
+        }
+        // off-diagonal separator-pair blocks of each child's update matrix
+        int ch[8], kc = g.children(F.box, F.level, ch);
+        for (int i = 0; i < kc; i++) {
+          int cf = P.box_gid[F.level + 1][ch[i]];
+          const FrontInfo& C = P.fronts[cf];
+          for (int u = 0; u < C.nsep; u++)
+            for (int v = 0; v < C.nsep; v++) {
+              if (u == v) continue;
+              int Pu = pos.at(C.sep[u]), Pv = pos.at(C.sep[v]);
+              if (!full && Pu < Pv) continue;  // upper triangle of this front
+              BlockOp op{};
+              op.src = cf;
+              op.dr0 = Pu;
+              op.dc0 = Pv;
+              op.nr = C.sep_len[u];
+              op.nc = C.sep_len[v];
+              if (full || C.sep_pos[u] > C.sep_pos[v]) {  // full: every block as it is stored
+                op.sr0 = C.n + C.sep_pos[u];
+                op.sc0 = C.n + C.sep_pos[v];
+                op.flags = 0;
+              } else {  // stored transposed in the child's lower triangle
+                op.sr0 = C.n + C.sep_pos[v];
+                op.sc0 = C.n + C.sep_pos[u];
+                op.flags = 1;
+              }
+              B.add_op(P.ops, P.tiles, op);
+            }
+        }
+        // interior child separators: their A11 is the diagonal block of A22
+        for (int e : sc) {
+          int sg = P.sep_gid[F.level + 1][e];
+          BlockOp op{};
+          op.src = NF + sg;
+          op.sr0 = op.sc0 = 0;
+          op.dr0 = op.dc0 = pos.at(sg);
+          op.nr = op.nc = P.seps[sg].k;
+          op.flags = full ? 0 : 2;
+          B.add_op(P.ops, P.tiles, op);
+        }
+      }
+    } else {
+      int s = mid - NF;
+      const SepInfo& S = P.seps[s];
+      B.sep_triples(s, P.triples, full);
+      if (S.level < g.L - 1) {
+        int ch[4], k = g.sep_children(S.idx, S.level, ch);
+        int q = 0;
+        for (int i = 0; i < k; i++) {
+          int sg = P.sep_gid[S.level + 1][ch[i]];
+          BlockOp op{};
+          op.src = NF + sg;
+          op.sr0 = op.sc0 = 0;
+          op.dr0 = op.dc0 = q;
+          op.nr = op.nc = P.seps[sg].k;
+          op.flags = full ? 0 : 2;
+          B.add_op(P.ops, P.tiles, op);
+          q += P.seps[sg].k;
+        }
+      }
+    }
+  }
+  P.tri_off[P.mats.size()] = (i64)P.triples.size();
+  P.tile_off[P.mats.size()] = (i64)P.tiles.size();
+  P.full = full;
+  if (full) {  // no fly split: LU fronts are zeroed and scattered whole
+    P.fly_off.assign(P.nfronts(), -1);
+    P.fly_tab.clear();
+    P.fly_tri.clear();
+    P.rest_tri.clear();
+    P.rest_off.clear();
+  } else {
+    build_fly(P);
+  }
+}
+
 std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int max_side, int n, const int* rp,
-                       const int* ci, bool& geom_error) {
+                       const int* ci, bool& geom_error, bool full) {
   geom_error = false;
   Grid& g = P.g;
   if (dim != 2 && dim != 3) return "dim must be 2 or 3";
@@ -643,108 +774,10 @@ std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int 
   }
   P.arena_elems = off;
 
-  // assembly programs
-  B.where.assign(n, -1);
-  int NF = P.nfronts();
-  P.triples.clear();
-  P.ops.clear();
-  P.tiles.clear();
-  P.tri_off.assign(P.mats.size() + 1, 0);
-  P.tile_off.assign(P.mats.size() + 1, 0);
-  for (int mid = 0; mid < (int)P.mats.size(); mid++) {
-    P.tri_off[mid] = (i64)P.triples.size();
-    P.tile_off[mid] = (i64)P.tiles.size();
-    if (mid < NF) {
-      int f = mid;
-      const FrontInfo& F = P.fronts[f];
-      B.front_triples(f, P.triples);
-      if (F.level < g.L - 1) {
-        // position of every child separator (level+1) inside this front
-        std::map<int, int> pos;
-        std::vector<int> sc, sh;
-        g.child_seps(F.box, F.level, sc, sh);
-        int p = 0;
-        for (int e : sc) {
-          int sg = P.sep_gid[F.level + 1][e];
-          pos[sg] = p;
-          p += P.seps[sg].k;
-        }
-        for (int s = 0; s < F.nsep; s++) {
-          const SepInfo& S = P.seps[F.sep[s]];
-          int ch[4], k = g.sep_children(S.idx, S.level, ch);
-          int q = F.n + F.sep_pos[s];
-          for (int i = 0; i < k; i++) {
-            int sg = P.sep_gid[S.level + 1][ch[i]];
-            pos[sg] = q;
-            q += P.seps[sg].k;
-          }
-        }
-        // off-diagonal separator-pair blocks of each child's update matrix
-        int ch[8], kc = g.children(F.box, F.level, ch);
-        for (int i = 0; i < kc; i++) {
-          int cf = P.box_gid[F.level + 1][ch[i]];
-          const FrontInfo& C = P.fronts[cf];
-          for (int u = 0; u < C.nsep; u++)
-            for (int v = 0; v < C.nsep; v++) {
-              if (u == v) continue;
-              int Pu = pos.at(C.sep[u]), Pv = pos.at(C.sep[v]);
-              if (Pu < Pv) continue;  // upper triangle of this front
-              BlockOp op{};
-              op.src = cf;
-              op.dr0 = Pu;
-              op.dc0 = Pv;
-              op.nr = C.sep_len[u];
-              op.nc = C.sep_len[v];
-              if (C.sep_pos[u] > C.sep_pos[v]) {
-                op.sr0 = C.n + C.sep_pos[u];
-                op.sc0 = C.n + C.sep_pos[v];
-                op.flags = 0;
-              } else {  // stored transposed in the child's lower triangle
-                op.sr0 = C.n + C.sep_pos[v];
-                op.sc0 = C.n + C.sep_pos[u];
-                op.flags = 1;
-              }
-              B.add_op(P.ops, P.tiles, op);
-            }
-        }
-        // interior child separators: their A11 is the diagonal block of A22
-        for (int e : sc) {
-          int sg = P.sep_gid[F.level + 1][e];
-          BlockOp op{};
-          op.src = NF + sg;
-          op.sr0 = op.sc0 = 0;
-          op.dr0 = op.dc0 = pos.at(sg);
-          op.nr = op.nc = P.seps[sg].k;
-          op.flags = 2;
-          B.add_op(P.ops, P.tiles, op);
-        }
-      }
-    } else {
-      int s = mid - NF;
-      const SepInfo& S = P.seps[s];
-      B.sep_triples(s, P.triples);
-      if (S.level < g.L - 1) {
-        int ch[4], k = g.sep_children(S.idx, S.level, ch);
-        int q = 0;
-        for (int i = 0; i < k; i++) {
-          int sg = P.sep_gid[S.level + 1][ch[i]];
-          BlockOp op{};
-          op.src = NF + sg;
-          op.sr0 = op.sc0 = 0;
-          op.dr0 = op.dc0 = q;
-          op.nr = op.nc = P.seps[sg].k;
-          op.flags = 2;
-          B.add_op(P.ops, P.tiles, op);
-          q += P.seps[sg].k;
-        }
-      }
-    }
-  }
-  P.tri_off[P.mats.size()] = (i64)P.triples.size();
-  P.tile_off[P.mats.size()] = (i64)P.tiles.size();
-  build_fly(P);
+  build_programs(P, full);
   return "";
 }
+
 
 // Split the entries of every front without block copies by where the left-looking
 // factorization first reads them: the tile of column block cb = col/64 and rows

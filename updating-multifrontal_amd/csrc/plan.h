@@ -116,6 +116,9 @@ struct Plan {
   std::vector<Triple> fly_tri;   // grouped by (column block, row tile)
   std::vector<Triple> rest_tri;  // the front's other entries
   std::vector<i64> rest_off;     // per matrix (size mats+1)
+  // programs write full matrices (LU fronts of an unsymmetric operator) instead of lower
+  // triangles (Cholesky fronts); no fly split then
+  bool full = false;
 
   int nfronts() const { return (int)fronts.size(); }
   int nseps_total() const { return (int)seps.size(); }
@@ -157,9 +160,13 @@ void merge_plans(Plan& M, std::vector<Plan>&& subs);
 // the fly_* / rest_* split of build_plan's assembly programs
 void build_fly(Plan& P);
 
+// (Re)builds the assembly programs of a planned tree, lower triangles or full matrices.
+void build_programs(Plan& P, bool full);
+
 // Builds the plan; returns an empty string on success or the error message.  `geom_error`
-// is set when the failure is the reference's geometry exception.
+// is set when the failure is the reference's geometry exception.  full: programs for full
+// matrices (LU fronts).
 std::string build_plan(Plan& P, int dim, int nx, int ny, int nz, int width, int max_side, int n, const int* rp,
-                       const int* ci, bool& geom_error);
+                       const int* ci, bool& geom_error, bool full = false);
 
 }  // namespace umf

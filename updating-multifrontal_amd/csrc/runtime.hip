@@ -118,6 +118,13 @@ struct umf_plan {
   int epoch = 0;           // value the next blocked sweep launch publishes its blocks with
   bool fenced = false;     // blocked sweeps publish with agent-scope release / acquire fences
   int device = 0;          // HIP device the plan's memory and stream live on
+  // LU fronts (unsymmetric operators): pivots and permuted interior DOF lists (indexed like the
+  // DOF lists), inverted diagonal blocks of U, transposed U rows of the trailing updates
+  bool lu = false;
+  int* d_ipiv = nullptr;
+  int* d_perm = nullptr;
+  double* d_uinv = nullptr;
+  double* d_ut = nullptr;
   int* d_ids = nullptr;  // id lists of the current schedule
   size_t ids_cap = 0;
   // tuning of the factorization's GEMM stages (umf_set_tune / umf_autotune): the outer column
@@ -632,7 +639,82 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
 
 // Factor over a dirty schedule in the precision the plan is set to (the FP32 front pool is
 // allocated on first use).
+// LU factorization over the dirty schedule (unsymmetric operators, FP64): per level, zero and
+// assemble the full separator and front matrices, then right-looking 64-column panels of every
+// front (panel with partial pivoting among the interior rows, interchanges on the other columns,
+// U row block, trailing update including the boundary block's Schur complement), the pivot-order
+// DOF lists, and the separators' Schur accumulation.  Every front's arithmetic depends on the
+// front alone, as in the Cholesky schedule.
+int run_factor_lu(umf_plan* p, const Plan::Dirty& D) {
+  const Plan& P = p->P;
+  Sched S = make_sched(P, D);
+  int rc = ensure_ids(p, S.ids.size());
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->ids_h = S.ids;
+  if (!p->ids_h.empty())
+    HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                          p->stream));
+  const int* ids = p->d_ids;
+  const int NF = P.nfronts();
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
+  double* A = p->arena;
+  for (int ell = P.L - 1; ell >= 0; ell--) {
+    const Sched::Lvl& L = S.lv[ell];
+    g_level = ell;
+    if (L.sm_cnt > 0) {
+      launch(p, "k_zero_full", 0, 0, k_zero_full<double>,
+             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+             (const MatDev*)p->d_mats, A);
+      launch(p, "k_assemble", 0, 0, k_assemble<double>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+             (const MatDev*)p->d_mats, A, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
+             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+    }
+    if (L.f_cnt == 0) continue;
+    launch(p, "k_zero_full", 0, 0, k_zero_full<double>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
+           ids + L.f_off, (const MatDev*)p->d_mats, A);
+    launch(p, "k_assemble", 0, 0, k_assemble<double>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
+           (const MatDev*)p->d_mats, A, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
+           (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+    int act = L.f_cnt;  // fronts sorted by pivot-block size: the prefix with n > k0 takes part
+    for (int k0 = 0; k0 < L.maxn; k0 += NB) {
+      while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
+      int maxN = 0;
+      double fl = 0;
+      for (int i = 0; i < act; i++) {
+        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+        maxN = std::max(maxN, F.n + F.m);
+        const double kb = std::min(NB, F.n - k0), r = F.n + F.m - k0 - kb;
+        fl += kb * kb * kb / 3.0 + 2.0 * r * kb * kb + 2.0 * r * r * kb;  // LU form: panel, TRSM, trailing
+      }
+      launch(p, "k_lu_panel", 0, 0, k_lu_panel, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts, A, k0,
+             p->d_ipiv, p->d_uinv, p->d_err);
+      launch(p, "k_lu_swap", 0, 0, k_lu_swap, dim3((maxN + 255) / 256, act), ids + L.f_off,
+             (const FrontDev*)p->d_fronts, A, k0, (const int*)p->d_ipiv);
+      const int rest = maxN - k0 - 1;  // columns right of the first panel column (an upper bound)
+      if (rest > 0) {
+        const int tb = (rest + NB - 1) / NB;
+        launch(p, "k_lu_trsm", 0, 0, k_lu_trsm, dim3(tb, act), ids + L.f_off, (const FrontDev*)p->d_fronts, A, k0,
+               p->d_ut);
+        launch(p, "k_lu_gemm", fl, 0, k_lu_gemm, dim3(tb * tb, act), ids + L.f_off, (const FrontDev*)p->d_fronts,
+               A, k0, (const double*)p->d_ut);
+      }
+    }
+    launch(p, "k_lu_perm", 0, 0, k_lu_perm, dim3(L.f_cnt), ids + L.f_off, (const FrontDev*)p->d_fronts,
+           (const int*)p->d_ipiv, (const int*)p->d_dofs, p->d_perm);
+    if (L.s_cnt > 0)
+      launch(p, "k_sep_add", 0, 0, k_sep_add_full<double>,
+             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.s_cnt), L.s_cnt), ids + L.s_off,
+             (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, A);
+  }
+  (void)NF;
+  g_level = -1;
+  HIPCHK(hipGetLastError());
+  return UMF_OK;
+}
+
 int factor_any(umf_plan* p, const Plan::Dirty& D, const int* ids_pre = nullptr) {
+  if (p->lu) return run_factor_lu(p, D);
   if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
@@ -700,7 +782,10 @@ int check_err(umf_plan* p) {
   if (h[0]) {
     const FrontInfo& F = p->P.fronts[h[1]];
     char buf[256];
-    if (p->P.forest)
+    if (p->lu)
+      snprintf(buf, sizeof buf, "zero pivot: level %d box %d pivot column %d (front of n=%d): singular pivot block",
+               F.level, F.box, h[2], F.n);
+    else if (p->P.forest)
       snprintf(buf, sizeof buf,
                "non-positive pivot: problem %d level %d box %d pivot column %d (front of n=%d): operator not SPD",
                F.prob, F.level, F.box, h[2], F.n);
@@ -784,6 +869,38 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
   const int* ids = ids_dev;
   int n = P.n;
   const FrontDev* fr = p->d_fronts;
+  if constexpr (std::is_same<T, double>::value) {
+    if (p->lu) {  // LU fronts: one workgroup per front throughout
+      if (mode == 0) {
+        for (int ell = L - 1; ell >= 0; ell--) {
+          g_level = ell;
+          launch(p, "k_solve_fwd", 0, 0, k_solve_fwd<double>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+                 (const double*)arena, (const int*)p->d_dofs, w, xs, (const int*)p->d_perm);
+          if (ell >= 1)
+            launch(p, "k_sep_acc", 0, 0, k_sep_acc<double>, dim3(A.s_cnt[ell]), ids + A.s_off[ell],
+                   (const SepDev*)p->d_seps, fr, (const int*)p->d_dofs, w, (const double*)xs);
+        }
+        for (int ell = 0; ell < L; ell++) {
+          g_level = ell;
+          launch(p, "k_lu_bwd", 0, 0, k_lu_bwd, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const double*)arena,
+                 (const int*)p->d_dofs, w, xs, (const double*)p->d_uinv);
+        }
+      } else {
+        for (int ell = L - 1; ell >= 0; ell--)
+          launch(p, "k_lu_apply_up", 0, 0, k_lu_apply_up, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+                 (const double*)arena, (const int*)p->d_dofs, w, xs, nr, (const int*)p->d_perm);
+        for (int ell = 1; ell < L; ell++) {
+          launch(p, "k_apply_down", 0, 0, k_apply_down<double>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
+                 (const double*)arena, xs, nr);
+          launch(p, "k_sep_acc", 0, 0, k_sep_acc<double>, dim3(A.s_cnt[ell]), ids + A.s_off[ell],
+                 (const SepDev*)p->d_seps, fr, (const int*)p->d_dofs, w, (const double*)xs);
+        }
+      }
+      g_level = -1;
+      HIPCHK(hipGetLastError());
+      return UMF_OK;
+    }
+  }
   // per level: solve flops nr*(2n^2 + 4nm) split evenly over the two sweeps; bytes = factor read
   auto work = [&](int ell, double& fl, double& by) {
     fl = by = 0;
@@ -846,7 +963,7 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       }
       if (cnt > i)
         launch(p, "k_solve_fwd", part(cnt - i, fl), part(cnt - i, by), k_solve_fwd<T>, dim3(cnt - i), li + i, fr,
-               (const T*)arena, (const int*)p->d_dofs, w, xs);
+               (const T*)arena, (const int*)p->d_dofs, w, xs, (const int*)nullptr);
     } else {
       int nbi, nbt;
       blocks(ell, nbi, nbt);
@@ -1122,6 +1239,7 @@ int solve_noalloc(umf_plan* p, const double* in, double* out, int nrhs, int ld) 
   if (!p->factored) return fail(UMF_ESTATE, "factor before solve/apply");
   if (p->fact_prec != UMF_PREC_FP64)
     return fail(UMF_ESTATE, "the no-alloc solve takes FP64 factors (the refined forms need work vectors of their own)");
+  if (p->lu) return fail(UMF_ESTATE, "the no-alloc solve takes FP64 Cholesky factors (not LU fronts)");
   const int n = p->P.n;
   if (nrhs < 0 || ld < n) return fail(UMF_EINVAL, "bad nrhs or ld");
   if (nrhs == 0) return UMF_OK;
@@ -1175,11 +1293,41 @@ int solve_noalloc(umf_plan* p, const double* in, double* out, int nrhs, int ld) 
   return UMF_OK;
 }
 
+// LU buffers of a plan whose programs write full matrices: per-front offsets of the inverted U
+// blocks and of the transposed U rows (in fronts_h), then the four allocations.
+int alloc_lu(umf_plan* p) {
+  const Plan& P = p->P;
+  i64 uo = 0, to = 0;
+  for (size_t f = 0; f < p->fronts_h.size(); f++) {
+    FrontDev& D = p->fronts_h[f];
+    D.uinv_off = uo;
+    uo += (i64)((D.n + NB - 1) / NB) * NB * NB;
+    D.ut_off = to;
+    to += ((i64)D.N * NB + 31) / 32 * 32;
+  }
+  const size_t nd = std::max<size_t>(P.dofs.size(), 1);
+  HIPCHK(hipMalloc((void**)&p->d_ipiv, sizeof(int) * nd));
+  HIPCHK(hipMalloc((void**)&p->d_perm, sizeof(int) * nd));
+  HIPCHK(hipMalloc((void**)&p->d_uinv, sizeof(double) * std::max<i64>(uo, 1)));
+  HIPCHK(hipMalloc((void**)&p->d_ut, sizeof(double) * std::max<i64>(to, 1)));
+  p->lu = true;
+  return UMF_OK;
+}
+
+void free_lu(umf_plan* p) {
+  for (void* q : {(void*)p->d_ipiv, (void*)p->d_perm, (void*)p->d_uinv, (void*)p->d_ut})
+    if (q) (void)hipFree(q);
+  p->d_ipiv = p->d_perm = nullptr;
+  p->d_uinv = p->d_ut = nullptr;
+  p->lu = false;
+}
+
 void free_plan(umf_plan* p) {
   void* ptrs[] = {p->arena, p->d_dofs, p->d_mats, p->d_fronts, p->d_seps, p->d_tri, p->d_tri_off, p->d_ops,
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
                   p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist,
-                  p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off, p->d_arrive};
+                  p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off, p->d_arrive, p->d_ipiv, p->d_perm,
+                  p->d_uinv, p->d_ut};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
@@ -1190,8 +1338,11 @@ void free_plan(umf_plan* p) {
   if (p->stream) (void)hipStreamDestroy(p->stream);
 }
 
-// Symmetric pattern, symmetric values, no duplicate entries (Cholesky fronts need all three).
-std::string check_operator(int n, const int* rp, const int* ci, const double* v) {
+// Symmetric pattern and no duplicate entries (every front layout needs both); symmetric values
+// (Cholesky fronts) -- or, with `unsym` non-null, asymmetric values are allowed and reported
+// there (LU fronts).
+std::string check_operator(int n, const int* rp, const int* ci, const double* v, bool* unsym = nullptr) {
+  if (unsym) *unsym = false;
   if (rp[0] != 0) return "row_ptr[0] must be 0";
   for (int i = 0; i < n; i++)
     if (rp[i + 1] < rp[i]) return "row_ptr must be non-decreasing";
@@ -1226,7 +1377,10 @@ std::string check_operator(int n, const int* rp, const int* ci, const double* v)
     for (size_t q = 0; q < row.size(); q++) {
       int k = cnt[i] + (int)q;
       if (trow[k] != row[q].first) return "pattern is not symmetric (row " + std::to_string(i) + ")";
-      if (v && tval[k] != row[q].second) return "values are not symmetric (row " + std::to_string(i) + ")";
+      if (v && tval[k] != row[q].second) {
+        if (!unsym) return "values are not symmetric (row " + std::to_string(i) + ")";
+        *unsym = true;
+      }
     }
   }
   return "";
@@ -1248,11 +1402,12 @@ int umf_plan_create(int dim, int nx, int ny, int nz, int width, int max_side_len
   if (!out || !row_ptr || !col_ind || !val) return fail(UMF_EINVAL, "null argument");
   *out = nullptr;
   if (n <= 0) return fail(UMF_EINVAL, "n must be positive");
-  std::string e = check_operator(n, row_ptr, col_ind, val);
+  bool unsym = false;  // asymmetric values: LU fronts, as the reference factors every front
+  std::string e = check_operator(n, row_ptr, col_ind, val, &unsym);
   if (!e.empty()) return fail(UMF_EINVAL, e);
   umf_plan* p = new umf_plan();
   bool geom = false;
-  e = build_plan(p->P, dim, nx, ny, nz, width, max_side_length, n, row_ptr, col_ind, geom);
+  e = build_plan(p->P, dim, nx, ny, nz, width, max_side_length, n, row_ptr, col_ind, geom, unsym);
   if (!e.empty()) {
     delete p;
     return fail(geom ? UMF_EGEOM : UMF_EINVAL, e);
@@ -1328,6 +1483,15 @@ int setup_device(umf_plan* p, umf_plan** out) {
                                 " GB: " + hipGetErrorString(he));
   UP(p->d_dofs, P.dofs);
   UP(p->d_mats, mats);
+  if (P.full) {
+    const int rc = alloc_lu(p);
+    if (rc) {
+      const std::string msg = g_err;
+      free_plan(p);
+      delete p;
+      return fail(rc, msg);
+    }
+  }
   UP(p->d_fronts, p->fronts_h);
   UP(p->d_seps, p->seps_h);
   UP(p->d_tri, P.triples);
@@ -1503,6 +1667,10 @@ int umf_plan_save(umf_plan* p, const char* path) {
   h.ir_tol = p->ir_tol;
   h.arena_elems = P.arena_elems;
   h.elem_size = !p->factored ? 0 : p->fact_prec != UMF_PREC_FP64 ? 4 : 8;
+  if (p->lu) {  // LU factors (pivots, inverted U blocks) are not stored: loaded plans refactor
+    h.factored = 0;
+    h.elem_size = 0;
+  }
   HIPCHK(hipStreamSynchronize(p->stream));
   std::vector<double> val((size_t)h.nnz);
   if (h.nnz) HIPCHK(hipMemcpy(val.data(), p->d_val, sizeof(double) * h.nnz, hipMemcpyDeviceToHost));
@@ -1595,7 +1763,7 @@ int umf_factor(umf_plan* p) {
   p->factored = false;
   p->fact_prec = p->prec;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
-  int rc = p->use_graph && !p->prof.on ? factor_graph(p, D) : factor_any(p, D);
+  int rc = p->use_graph && !p->prof.on && !p->lu ? factor_graph(p, D) : factor_any(p, D);
   if (rc) return rc;
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -1663,8 +1831,12 @@ int umf_update(umf_plan* p, const int* row_ptr, const int* col_ind, const double
     if (row_ptr[i] != P.rp[i]) return fail(UMF_EPATTERN, "row_ptr differs from the planned operator");
   for (size_t k = 0; k < P.ci.size(); k++)
     if (col_ind[k] != P.ci[k]) return fail(UMF_EPATTERN, "col_ind differs from the planned operator");
-  std::string e = check_operator(n, row_ptr, col_ind, val);
+  bool unsym = false;
+  std::string e = check_operator(n, row_ptr, col_ind, val, &unsym);
   if (!e.empty()) return fail(UMF_EINVAL, e);
+  if (unsym && !p->lu)
+    return fail(UMF_EINVAL, "values are not symmetric: this plan factors with Cholesky (umf_set_lu switches it to LU "
+                            "fronts)");
   p->val_h.assign(val, val + row_ptr[n]);
   return do_update(p, p->val_h.data(), false, n_mod, mod_dofs);
 }
@@ -1842,7 +2014,73 @@ int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out,
   auto at = [&](int i, int j) { return h[(size_t)j * N + i]; };
   std::vector<double> r;
   int R = 0, Cc = 0;
-  if (which == UMF_BLOCK_L || which == UMF_BLOCK_A22) {
+  if (p->lu) {  // LU front: L\U of P A22, U12, L21, A11 - L21 U12 (full)
+    const FrontInfo& FI = p->P.fronts[f];
+    std::vector<int> piv(n), pi(n);
+    if (n) HIPCHK(hipMemcpy(piv.data(), p->d_ipiv + FI.int_off, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; j++) pi[j] = j;
+    for (int j = 0; j < n; j++) std::swap(pi[j], pi[piv[j]]);  // (P b)[j] = b[pi[j]]
+    auto Lf = [&](int i, int k) { return i == k ? 1.0 : (i > k ? at(i, k) : 0.0); };
+    if (which == UMF_BLOCK_L) {
+      R = Cc = n;
+      r.assign((size_t)n * n, 0.0);
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) r[(size_t)j * n + i] = at(i, j);
+    } else if (which == UMF_BLOCK_A22) {  // A22 = P^T L U
+      R = Cc = n;
+      r.assign((size_t)n * n, 0.0);
+      for (int c = 0; c < n; c++)
+        for (int j = 0; j < n; j++) {
+          double sum = 0;
+          for (int k = 0; k <= std::min(j, c); k++) sum += Lf(j, k) * at(k, c);
+          r[(size_t)c * n + pi[j]] = sum;
+        }
+    } else if (which == UMF_BLOCK_XR) {  // XR = U^{-1} U12
+      R = n;
+      Cc = m;
+      r.assign((size_t)n * m, 0.0);
+      for (int c = 0; c < m; c++) {
+        double* x = &r[(size_t)c * n];
+        for (int i = 0; i < n; i++) x[i] = at(i, n + c);
+        for (int j = n - 1; j >= 0; j--) {
+          x[j] /= at(j, j);
+          for (int i = 0; i < j; i++) x[i] -= at(i, j) * x[j];
+        }
+      }
+    } else if (which == UMF_BLOCK_XL) {  // XL = L21 L^{-1} P
+      R = m;
+      Cc = n;
+      r.assign((size_t)m * n, 0.0);
+      std::vector<double> y(n);
+      for (int i = 0; i < m; i++) {
+        for (int c = n - 1; c >= 0; c--) {  // y L = L21(i, :)
+          double v = at(n + i, c);
+          for (int k = c + 1; k < n; k++) v -= y[k] * Lf(k, c);
+          y[c] = v;
+        }
+        for (int j = 0; j < n; j++) r[(size_t)pi[j] * m + i] = y[j];
+      }
+    } else if (which == UMF_BLOCK_S) {  // S = -L21 U12
+      R = Cc = m;
+      r.assign((size_t)m * m, 0.0);
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++) {
+          double sum = 0;
+          for (int k = 0; k < n; k++) sum += at(n + i, k) * at(k, n + j);
+          r[(size_t)j * m + i] = -sum;
+        }
+    } else if (which == UMF_BLOCK_A11POST) {
+      R = Cc = m;
+      r.assign((size_t)m * m, 0.0);
+      std::vector<int> run(m, -1);
+      for (int q = 0; q < F.nsep; q++)
+        for (int i = 0; i < F.sep_len[q]; i++) run[F.sep_pos[q] + i] = q;
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++) r[(size_t)j * m + i] = run[i] == run[j] ? NAN : at(n + i, n + j);
+    } else {
+      return fail(UMF_EINVAL, "unknown block selector");
+    }
+  } else if (which == UMF_BLOCK_L || which == UMF_BLOCK_A22) {
     R = Cc = n;
     std::vector<double> Lm((size_t)n * n, 0.0);
     for (int j = 0; j < n; j++)
@@ -1918,6 +2156,10 @@ int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
   std::vector<double> h;
   int rc = read_pool(p, S.off, (size_t)S.k * S.k, h);
   if (rc) return rc;
+  if (p->lu) {  // full matrix
+    std::copy(h.begin(), h.end(), out);
+    return UMF_OK;
+  }
   for (int j = 0; j < S.k; j++)
     for (int i = j; i < S.k; i++) {
       out[(size_t)j * S.k + i] = h[(size_t)j * S.k + i];
@@ -1939,6 +2181,7 @@ int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3)
     return fail(UMF_EINVAL, "unknown precision");
   if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
+  if (p->lu && precision != UMF_PREC_FP64) return fail(UMF_EINVAL, "LU fronts (unsymmetric operators) are FP64 only");
   if (precision != p->prec) p->factored = false;
   p->prec = precision;
   p->ir_tol = tol;
@@ -2015,6 +2258,55 @@ int umf_set_graph(umf_plan* p, int on) {
   p->use_graph = on != 0;
   return UMF_OK;
 }
+
+int umf_set_lu(umf_plan* p, int on) {
+  if (!p) return fail(UMF_EINVAL, "null plan");
+  if (p->P.forest) return fail(UMF_EINVAL, "batches factor with Cholesky fronts");
+  if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
+  const bool want = on != 0;
+  if (want == p->lu) return UMF_OK;
+  if (want && p->prec != UMF_PREC_FP64) return fail(UMF_EINVAL, "LU fronts (unsymmetric operators) are FP64 only");
+  if (!want && !p->val_h.empty()) {
+    bool unsym = false;
+    const std::string e = check_operator(p->P.n, p->P.rp.data(), p->P.ci.data(), p->val_h.data(), &unsym);
+    if (!e.empty() || unsym) return fail(UMF_EINVAL, "the operator's values are not symmetric: it needs LU fronts");
+  }
+  if (!p->host_only) HIPCHK(hipStreamSynchronize(p->stream));
+  Plan& P = p->P;
+  build_programs(P, want);
+  p->lu = want;
+  if (p->host_only) return UMF_OK;
+  for (void** q : {(void**)&p->d_tri, (void**)&p->d_tri_off, (void**)&p->d_ops, (void**)&p->d_tiles,
+                   (void**)&p->d_tile_off, (void**)&p->d_fly_tab, (void**)&p->d_fly_tri, (void**)&p->d_rest_tri,
+                   (void**)&p->d_rest_off}) {
+    if (*q) HIPCHK(hipFree(*q));
+    *q = nullptr;
+  }
+  HIPCHK(upload(&p->d_tri, P.triples));
+  HIPCHK(upload(&p->d_tri_off, P.tri_off));
+  HIPCHK(upload(&p->d_ops, P.ops));
+  HIPCHK(upload(&p->d_tiles, P.tiles));
+  HIPCHK(upload(&p->d_tile_off, P.tile_off));
+  if (!P.rest_off.empty()) {
+    HIPCHK(upload(&p->d_fly_tab, P.fly_tab));
+    HIPCHK(upload(&p->d_fly_tri, P.fly_tri));
+    HIPCHK(upload(&p->d_rest_tri, P.rest_tri));
+    HIPCHK(upload(&p->d_rest_off, P.rest_off));
+  }
+  for (size_t f = 0; f < p->fronts_h.size(); f++) p->fronts_h[f].fly_off = P.fly_off.empty() ? -1 : P.fly_off[f];
+  free_lu(p);
+  if (want) {
+    const int rc = alloc_lu(p);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(p->d_fronts, p->fronts_h.data(), sizeof(FrontDev) * p->fronts_h.size(), hipMemcpyHostToDevice));
+  if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
+  p->fgraph = nullptr;
+  p->factored = false;
+  return UMF_OK;
+}
+
+int umf_get_lu(const umf_plan* p) { return p ? (p->lu ? 1 : 0) : fail(UMF_EINVAL, "null plan"); }
 
 int umf_set_sweep_sync(umf_plan* p, int fenced) {
   if (!p) return fail(UMF_EINVAL, "null plan");
@@ -2233,6 +2525,7 @@ int umf_server_create(umf_plan* base, int nslots, double pool_bytes, umf_server*
   *out = nullptr;
   if (base->host_only || !base->factored) return fail(UMF_ESTATE, "the base plan must be factored");
   if (base->P.forest) return fail(UMF_EINVAL, "serving takes a single-problem plan");
+  if (base->lu) return fail(UMF_EINVAL, "serving takes Cholesky fronts (symmetric operators)");
   if (nslots < 1 || nslots > 32) return fail(UMF_EINVAL, "slots must be 1..32");
   if (!(pool_bytes >= 0)) return fail(UMF_EINVAL, "bad pool size");
   HIPCHK(hipStreamSynchronize(base->stream));

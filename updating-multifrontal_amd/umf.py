@@ -4,7 +4,8 @@
 (include/DE/mf.hpp:67-80, :173-188): ``Factor()``, ``Update(A, modifiedDOFs)``, ``Solve(b)``,
 ``Apply(x)``.  ``Solve``/``Apply`` also accept an n x nrhs array.  Errors follow the
 reference: geometry problems raise RuntimeError with the reference's message; a non-SPD pivot
-raises ArithmeticError (the reference asserts on LAPACK info != 0).
+(Cholesky fronts) or a zero pivot (LU fronts, for operators with asymmetric values) raises
+ArithmeticError (the reference asserts on LAPACK info != 0).
 
 The compute path is the HIP library only: if libumf_hip.so is missing this module raises at
 import time rather than falling back to anything else.
@@ -87,6 +88,8 @@ _stats = _sig("umf_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 _set_prec = _sig("umf_set_precision", C.c_int, [_vp, C.c_int, C.c_double, C.c_int])
 _set_graph = _sig("umf_set_graph", C.c_int, [_vp, C.c_int])
 _set_sweep_sync = _sig("umf_set_sweep_sync", C.c_int, [_vp, C.c_int])
+_set_lu = _sig("umf_set_lu", C.c_int, [_vp, C.c_int])
+_get_lu = _sig("umf_get_lu", C.c_int, [_vp])
 _set_tune = _sig("umf_set_tune", C.c_int, [_vp, C.c_int, C.c_int])
 _get_tune = _sig("umf_get_tune", C.c_int, [_vp, _ip, _ip])
 _autotune = _sig("umf_autotune", C.c_int, [_vp, C.c_int, _dp])
@@ -110,7 +113,7 @@ EXPORTED = ["umf_plan_create", "umf_plan_analyze", "umf_analyze_update", "umf_pl
             "umf_solve_device", "umf_solve_device_noalloc", "umf_apply_device", "umf_spmv_device", "umf_last_error", "umf_levels",
             "umf_leaf_width", "umf_num_boxes", "umf_num_seps", "umf_box_dofs", "umf_sep_dofs", "umf_dirty",
             "umf_get_front_block", "umf_get_sep_block", "umf_get_stats", "umf_stream", "umf_set_precision", "umf_set_graph",
-            "umf_set_tune", "umf_get_tune", "umf_autotune", "umf_set_sweep_sync",
+            "umf_set_tune", "umf_get_tune", "umf_autotune", "umf_set_sweep_sync", "umf_set_lu", "umf_get_lu",
             "umf_batch_create", "umf_batch_info", "umf_box_scale_device", "umf_plan_save", "umf_plan_load",
             "umf_server_create", "umf_server_destroy", "umf_server_footprint", "umf_server_admit",
             "umf_server_release", "umf_server_pool", "umf_server_set_values", "umf_server_box_scale",
@@ -196,6 +199,17 @@ class _MF:
         """Run Factor() as one captured HIP graph (captured at the next Factor, then replayed;
         bit-identical to the direct launches)."""
         _check(_set_graph(self._h, 1 if on else 0))
+
+    def set_lu(self, on=True):
+        """LU fronts (partial pivoting within each front's interior rows, as the reference
+        factors every front) instead of Cholesky; an operator with asymmetric values gets them
+        at construction.  Factor() again afterwards."""
+        _check(_set_lu(self._h, 1 if on else 0))
+
+    @property
+    def lu(self):
+        """True when the fronts are factored by LU (unsymmetric operators)."""
+        return _get_lu(self._h) == 1
 
     def sweep_sync(self, fenced):
         """Blocked solve sweeps: publish blocks with agent-scope release / acquire fences
