@@ -202,9 +202,11 @@ def test_update_pattern_mismatch_rejected():
         G.Update(B, [0])
 
 
-@pytest.mark.parametrize("args", [["2", "127", "1", "8"], ["3", "15", "1", "4"], ["2", "63", "1", "4"]])
+@pytest.mark.parametrize("args", [["2", "127", "1", "8"], ["3", "15", "1", "4"], ["2", "63", "1", "4"],
+                                  ["2", "63", "2", "5"], ["2", "61", "3", "6"], ["3", "15", "2", "3"]])
 def test_dropin_cpp_driver(args):
-    """The reference's test flow (testMF2D/testMF3D) through the drop-in C++ headers."""
+    """The reference's test flow (testMF2D/testMF3D) through the drop-in C++ headers; widths 2-3
+    with a random field are unsymmetric operators on LU fronts."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "updating-multifrontal_amd",

@@ -914,7 +914,8 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
   // Levels of few (<= 16) fronts, or of at most 128 fronts of 2048+ rows, are swept by 64-row
   // blocks, one workgroup each (k_fwd_blk / k_bwd_blk): one workgroup per front would leave
   // most of the 256 CUs idle while it streams its factor (3D 127^3: 64 fronts of 8557 rows at
-  // 0.7 TB/s).  Levels of many fronts keep one workgroup per front.
+  // 0.7 TB/s).  Levels of many fronts keep one workgroup per front (blocks on the 3D 127^3 leaf
+  // level, 512 fronts of 4725 rows, measured slower: FP32 sweeps 40.6 -> 47.8 ms per solve).
   std::vector<int> maxN(L, 0);
   for (int ell = 0; ell < L; ell++)
     for (int f : P.box_gid[ell]) maxN[ell] = std::max(maxN[ell], P.fronts[f].n + P.fronts[f].m);

@@ -85,6 +85,12 @@ class Front {
     return Y;
   }
 
+  // Front factorization: LU (partial pivoting within each front's interior rows, as the
+  // reference's every front) or Cholesky.  Operators with asymmetric values get LU fronts at
+  // construction; SetLU(true) makes a symmetric (e.g. indefinite) operator use them too.
+  void SetLU(bool on) { check(umf_set_lu(plan_, on ? 1 : 0)); }
+  bool IsLU() const { return umf_get_lu(plan_) == 1; }
+
   // Plan files (no reference counterpart): geometry, current operator values and, when
   // factored, the factors; the file constructors of MF2D / MF3D restore them without
   // refactoring (umf_plan_save / umf_plan_load).
