@@ -3,11 +3,14 @@
 // Every kernel works on a batch: a list of front (or separator) ids of one tree level, one
 // front per blockIdx.y (or blockIdx.x), so a whole level runs in one launch per stage.
 //   assembly   k_zero_lower, k_assemble, k_sep_add          (HBM-bound gathers / scatters)
-//   factor     k_chol_diag, k_chol_panel, k_gemm<0>, k_gemm<1>   (blocked Cholesky of the
-//              pivot columns, left- or right-looking, then the Schur update; MFMA)
-//   solve      k_solve_fwd, k_solve_bwd, k_sep_acc           (multi-RHS level sweeps)
+//   factor     k_chol_diag, k_chol_dpanel, k_chol_panel, k_gemm_group, k_gemm_trail,
+//              k_gemm_schur (blocked Cholesky of the pivot columns, left- or right-looking,
+//              then the Schur update; MFMA)
+//   LU fronts  k_zero_full, k_lu_panel, k_lu_swap, k_lu_trsm, k_lu_gemm, k_lu_perm,
+//              k_sep_add_full, k_lu_bwd, k_lu_apply_up (unsymmetric operators)
+//   solve      k_solve_fwd(_r), k_solve_bwd, k_fwd_blk, k_bwd_blk, k_sep_acc (multi-RHS sweeps)
 //   apply      k_apply_up, k_apply_down
-//   residual   k_spmv
+//   residual   k_spmv, k_spmv_rows, k_dots, k_absmax, k_rowabs, k_cg_* (FP64 refinement)
 //   operator   k_box_scale, k_lap_rows, k_lap_collect (coefficient-change requests)
 #include <hip/hip_runtime.h>
 
@@ -514,7 +517,7 @@ __device__ __forceinline__ bool wg_tile(int xt, int nf, int& tile, int& f) {
 //   k_chol_panel: rows below the block: X = (A(rows, jb block) - L(rows, 0:K) L(jb.., 0:K)^T)
 //                 Linv_jj^T, i.e. the panel TRSM as a product with the inverted diagonal block.
 //                 Left-looking (K = jb): every tile is formed once, from the untouched input.
-//                 Right-looking (K = 0): the trailing region was updated by k_gemm<0>.
+//                 Right-looking (K = 0): the trailing region was updated by k_gemm_trail.
 // A non-positive (or NaN) pivot records the first failing (front, column) in err[0..2].
 // acc += P Q^T for 16 x 16 blocks held in LDS, one wave, one MFMA per 4 k (K = 16):
 // P(i,k) = P[i*psi + k*psk], Q(j,k) = Q[j*qsj + k*qsk]; accumulator r of a lane is
@@ -1083,9 +1086,9 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
 // SPLIT = false: the native MFMA of T; true (T = float): the product on BF16 pieces
 // (tile_gemm_nt_bf).
-template <class T, int MODE, int TS = 64, bool SPLIT = false>
-__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                              T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
+template <class T, int MODE, int TS, bool SPLIT>
+__device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                          T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
   static_assert(!SPLIT || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
@@ -1174,6 +1177,28 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm(const int* __re
           }
         }
   }
+}
+
+// The three GEMM stages as kernels of their own names (the profiles and the runtime's timing
+// name the same launches): the group update (K = 64 or 256 columns of a 256-column group), the
+// trailing update (K = 256 / the outer group width), and the Schur complement (K = n).
+template <class T, int TS = 64, bool SPLIT = false>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_group(const int* __restrict__ ids,
+                                                                   const FrontDev* __restrict__ fr, T* __restrict__ A,
+                                                                   int k0, int kw, int ccap, int xt, int nf) {
+  gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
+}
+template <class T, int TS = 64, bool SPLIT = false>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_trail(const int* __restrict__ ids,
+                                                                   const FrontDev* __restrict__ fr, T* __restrict__ A,
+                                                                   int k0, int kw, int ccap, int xt, int nf) {
+  gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
+}
+template <class T, int TS = 64, bool SPLIT = false>
+__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_schur(const int* __restrict__ ids,
+                                                                   const FrontDev* __restrict__ fr, T* __restrict__ A,
+                                                                   int k0, int kw, int ccap, int xt, int nf) {
+  gemm_body<T, 1, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
 
 

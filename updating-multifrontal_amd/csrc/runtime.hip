@@ -299,12 +299,19 @@ inline int tiles_of(int s, int rend, int cend, int ts = 64) {
 
 // k_gemm instance of a stage: the native MFMA of T, or for FP32 fronts with `split` the
 // product on BF16 pieces
-template <class T, int MODE, int TS>
+// STAGE 0 = group update, 1 = trailing update, 2 = Schur complement
+template <class T, int STAGE, int TS>
 auto gemm_fn(bool split) {
+  auto pick = [](auto sp) {
+    constexpr bool S = decltype(sp)::value;
+    if constexpr (STAGE == 0) return k_gemm_group<T, TS, S>;
+    else if constexpr (STAGE == 1) return k_gemm_trail<T, TS, S>;
+    else return k_gemm_schur<T, TS, S>;
+  };
   if constexpr (std::is_same<T, float>::value) {
-    if (split) return k_gemm<T, MODE, TS, true>;
+    if (split) return pick(std::true_type{});
   }
-  return k_gemm<T, MODE, TS, false>;
+  return pick(std::false_type{});
 }
 
 // Workgroups per item (front, separator) of a level stage whose largest item can use `want`
@@ -573,10 +580,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
       auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
         if (wide(t128, act))
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 128>(split), dim3(t128, act), ids + L.f_off,
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(split), dim3(t128, act), ids + L.f_off,
                  (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
         else
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
                  ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
       };
       if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
@@ -613,10 +620,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
         if (tiles > 0) {
           if (wide(tiles128, c))
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 128>(split), grid(tiles128), ids + L.f_off + i,
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(split), grid(tiles128), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
           else
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 1, 64>(split), grid(tiles), ids + L.f_off + i,
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(split), grid(tiles), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
         }
         i = e;
