@@ -457,6 +457,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 2D secondary workloads")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-graph", action="store_true", help="skip the Factor() direct-vs-graph comparison")
     ap.add_argument("--ir-tol", type=float, default=1e-10, help="backward error the refined forms refine to")
     ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)  # child of the CPU baseline
     ap.add_argument("--cpu-only", default=None, choices=sorted(CPU_SAMPLES),
@@ -523,7 +524,7 @@ def main():
                           keep_x64=(f == "fp64"))
     main_ = res[value_form]
     plan.SetPrecision(value_form, tol=args.ir_tol, max_iter=100)
-    fgraph = factor_graph_timing(plan, torch)
+    fgraph = None if args.no_graph else factor_graph_timing(plan, torch)
     out = {
         "metric": METRIC,
         "value": main_["value"],

@@ -10,7 +10,7 @@
 //              k_sep_add_full, k_lu_bwd, k_lu_apply_up (unsymmetric operators)
 //   solve      k_solve_fwd(_r), k_solve_bwd, k_fwd_blk, k_bwd_blk, k_sep_acc (multi-RHS sweeps)
 //   apply      k_apply_up, k_apply_down
-//   residual   k_spmv, k_spmv_rows, k_dots, k_absmax, k_rowabs, k_cg_* (FP64 refinement)
+//   residual   k_spmv, k_spmv_dot, k_pc_in/out, k_cg_*, k_red_final, k_rowabs (FP64 refinement)
 //   operator   k_box_scale, k_lap_rows, k_lap_collect (coefficient-change requests)
 #include <hip/hip_runtime.h>
 
@@ -777,6 +777,107 @@ __global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ id
         sm.d.Ts[j * LD + i] = v;
       }
   diag_core(sm.d, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
+}
+
+// Small fronts assembled from the operator alone (no child blocks: leaf fronts) with one pivot
+// block (1 <= n <= 64) and m <= LEAF_M boundary rows, one workgroup per front, the whole
+// factorization in LDS: the pivot block and the panel rows are scattered from the front's CSR
+// triples, the block is factored with its inverse (diag_core: L11 to the front, Linv to its
+// dinv block), the panel becomes L21 = A12 Linv^T, and the boundary block, scattered into the
+// freed diagonal buffer, takes its Schur complement S = A11 - L21 L21^T.  The front is written
+// once (lower triangle: L11, L21, S) and never read: the zero / assemble / diagonal / panel /
+// Schur launches of the general path, with their HBM round trips of the front, collapse into
+// one.  Which fronts take this kernel is a property of the plan, so their arithmetic does not
+// depend on the pass that factors them.
+#define LEAF_M 48
+#define LDP (LEAF_M + 1)
+template <class T>
+__global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                              T* __restrict__ A, const i64* __restrict__ tri_off,
+                                              const Triple* __restrict__ tri, const double* __restrict__ csr_val,
+                                              int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  __shared__ DiagSmem<T> sd;
+  __shared__ T Pn[NB * LDP];  // Pn[c*LDP + r]: A12(r, c), then L21(r, c); r < m, c < 64
+  T* Ts = sd.Ts;
+  T* U = Ts + LD;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, c16 = lane & 15, q4 = lane >> 4;
+  const int n = F.n, m = F.m;
+  T* a = A + F.off;
+  // pivot block: strict upper triangle zero, identity beyond n; panel rows zero
+  for (int e = t; e < LD * LD; e += 256) {
+    const int c = e / LD, r = e - c * LD;
+    Ts[e] = (r == c && r >= n && r < NB) ? T(1) : T(0);
+  }
+  for (int e = t; e < NB * LDP; e += 256) Pn[e] = T(0);
+  __syncthreads();
+  const i64 t0 = tri_off[fid], t1 = tri_off[fid + 1];
+  for (i64 i = t0 + t; i < t1; i += 256) {
+    const Triple x = tri[i];
+    if (x.col >= n) continue;  // the boundary block comes later, the A21 block is not used
+    if (x.row < n) {
+      if (x.row >= x.col) Ts[x.col * LD + x.row] = (T)csr_val[x.k];
+    } else {
+      Pn[x.col * LDP + (x.row - n)] = (T)csr_val[x.k];
+    }
+  }
+  __syncthreads();
+  diag_core(sd, a, F.N, n, A + F.dinv_off, err, fid, 0);
+  __syncthreads();
+  if (m == 0) return;
+  const int nk = (n + 15) >> 4;  // 16-column blocks of the pivot columns
+  // panel: wave w forms rows [16w, 16w + 16) of L21 = A12 Linv^T (Linv(r, c) at U[r*LD + c])
+  if (16 * w < m) {
+    V4<T> acc[4];
+#pragma unroll
+    for (int cj = 0; cj < 4; cj++) {
+      acc[cj] = vzero<T>();
+      if (cj >= nk) continue;
+#pragma unroll
+      for (int kb = 0; kb <= cj; kb++) {
+        if (kb == cj)
+          mma16<0, 1>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
+        else
+          mma16<0, 0>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
+      }
+    }
+    const int r = 16 * w + c16;
+#pragma unroll
+    for (int cj = 0; cj < 4; cj++)
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) {
+        const int c = 16 * cj + mrow<T>(q4, qq);
+        const T v = c < n ? acc[cj][qq] : T(0);
+        Pn[c * LDP + r] = v;  // this wave's rows only: no other wave reads them before the barrier
+        if (c < n && r < m) a[(i64)c * F.N + n + r] = v;
+      }
+  }
+  __syncthreads();
+  // boundary block A11 (lower triangle) into the diagonal buffer
+  for (int e = t; e < LD * LD; e += 256) Ts[e] = T(0);
+  __syncthreads();
+  for (i64 i = t0 + t; i < t1; i += 256) {
+    const Triple x = tri[i];
+    if (x.col >= n && x.row >= x.col) Ts[(x.col - n) * LD + (x.row - n)] = (T)csr_val[x.k];
+  }
+  __syncthreads();
+  // S = A11 - L21 L21^T, lower 16 x 16 tiles dealt round-robin to the waves
+  const int mt = (m + 15) >> 4;
+  for (int tl = w; tl < mt * (mt + 1) / 2; tl += 4) {
+    int bi = 0;
+    while ((bi + 1) * (bi + 2) / 2 <= tl) bi++;
+    const int bj = tl - bi * (bi + 1) / 2;
+    V4<T> acc = vzero<T>();
+    for (int kb = 0; kb < nk; kb++)
+      mma16(acc, Pn + 16 * kb * LDP + 16 * bi, 1, LDP, Pn + 16 * kb * LDP + 16 * bj, 1, LDP);
+    const int r = 16 * bi + c16;
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) {
+      const int c = 16 * bj + mrow<T>(q4, qq);
+      if (r < m && c <= r) a[(i64)(n + c) * F.N + n + r] = Ts[c * LD + r] - acc[qq];
+    }
+  }
 }
 
 // Right-looking levels: the diagonal block's factorization and the panel below it in one
@@ -2159,87 +2260,196 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
 
 // ----------------------------------------------------------------------------- refinement
 // FP64 kernels of the preconditioned conjugate-gradient refinement that turns the FP32 factor
-// into an FP64-accurate solve.  Vectors are row-major n x 16 blocks (v[i*16 + r]).
-// y = A x, 16 lanes per row (one right-hand side each): the 16 x reads of a nonzero are one
-// contiguous 128-byte segment.
-__global__ __launch_bounds__(256) void k_spmv_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                   const double* __restrict__ v, const double* __restrict__ x,
-                                                   double* __restrict__ y) {
-  const int i = blockIdx.x * 16 + (threadIdx.x >> 4), r = threadIdx.x & 15;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = rp[i]; k < rp[i + 1]; k++) s += v[k] * x[(i64)ci[k] * RB + r];
-  y[(i64)i * RB + r] = s;
+// into an FP64-accurate solve.  Its vectors are row-major n x cw blocks (v[i*cw + c]) with cw
+// the smallest power of two >= the number of right-hand sides (1..16): a solve of 4 columns
+// moves 4 columns, not 16.  The FP32 preconditioner's work block stays n x 16 (the sweeps' MFMA
+// width); only its first cw columns are written and read here.
+//
+// Reductions (dot products, maxima) run over NCH fixed chunks of rows: chunk q holds rows
+// [q*ch, (q+1)*ch), ch = ceil(n / NCH); inside a chunk, row lane rl = 0..15 sums rows i with
+// (i - q*ch) % 16 == rl in ascending order, the chunk's value is the sum over rl in order, and
+// the result the sum over chunks in a fixed order (k_red_final).  The order depends on n alone,
+// not on cw or the grid: a column's result is the same whatever other columns share the solve.
+#define NCH 8192
+struct ChunkLane {
+  int c, rl, g, q;  // column, row lane, chunk within the workgroup, chunk
+  i64 i0, i1;       // rows of the chunk
+};
+// Workgroup of 256 threads = 16 / cw chunks x 16 row lanes x cw columns (t = (g*16 + rl)*cw + c);
+// grid NCH * cw / 16.
+__device__ __forceinline__ ChunkLane chunk_lane(int n, int lcw) {
+  ChunkLane L;
+  const int t = threadIdx.x;
+  L.c = t & ((1 << lcw) - 1);
+  L.rl = (t >> lcw) & 15;
+  L.g = t >> (lcw + 4);
+  L.q = blockIdx.x * (16 >> lcw) + L.g;
+  const i64 ch = ((i64)n + NCH - 1) / NCH;
+  L.i0 = min((i64)n, (i64)L.q * ch);
+  L.i1 = min((i64)n, L.i0 + ch);
+  return L;
 }
-
-// Per-column dot products of one or two pairs, first pass: block g sums rows
-// [g*chunk, (g+1)*chunk) in a fixed order into part[g*32 + 16*pair + r].
-__global__ __launch_bounds__(256) void k_dots(int n, int npair, const double* __restrict__ a0,
-                                              const double* __restrict__ b0, const double* __restrict__ a1,
-                                              const double* __restrict__ b1, double* __restrict__ part) {
-  const int r = threadIdx.x & 15, rl = threadIdx.x >> 4;
-  const int chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-  double s0 = 0.0, s1 = 0.0;
-  for (int i = i0 + rl; i < i1; i += 16) {
-    const i64 e = (i64)i * RB + r;
-    s0 += a0[e] * b0[e];
-    if (npair > 1) s1 += a1[e] * b1[e];
-  }
-  __shared__ double red[2][256];
-  red[0][threadIdx.x] = s0;
-  red[1][threadIdx.x] = s1;
+// Sum (MAX = false) or NaN-keeping max (MAX = true) of v over the row lanes of each (chunk,
+// column) of the workgroup, rl in order, into part[(pair*NCH + q)*16 + c].
+template <bool MAX>
+__device__ __forceinline__ void chunk_reduce(double v, int lcw, int pair, double* __restrict__ red,
+                                             double* __restrict__ part) {
+  red[threadIdx.x] = v;
   __syncthreads();
-  if (threadIdx.x < 32) {
-    const int pr = threadIdx.x >> 4, c = threadIdx.x & 15;
-    double t = 0.0;
-    for (int q = 0; q < 16; q++) t += red[pr][q * 16 + c];
-    part[(i64)blockIdx.x * 32 + threadIdx.x] = t;
+  const int cw = 1 << lcw, per = 16 >> lcw;  // chunks per workgroup
+  if ((int)threadIdx.x < per * cw) {
+    const int g = threadIdx.x >> lcw, c = threadIdx.x & (cw - 1);
+    double s = red[(g * 16) * cw + c];
+    for (int rl = 1; rl < 16; rl++) {
+      const double x = red[(g * 16 + rl) * cw + c];
+      s = MAX ? ((x > s || x != x) ? x : s) : s + x;
+    }
+    part[((i64)pair * NCH + blockIdx.x * per + g) * 16 + c] = s;
   }
+  __syncthreads();
 }
-// Second pass: out[c] = sum_g part[g*32 + c], c < 32, in a fixed order (8 interleaved partial
-// sums per column, then combined).
-__global__ __launch_bounds__(256) void k_dots_final(int nblk, const double* __restrict__ part,
-                                                    double* __restrict__ out) {
-  const int c = threadIdx.x & 31, s8 = threadIdx.x >> 5;
+__device__ __forceinline__ double nanmax(double m, double v) { return (v > m || v != v) ? v : m; }
+
+// Second pass: out[pair*16 + c] = sum (or max) over the NCH chunks of part, c < cw (8 interleaved
+// partial results per column, then combined in order); out for c >= cw is zero.
+template <bool MAX>
+__global__ __launch_bounds__(256) void k_red_final(int npair, int cw, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  const int o = threadIdx.x & 31, pr = o >> 4, c = o & 15, s8 = threadIdx.x >> 5;
   double s = 0.0;
-  for (int g = s8; g < nblk; g += 8) s += part[(i64)g * 32 + c];
+  if (pr < npair && c < cw)
+    for (int q = s8; q < NCH; q += 8) {
+      const double x = part[((i64)pr * NCH + q) * 16 + c];
+      s = MAX ? nanmax(s, x) : s + x;
+    }
   __shared__ double red[256];
   red[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x < 32) {
-    double t = 0.0;
-    for (int q = 0; q < 8; q++) t += red[q * 32 + c];
-    out[c] = t;
+    double t = red[threadIdx.x];
+    for (int k = 1; k < 8; k++) t = MAX ? nanmax(t, red[k * 32 + threadIdx.x]) : t + red[k * 32 + threadIdx.x];
+    if (pr < npair) out[o] = t;
   }
 }
-// Max that keeps a NaN (a NaN residual must not read as converged).
-__device__ __forceinline__ double nanmax(double m, double v) { return (v > m || v != v) ? v : m; }
-// Per-column max |a| of one or two row-major n x 16 blocks, first pass: block g covers rows
-// [g*chunk, (g+1)*chunk) into part[g*32 + 16*pair + r].  (Max is order-independent: the result
-// does not depend on the grid.)
-__global__ __launch_bounds__(256) void k_absmax(int n, int npair, const double* __restrict__ a0,
-                                                const double* __restrict__ a1, double* __restrict__ part) {
-  const int r = threadIdx.x & 15, rl = threadIdx.x >> 4;
-  const int chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-  double m0 = 0.0, m1 = 0.0;
-  for (int i = i0 + rl; i < i1; i += 16) {
-    const i64 e = (i64)i * RB + r;
-    m0 = nanmax(m0, fabs(a0[e]));
-    if (npair > 1) m1 = nanmax(m1, fabs(a1[e]));
+
+// Start of the refinement: B = R = the right-hand sides (column-major in, columns >= nr zero),
+// X = 0; part (pair 0) <- max |b| per column.
+__global__ __launch_bounds__(256) void k_cg_init(int n, int lcw, int nr, const double* __restrict__ in, int ld,
+                                                 double* __restrict__ Bv, double* __restrict__ R,
+                                                 double* __restrict__ X, double* __restrict__ part) {
+  __shared__ double red[256];
+  const ChunkLane L = chunk_lane(n, lcw);
+  const int cw = 1 << lcw;
+  double mx = 0.0;
+  for (i64 i = L.i0 + L.rl; i < L.i1; i += 16) {
+    const double v = L.c < nr ? in[(i64)L.c * ld + i] : 0.0;
+    const i64 e = i * cw + L.c;
+    Bv[e] = v;
+    R[e] = v;
+    X[e] = 0.0;
+    mx = nanmax(mx, fabs(v));
   }
-  __shared__ double red[2][256];
-  red[0][threadIdx.x] = m0;
-  red[1][threadIdx.x] = m1;
-  __syncthreads();
-  if (threadIdx.x < 32) {
-    const int pr = threadIdx.x >> 4, c = threadIdx.x & 15;
-    double t = 0.0;
-    for (int q = 0; q < 16; q++) t = nanmax(t, red[pr][q * 16 + c]);
-    part[(i64)blockIdx.x * 32 + threadIdx.x] = t;
-  }
+  chunk_reduce<true>(mx, lcw, 0, red, part);
 }
+
+// w (the preconditioner's n x 16 FP32 work block) <- (float) r, columns < cw.
+__global__ __launch_bounds__(256) void k_pc_in(int n, int lcw, const double* __restrict__ R, float* __restrict__ w) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ((i64)n << lcw)) return;
+  w[(e >> lcw) * RB + (e & ((1 << lcw) - 1))] = (float)R[e];
+}
+
+// z = (double) w after the preconditioner (P = z as well when P is given, the first
+// direction); part (pair 0) <- r.z per column.
+__global__ __launch_bounds__(256) void k_pc_out(int n, int lcw, const float* __restrict__ w,
+                                                const double* __restrict__ R, double* __restrict__ Z,
+                                                double* __restrict__ P, double* __restrict__ part) {
+  __shared__ double red[256];
+  const ChunkLane L = chunk_lane(n, lcw);
+  const int cw = 1 << lcw;
+  double s = 0.0;
+  for (i64 i = L.i0 + L.rl; i < L.i1; i += 16) {
+    const i64 e = i * cw + L.c;
+    const double z = (double)w[i * RB + L.c];
+    Z[e] = z;
+    if (P) P[e] = z;
+    s += R[e] * z;
+  }
+  chunk_reduce<false>(s, lcw, 0, red, part);
+}
+
+// q = A p (the CSR operator, one thread per row and column); part (pair 0) <- p.q per column.
+__global__ __launch_bounds__(256) void k_spmv_dot(int n, int lcw, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const double* __restrict__ v,
+                                                  const double* __restrict__ P, double* __restrict__ Q,
+                                                  double* __restrict__ part) {
+  __shared__ double red[256];
+  const ChunkLane L = chunk_lane(n, lcw);
+  const int cw = 1 << lcw;
+  double s = 0.0;
+  for (i64 i = L.i0 + L.rl; i < L.i1; i += 16) {
+    double y = 0.0;
+    const int k1 = rp[i + 1];
+    for (int k = rp[i]; k < k1; k++) y += v[k] * P[(i64)ci[k] * cw + L.c];
+    const i64 e = i * cw + L.c;
+    Q[e] = y;
+    s += P[e] * y;
+  }
+  chunk_reduce<false>(s, lcw, 0, red, part);
+}
+
+// x += a p, r -= a q with a = rz / pq per column (columns outside `active`, a bit mask, stay as
+// they are); w <- (float) r, the next preconditioner input; part <- max |r| (pair 0) and
+// max |x| (pair 1) per column.
+__global__ __launch_bounds__(256) void k_cg_xr(int n, int lcw, const double* __restrict__ rz,
+                                               const double* __restrict__ pq, int active,
+                                               const double* __restrict__ P, const double* __restrict__ Q,
+                                               double* __restrict__ X, double* __restrict__ R,
+                                               float* __restrict__ w, double* __restrict__ part) {
+  __shared__ double red[256];
+  const ChunkLane L = chunk_lane(n, lcw);
+  const int cw = 1 << lcw;
+  const bool on = (active >> L.c) & 1;
+  const double a = on && pq[L.c] != 0.0 ? rz[L.c] / pq[L.c] : 0.0;
+  double mr = 0.0, mx = 0.0;
+  for (i64 i = L.i0 + L.rl; i < L.i1; i += 16) {
+    const i64 e = i * cw + L.c;
+    double x = X[e], r = R[e];
+    if (on) {
+      x += a * P[e];
+      r -= a * Q[e];
+      X[e] = x;
+      R[e] = r;
+    }
+    w[i * RB + L.c] = (float)r;
+    mr = nanmax(mr, fabs(r));
+    mx = nanmax(mx, fabs(x));
+  }
+  chunk_reduce<true>(mr, lcw, 0, red, part);
+  chunk_reduce<true>(mx, lcw, 1, red, part);
+}
+
+// p = z + b p with b = rz_new / rz per column, active columns.
+__global__ __launch_bounds__(256) void k_cg_p(int n, int lcw, const double* __restrict__ rz_new,
+                                              const double* __restrict__ rz, int active,
+                                              const double* __restrict__ Z, double* __restrict__ P) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ((i64)n << lcw)) return;
+  const int c = (int)(e & ((1 << lcw) - 1));
+  if (!((active >> c) & 1)) return;
+  const double b = rz[c] != 0.0 ? rz_new[c] / rz[c] : 0.0;
+  P[e] = Z[e] + b * P[e];
+}
+
+// out (column-major, ld) <- X, columns < nr.
+__global__ __launch_bounds__(256) void k_cg_out(int n, int lcw, int nr, const double* __restrict__ X,
+                                                double* __restrict__ out, int ld) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ((i64)n << lcw)) return;
+  const int c = (int)(e & ((1 << lcw) - 1));
+  if (c < nr) out[(i64)c * ld + (e >> lcw)] = X[e];
+}
+
 // ||A||_inf of a CSR operator, first pass: part[g*32] = max over block g's rows of sum_j |a_ij|.
 __global__ __launch_bounds__(256) void k_rowabs(int n, const int* __restrict__ rp, const double* __restrict__ v,
                                                 double* __restrict__ part) {
@@ -2260,7 +2470,7 @@ __global__ __launch_bounds__(256) void k_rowabs(int n, const int* __restrict__ r
   }
   if (threadIdx.x < 32) part[blockIdx.x * 32 + threadIdx.x] = threadIdx.x == 0 ? red[0] : 0.0;
 }
-// Second pass of the maxima: out[c] = max_g part[g*32 + c], c < 32.
+// Second pass of k_rowabs: out[c] = max_g part[g*32 + c], c < 32.
 __global__ __launch_bounds__(256) void k_max_final(int nblk, const double* __restrict__ part,
                                                    double* __restrict__ out) {
   const int c = threadIdx.x & 31, s8 = threadIdx.x >> 5;
@@ -2274,30 +2484,6 @@ __global__ __launch_bounds__(256) void k_max_final(int nblk, const double* __res
     for (int q = 0; q < 8; q++) t = nanmax(t, red[q * 32 + c]);
     out[c] = t;
   }
-}
-// x += a p, r -= a q with a = rz / pq per column; columns outside `active` (bit mask) frozen.
-__global__ __launch_bounds__(256) void k_cg_xr(i64 n16, const double* __restrict__ rz, const double* __restrict__ pq,
-                                               int active, const double* __restrict__ p,
-                                               const double* __restrict__ q, double* __restrict__ x,
-                                               double* __restrict__ r) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= n16) return;
-  const int c = (int)(e & 15);
-  if (!((active >> c) & 1)) return;
-  const double a = pq[c] != 0.0 ? rz[c] / pq[c] : 0.0;
-  x[e] += a * p[e];
-  r[e] -= a * q[e];
-}
-// p = z + b p with b = rz_new / rz per column.
-__global__ __launch_bounds__(256) void k_cg_p(i64 n16, const double* __restrict__ rz_new,
-                                              const double* __restrict__ rz, int active,
-                                              const double* __restrict__ z, double* __restrict__ p) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= n16) return;
-  const int c = (int)(e & 15);
-  if (!((active >> c) & 1)) return;
-  const double b = rz[c] != 0.0 ? rz_new[c] / rz[c] : 0.0;
-  p[e] = z[e] + b * p[e];
 }
 
 // ----------------------------------------------------------------------------- operator values

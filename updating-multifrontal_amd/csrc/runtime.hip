@@ -73,6 +73,30 @@ struct Prof {
   }
 };
 
+namespace umf {
+// Schedule of one factor/update pass: per level, the front ids to (re)factor, the separator ids
+// to rebuild and the separator ids receiving Schur contributions, all uploaded in one buffer.
+struct Sched {
+  struct Lvl {
+    int f_off, f_cnt;      // fronts of the general path (front ids == matrix ids)
+    int lf_off, lf_cnt;    // fronts of the one-launch small-front kernel (leaf_front)
+    int s_off, s_cnt;      // separators (separator ids)
+    int sm_off, sm_cnt;    // separator matrix ids (NF + id) for zero/assemble
+    int maxN, maxn, maxm;  // extents over the level's fronts
+    int max_k;
+    int f_grid, s_grid;    // assembly grid widths (tiles / triples per matrix)
+  };
+  std::vector<Lvl> lv;
+  std::vector<int> ids;
+};
+
+// Level lists of all fronts / separators (solve and apply touch every front).
+struct AllLists {
+  std::vector<int> ids;
+  std::vector<int> f_off, f_cnt, s_off, s_cnt;
+};
+}  // namespace umf
+
 struct umf_plan {
   // the planner's output; serving slots share their base plan's (umf_server_create)
   std::shared_ptr<Plan> Pp = std::make_shared<Plan>();
@@ -139,6 +163,14 @@ struct umf_plan {
   int fgraph_prec = -1;
   int* d_fids = nullptr;
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
+  std::vector<std::vector<int>> gen_fronts;  // per level: fronts on the general factor path
+  // the sweeps' level lists (fixed by the plan): built and uploaded once, then every sweep
+  // launches from them without a host rebuild or a wait for the stream
+  std::unique_ptr<umf::AllLists> sweep_lists;
+  int* d_sids = nullptr;
+  // the full factorization's schedule (fixed by the plan), its id list in d_fids
+  std::unique_ptr<umf::Sched> full_sched;
+  std::unique_ptr<Plan::Dirty> full_dirty;
   double* d_xs = nullptr;  // solve scratch: sum_f N_f x RB
   double* d_w = nullptr;   // solve work block n x RB, row-major
   // FP32 form: its own front pool and solve buffers, the FP64 refinement vectors
@@ -149,8 +181,8 @@ struct umf_plan {
   float* arena32 = nullptr;
   float* d_xs32 = nullptr;
   float* d_w32 = nullptr;
-  double* d_cg = nullptr;    // 5 row-major n x RB blocks: B, X, R, P, Q/Z
-  double* d_part = nullptr;  // reduction partials (DOT_BLOCKS x 32)
+  double* d_cg = nullptr;    // 5 blocks of n x RB doubles: B, X, R, P, Q/Z (n x cw row-major each)
+  double* d_part = nullptr;  // reduction partials (2 x NCH x 16)
   double* d_dots = nullptr;  // 4 x 32 reduction results
   double norm_a = 0;         // ||A||_inf of the current operator
   double* d_stage = nullptr;  // device copy of host right-hand sides
@@ -169,25 +201,20 @@ struct umf_plan {
 
 namespace {
 
-// Schedule of one factor/update pass: per level, the front ids to (re)factor, the separator ids
-// to rebuild and the separator ids receiving Schur contributions, all uploaded in one buffer.
-struct Sched {
-  struct Lvl {
-    int f_off, f_cnt;      // fronts (front ids == matrix ids)
-    int s_off, s_cnt;      // separators (separator ids)
-    int sm_off, sm_cnt;    // separator matrix ids (NF + id) for zero/assemble
-    int maxN, maxn, maxm;  // extents over the level's fronts
-    int max_k;
-    int f_grid, s_grid;    // assembly grid widths (tiles / triples per matrix)
-  };
-  std::vector<Lvl> lv;
-  std::vector<int> ids;
-};
+// Fronts factored whole by k_leaf: assembled from the operator alone (no child blocks), one
+// pivot block, at most LEAF_M boundary rows.  A property of the plan, never of the pass.
+inline bool leaf_front(const Plan& P, int f) {
+  const FrontInfo& F = P.fronts[f];
+  return F.n >= 1 && F.n <= NB && F.m <= LEAF_M && P.tile_off[f + 1] == P.tile_off[f];
+}
 
-Sched make_sched(const Plan& P, const Plan::Dirty& D) {
+// small: split off the leaf_front fronts (Cholesky schedules); LU schedules keep every front on
+// the general path.
+Sched make_sched(const Plan& P, const Plan::Dirty& D, bool small = false) {
   Sched S;
   S.lv.assign(P.L, {});
   int NF = P.nfronts();
+  std::vector<int> leaves;
   auto grid_of = [&](int mid) {
     i64 tiles = P.tile_off[mid + 1] - P.tile_off[mid];
     i64 tri = P.tri_off[mid + 1] - P.tri_off[mid];
@@ -198,8 +225,13 @@ Sched make_sched(const Plan& P, const Plan::Dirty& D) {
     L.f_off = (int)S.ids.size();
     L.maxN = L.maxn = L.maxm = 0;
     L.f_grid = L.s_grid = 1;
+    leaves.clear();
     for (int b : D.boxes[ell]) {
       int f = P.box_gid[ell][b];
+      if (small && leaf_front(P, f)) {
+        leaves.push_back(f);
+        continue;
+      }
       S.ids.push_back(f);
       L.maxN = std::max(L.maxN, P.fronts[f].n + P.fronts[f].m);
       L.maxn = std::max(L.maxn, P.fronts[f].n);
@@ -211,6 +243,9 @@ Sched make_sched(const Plan& P, const Plan::Dirty& D) {
     // its fronts (those with n > k0), not over every front of the level
     std::stable_sort(S.ids.begin() + L.f_off, S.ids.end(),
                      [&](int a, int b) { return P.fronts[a].n > P.fronts[b].n; });
+    L.lf_off = (int)S.ids.size();
+    L.lf_cnt = (int)leaves.size();
+    S.ids.insert(S.ids.end(), leaves.begin(), leaves.end());
     L.s_off = (int)S.ids.size();
     L.max_k = 0;
     if (ell >= 1)
@@ -320,13 +355,29 @@ auto gemm_fn(bool split) {
 // tens of thousands of small fronts and one large one then launch one workgroup per front.
 inline int per_item(int want, int cnt) { return std::max(1, std::min(want, (32768 + cnt - 1) / std::max(cnt, 1))); }
 
+// The fronts of level ell that take the general path (not leaf_front), cached per plan: the
+// level-wide algorithm choices of the factorization are made over them.
+const std::vector<int>& general_fronts(umf_plan* p, int ell) {
+  const Plan& P = p->P;
+  if ((int)p->gen_fronts.size() != P.L) {
+    p->gen_fronts.assign(P.L, {});
+    for (int l = 0; l < P.L; l++)
+      for (int f : P.box_gid[l])
+        if (!leaf_front(P, f)) p->gen_fronts[l].push_back(f);
+  }
+  return p->gen_fronts[ell];
+}
+
 // Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
 // ids_pre: the schedule's id list already on the device (a factorization being captured into a
 // graph: no synchronization, allocation or copy may enter the stream).
 template <class T>
-int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const int* ids_pre = nullptr) {
+int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const int* ids_pre = nullptr,
+               const Sched* S_pre = nullptr) {
   const Plan& P = p->P;
-  Sched S = make_sched(P, D);
+  Sched S_own;
+  if (!S_pre) S_own = make_sched(P, D, true);
+  const Sched& S = S_pre ? *S_pre : S_own;
   const int* ids = ids_pre;
   if (!ids) {
     int rc = ensure_ids(p, S.ids.size());
@@ -375,7 +426,19 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
              (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
              (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
     }
-    if (L.f_cnt == 0) continue;
+    // 2'. small leaf fronts: assembly, factorization and Schur complement in one launch
+    if (L.lf_cnt > 0) {
+      double fl = 0, by = 0;
+      for (int i = 0; i < L.lf_cnt; i++) {
+        const int f = S.ids[L.lf_off + i];
+        const double n = P.fronts[f].n, m = P.fronts[f].m;
+        fl += 2.0 * n * n * n / 3.0 + m * n * n + n * m * (m + 1);  // factor + inverse, panel, Schur
+        by += sizeof(T) * (lowerN(n) + m * n + lowerN(m) + NB * NB) + 20.0 * (P.tri_off[f + 1] - P.tri_off[f]);
+      }
+      launch(p, "k_leaf", fl, by, k_leaf<T>, dim3(L.lf_cnt), ids + L.lf_off, (const FrontDev*)p->d_fronts, arena,
+             (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const double*)p->d_val, p->d_err);
+    }
+    if (L.f_cnt > 0) [&] {
     // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
     //    64 at a time.  Levels with many fronts go left-looking (each block column is formed
     //    once by an MFMA GEMM over the finished columns, then factored; no trailing
@@ -388,7 +451,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     // level, never from the fronts this pass happens to factor: an Update then computes every
     // dirty front exactly as a full factorization of the new operator would, bit for bit, and a
     // front's arithmetic does not depend on what else shares its launches.
-    const std::vector<int>& level_fronts = P.box_gid[ell];
+    const std::vector<int>& level_fronts = general_fronts(p, ell);
     const bool left = level_fronts.size() >= 32;
     // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
     // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
@@ -629,6 +692,7 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         i = e;
       }
     }
+    }();
     // 5. separators of this level receive both neighbours' Schur blocks
     if (L.s_cnt > 0) {
       double by = 0;
@@ -720,14 +784,32 @@ int run_factor_lu(umf_plan* p, const Plan::Dirty& D) {
   return UMF_OK;
 }
 
-int factor_any(umf_plan* p, const Plan::Dirty& D, const int* ids_pre = nullptr) {
+// S_pre: the pass's schedule when the caller holds it (the full factorization's, whose id list
+// ids_pre is already on the device).
+int factor_any(umf_plan* p, const Plan::Dirty& D, const int* ids_pre = nullptr, const Sched* S_pre = nullptr) {
   if (p->lu) return run_factor_lu(p, D);
   if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
-    return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3, ids_pre);
+    return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3, ids_pre, S_pre);
   }
-  return run_factor<double>(p, D, p->arena, false, ids_pre);
+  return run_factor<double>(p, D, p->arena, false, ids_pre, S_pre);
+}
+
+// The full factorization's dirty set and Cholesky schedule, built once per plan, with its id
+// list uploaded once to d_fids (read, never written, by every later full factorization).
+int ensure_full_sched(umf_plan* p) {
+  if (p->full_sched) return UMF_OK;
+  std::unique_ptr<Plan::Dirty> D(new Plan::Dirty(p->P.all_dirty()));
+  std::unique_ptr<Sched> S(new Sched(make_sched(p->P, *D, true)));
+  if (p->d_fids) (void)hipFree(p->d_fids);
+  p->d_fids = nullptr;
+  HIPCHK(hipMalloc((void**)&p->d_fids, sizeof(int) * std::max<size_t>(S->ids.size(), 1)));
+  if (!S->ids.empty())
+    HIPCHK(hipMemcpyAsync(p->d_fids, S->ids.data(), sizeof(int) * S->ids.size(), hipMemcpyHostToDevice, p->stream));
+  p->full_dirty = std::move(D);
+  p->full_sched = std::move(S);
+  return UMF_OK;
 }
 
 // The full factorization as one HIP graph launch: captured on first use (and again after a
@@ -738,16 +820,13 @@ int factor_graph(umf_plan* p, const Plan::Dirty& D) {
   if (!p->fgraph || p->fgraph_prec != p->prec) {
     if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
     p->fgraph = nullptr;
-    Sched S = make_sched(p->P, D);
+    int rs = ensure_full_sched(p);  // the Cholesky schedule run_factor replays
+    if (rs) return rs;
     HIPCHK(hipStreamSynchronize(p->stream));
-    if (p->d_fids) (void)hipFree(p->d_fids);
-    p->d_fids = nullptr;
-    HIPCHK(hipMalloc((void**)&p->d_fids, sizeof(int) * std::max<size_t>(S.ids.size(), 1)));
-    if (!S.ids.empty()) HIPCHK(hipMemcpy(p->d_fids, S.ids.data(), sizeof(int) * S.ids.size(), hipMemcpyHostToDevice));
     if (p->prec != UMF_PREC_FP64 && !p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
     HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = factor_any(p, D, p->d_fids);
+    const int rc = factor_any(p, D, p->d_fids, p->full_sched.get());
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(p->stream, &g);
     if (rc || ec != hipSuccess || !g) {
@@ -817,18 +896,16 @@ int ensure_solve_scratch(umf_plan* p) {
     return UMF_OK;
   }
   if (!p->d_xs32) HIPCHK(hipMalloc((void**)&p->d_xs32, sizeof(float) * nx));
-  if (!p->d_w32) HIPCHK(hipMalloc((void**)&p->d_w32, sizeof(float) * nw));
+  if (!p->d_w32) {  // zeroed once: columns a narrower refinement never writes hold finite values
+    HIPCHK(hipMalloc((void**)&p->d_w32, sizeof(float) * nw));
+    HIPCHK(hipMemsetAsync(p->d_w32, 0, sizeof(float) * nw, p->stream));
+  }
   if (!p->d_cg) HIPCHK(hipMalloc((void**)&p->d_cg, sizeof(double) * 5 * nw));
-  if (!p->d_part) HIPCHK(hipMalloc((void**)&p->d_part, sizeof(double) * DOT_BLOCKS * 32));
+  if (!p->d_part) HIPCHK(hipMalloc((void**)&p->d_part, sizeof(double) * std::max(DOT_BLOCKS * 32, 2 * NCH * 16)));
   if (!p->d_dots) HIPCHK(hipMalloc((void**)&p->d_dots, sizeof(double) * 5 * 32));
   return UMF_OK;
 }
 
-// Level lists of all fronts / separators (solve and apply touch every front).
-struct AllLists {
-  std::vector<int> ids;
-  std::vector<int> f_off, f_cnt, s_off, s_cnt;
-};
 AllLists all_lists(const Plan& P) {
   AllLists A;
   int L = P.L;
@@ -849,6 +926,18 @@ AllLists all_lists(const Plan& P) {
   return A;
 }
 
+// The plan's sweep lists on the device, built and uploaded on first use.  They outlive every
+// launch that reads them, so later sweeps neither rebuild nor wait.
+int ensure_sweep_lists(umf_plan* p) {
+  if (!p->sweep_lists) p->sweep_lists.reset(new AllLists(all_lists(p->P)));
+  if (p->d_sids) return UMF_OK;
+  const AllLists& A = *p->sweep_lists;
+  HIPCHK(hipMalloc((void**)&p->d_sids, sizeof(int) * std::max<size_t>(A.ids.size(), 1)));
+  if (!A.ids.empty())
+    HIPCHK(hipMemcpyAsync(p->d_sids, A.ids.data(), sizeof(int) * A.ids.size(), hipMemcpyHostToDevice, p->stream));
+  return UMF_OK;
+}
+
 // Epoch of the next blocked sweep launch: flags still holding an earlier epoch read as unset.
 int next_epoch(umf_plan* p) {
   p->epoch = p->epoch == 0x7fffffff ? 1 : p->epoch + 1;
@@ -863,16 +952,18 @@ template <class T>
 int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = nullptr) {
   const Plan& P = p->P;
   int L = P.L;
-  AllLists A = all_lists(P);
+  if (!p->sweep_lists) p->sweep_lists.reset(new AllLists(all_lists(P)));
+  const AllLists& A = *p->sweep_lists;
   if (!ids_dev) {
-    int rc = ensure_ids(p, A.ids.size());
+    int rc = ensure_sweep_lists(p);
     if (rc) return rc;
-    ids_dev = p->d_ids;
+    ids_dev = p->d_sids;
+  } else {  // caller's scratch (the no-alloc solve): the previous list may still be in flight
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->ids_h = A.ids;
+    HIPCHK(hipMemcpyAsync(ids_dev, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                          p->stream));
   }
-  HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
-  p->ids_h = A.ids;
-  HIPCHK(hipMemcpyAsync(ids_dev, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
-                        p->stream));
   const int* ids = ids_dev;
   int n = P.n;
   const FrontDev* fr = p->d_fronts;
@@ -1017,68 +1108,49 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
   return UMF_OK;
 }
 
-// Per-column dot products of one or two pairs of row-major n x RB blocks into out[0:32]
-// (pair 0 in out[0:16], pair 1 in out[16:32]); deterministic two-pass reduction.
-void dots(umf_plan* p, const double* a0, const double* b0, const double* a1, const double* b1, double* out) {
-  const int n = p->P.n;
-  launch(p, "k_dots", 0, 16.0 * RB * n * (a1 ? 2 : 1), k_dots, dim3(DOT_BLOCKS), n, a1 ? 2 : 1, a0, b0,
-         a1 ? a1 : a0, b1 ? b1 : b0, p->d_part);
-  launch(p, "k_dots_final", 0, 0, k_dots_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, out);
-}
-
-// Per-column max |.| of one or two row-major n x RB blocks into out[0:16] (and out[16:32]).
-void absmax(umf_plan* p, const double* a0, const double* a1, double* out) {
-  const int n = p->P.n;
-  launch(p, "k_absmax", 0, 8.0 * RB * n * (a1 ? 2 : 1), k_absmax, dim3(DOT_BLOCKS), n, a1 ? 2 : 1, a0, a1 ? a1 : a0,
-         p->d_part);
-  launch(p, "k_max_final", 0, 0, k_max_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, out);
-}
-
-// FP32 form: z = M^{-1} r with M the FP32 multifrontal factorization (r, z FP64 row-major).
-int precondition(umf_plan* p, const double* r, double* z) {
-  const i64 n16 = (i64)p->P.n * RB;
-  const unsigned g = (unsigned)((n16 + 255) / 256);
-  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<float, double>, dim3(g), n16, r, p->d_w32);
-  int rc = sweep<float>(p, 0, RB, p->arena32, p->d_w32, p->d_xs32);
-  if (rc) return rc;
-  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<double, float>, dim3(g), n16, (const float*)p->d_w32, z);
-  return UMF_OK;
-}
-
 // FP32 form of Solve for nr <= RB columns: preconditioned conjugate gradients in FP64 (FP64
 // operator and residuals, FP32 factor as preconditioner), x0 = 0, each column stopped once its
 // normwise backward error in the infinity norm, ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf),
-// is <= ir_tol.  Returns the iteration count and the largest final backward error through
-// it / berr.
+// is <= ir_tol.  The vectors are n x cw blocks, cw = the smallest power of two >= nr; every
+// reduction runs in an order fixed by n alone (k_red_final).  Per iteration: k_spmv_dot (q = A p,
+// p.q), k_cg_xr (x, r, the preconditioner's input, max |r| and |x|), the convergence check on the
+// host, then the FP32 sweep, k_pc_out (z, r.z) and k_cg_p.  Returns the iteration count and the
+// largest final backward error through it / berr.
 int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out, double& berr_out) {
   const int n = p->P.n;
-  const i64 n16 = (i64)n * RB;
-  const unsigned g16 = (unsigned)((n16 + 255) / 256), gsp = (unsigned)((n + 15) / 16);
+  int lcw = 0;
+  while ((1 << lcw) < nr) lcw++;
+  const int cw = 1 << lcw;
+  const i64 nc = (i64)n << lcw, nw = (i64)n * RB;
+  const unsigned ge = (unsigned)((nc + 255) / 256), gr = (unsigned)(NCH >> (4 - lcw));
   double* Bv = p->d_cg;
-  double* X = Bv + n16;
-  double* R = X + n16;
-  double* Pd = R + n16;
-  double* QZ = Pd + n16;
+  double* X = Bv + nw;
+  double* R = X + nw;
+  double* Pd = R + nw;
+  double* QZ = Pd + nw;
   double* S1 = p->d_dots + 32;  // p.q
   double* S2 = p->d_dots + 64;  // max |r| | max |x| per column
   double* rz = p->d_dots;       // r.z, current
   double* rzn = p->d_dots + 96; // r.z, next
-  // ||A||_inf of the operator the plan holds
+  float* w = p->d_w32;
+  const double vb = 8.0 * (double)nc;  // bytes of one n x cw FP64 vector
   const i64 nnz = (i64)p->P.ci.size();
+  // ||A||_inf of the operator the plan holds
   launch(p, "k_rowabs", 0, 12.0 * nnz, k_rowabs, dim3(DOT_BLOCKS), n, (const int*)p->d_rp, (const double*)p->d_val,
          p->d_part);
   launch(p, "k_max_final", 0, 0, k_max_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
-  launch(p, "k_to_rows", 0, 16.0 * n * nr, k_to_rows<double>, dim3((n + 63) / 64), n, in, ld, nr, Bv);
-  HIPCHK(hipMemsetAsync(X, 0, sizeof(double) * n16, p->stream));
-  HIPCHK(hipMemcpyAsync(R, Bv, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
-  absmax(p, Bv, nullptr, S2);
+  launch(p, "k_cg_init", 0, 4.0 * vb, k_cg_init, dim3(gr), n, lcw, nr, in, ld, Bv, R, X, p->d_part);
+  launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(1), 1, cw, (const double*)p->d_part, S2);
   double h[32], na;
   HIPCHK(hipMemcpyAsync(&na, p->d_dots + 128, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
-  int rc = precondition(p, R, QZ);
+  // z = M^{-1} r, p = z, r.z
+  launch(p, "k_pc_in", 0, 1.5 * vb, k_pc_in, dim3(ge), n, lcw, (const double*)R, w);
+  int rc = sweep<float>(p, 0, RB, p->arena32, w, p->d_xs32);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(Pd, QZ, sizeof(double) * n16, hipMemcpyDeviceToDevice, p->stream));
-  dots(p, R, QZ, nullptr, nullptr, rz);
+  launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ, Pd,
+         p->d_part);
+  launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, rz);
   HIPCHK(hipStreamSynchronize(p->stream));
   p->norm_a = na;
   double nb[RB];
@@ -1088,12 +1160,12 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   int it = 0;
   while (active && it < p->ir_max) {
     it++;
-    launch(p, "k_spmv_rows", 0, 0, k_spmv_rows, dim3(gsp), n, (const int*)p->d_rp, (const int*)p->d_ci,
-           (const double*)p->d_val, (const double*)Pd, QZ);
-    dots(p, Pd, QZ, nullptr, nullptr, S1);
-    launch(p, "k_cg_xr", 0, 0, k_cg_xr, dim3(g16), n16, (const double*)rz, (const double*)S1, active,
-           (const double*)Pd, (const double*)QZ, X, R);
-    absmax(p, R, X, S2);
+    launch(p, "k_spmv_dot", 0, 12.0 * nnz + 4.0 * (double)n + 3.0 * vb, k_spmv_dot, dim3(gr), n, lcw,
+           (const int*)p->d_rp, (const int*)p->d_ci, (const double*)p->d_val, (const double*)Pd, QZ, p->d_part);
+    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, S1);
+    launch(p, "k_cg_xr", 0, 6.5 * vb, k_cg_xr, dim3(gr), n, lcw, (const double*)rz, (const double*)S1, active,
+           (const double*)Pd, (const double*)QZ, X, R, w, p->d_part);
+    launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(1), 2, cw, (const double*)p->d_part, S2);
     HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     for (int c = 0; c < nr; c++) {
@@ -1103,15 +1175,16 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
       if (berr[c] <= p->ir_tol) active &= ~(1 << c);
     }
     if (!active) break;
-    rc = precondition(p, R, QZ);
+    rc = sweep<float>(p, 0, RB, p->arena32, w, p->d_xs32);  // w holds (float) r from k_cg_xr
     if (rc) return rc;
-    dots(p, R, QZ, nullptr, nullptr, rzn);
-    launch(p, "k_cg_p", 0, 0, k_cg_p, dim3(g16), n16, (const double*)rzn, (const double*)rz, active,
+    launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ,
+           (double*)nullptr, p->d_part);
+    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, rzn);
+    launch(p, "k_cg_p", 0, 3.0 * vb, k_cg_p, dim3(ge), n, lcw, (const double*)rzn, (const double*)rz, active,
            (const double*)QZ, Pd);
     std::swap(rz, rzn);
   }
-  launch(p, "k_from_rows", 0, 16.0 * n * nr, k_from_rows<double>, dim3((n + 63) / 64), n, (const double*)X, out, ld,
-         nr);
+  launch(p, "k_cg_out", 0, 2.0 * 8.0 * n * nr, k_cg_out, dim3(ge), n, lcw, nr, (const double*)X, out, ld);
   it_out = it;
   berr_out = 0;
   for (int c = 0; c < nr; c++) berr_out = std::max(berr_out, berr[c]);
@@ -1340,6 +1413,7 @@ void free_plan(umf_plan* p) {
     if (q) (void)hipFree(q);
   if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
   if (p->d_fids) (void)hipFree(p->d_fids);
+  if (p->d_sids) (void)hipFree(p->d_sids);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
@@ -1767,11 +1841,15 @@ int umf_factor(umf_plan* p) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (p->host_only) return fail(UMF_ESTATE, "plan was analyzed only (no device memory)");
   if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
-  Plan::Dirty D = p->P.all_dirty();
+  int rc = ensure_full_sched(p);
+  if (rc) return rc;
+  const Plan::Dirty& D = *p->full_dirty;
   p->factored = false;
   p->fact_prec = p->prec;
   HIPCHK(hipEventRecord(p->ev0, p->stream));
-  int rc = p->use_graph && !p->prof.on && !p->lu ? factor_graph(p, D) : factor_any(p, D);
+  if (p->lu) rc = factor_any(p, D);
+  else if (p->use_graph && !p->prof.on) rc = factor_graph(p, D);
+  else rc = factor_any(p, D, p->d_fids, p->full_sched.get());
   if (rc) return rc;
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -2310,6 +2388,12 @@ int umf_set_lu(umf_plan* p, int on) {
   HIPCHK(hipMemcpy(p->d_fronts, p->fronts_h.data(), sizeof(FrontDev) * p->fronts_h.size(), hipMemcpyHostToDevice));
   if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
   p->fgraph = nullptr;
+  // schedules made from the previous programs (assembly grids, the general-path fronts)
+  p->full_sched.reset();
+  p->full_dirty.reset();
+  p->gen_fronts.clear();
+  if (p->d_fids) (void)hipFree(p->d_fids);
+  p->d_fids = nullptr;
   p->factored = false;
   return UMF_OK;
 }
@@ -2516,7 +2600,8 @@ int make_slot(umf_plan* b, umf_plan** out) {
   SLOTCHK(hipMalloc((void**)&q->d_blist, sizeof(int) * (P.n + 1)));
 #undef SLOTCHK
   q->factored = true;
-  const int rc = ensure_solve_scratch(q);
+  int rc = ensure_solve_scratch(q);
+  if (!rc) rc = ensure_sweep_lists(q);
   if (rc) {
     free_slot(q);
     return rc;
