@@ -1310,6 +1310,7 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_schur(const int
 // of a lane is row mrow(lane/16, r), right-hand side lane%16.
 #define RB 16
 #define GU 4  // 16-row groups in flight per wave
+#define FG 2  // the same in k_solve_fwd (4 took 189 VGPRs in FP32: 2 waves per SIMD)
 
 // *p when ok, else 0, with the load unconditional: p is valid either way (the caller clamps the
 // index), so the loads of an unrolled loop are issued together instead of one branch and wait
@@ -1341,7 +1342,7 @@ __device__ __forceinline__ void gather_rhs(const FrontDev& F, T* __restrict__ X,
 // perm (LU fronts): the interior rows are gathered in pivot order, b_rd -> P b_rd, from the
 // front's permuted DOF list perm + int_off (L then is unit lower; dinv holds its inverted blocks).
 template <class T>
-__global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) void k_solve_fwd(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                    const T* __restrict__ A, const int* __restrict__ dofs,
                                                    T* __restrict__ W, T* __restrict__ xs,
                                                    const int* __restrict__ perm) {
@@ -1355,11 +1356,14 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
   for (int jb = 0; jb < F.n; jb += NB) {
     const int kb = min(NB, F.n - jb);
     __syncthreads();
-    {  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
+    {  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15; Linv is lower triangular, so the 4-column
+       // slices right of the wave's rows, and those past kb, hold only zeros and are skipped
       const T* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
       V4<T> acc = vzero<T>();
+      const int k4end = min(4 * (w + 1), (kb + 3) >> 2);
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) {
+        if (k4 >= k4end) break;
         const int k = 4 * k4 + q;
         const T av = Li[k * NB + 16 * w + c16];
         const T bv = ld_or0(X + (i64)(jb + min(k, kb - 1)) * RB + c16, k < kb);
@@ -1374,12 +1378,12 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
       }
     }
     __syncthreads();
-    // rows below, GU 16-row groups per wave in flight; out-of-range rows read row N-1 and are
+    // rows below, FG 16-row groups per wave in flight; out-of-range rows read row N-1 and are
     // never written
-    for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * GU) {
-      T av[GU][NB / 4], xv[GU][4];
+    for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * FG) {
+      T av[FG][NB / 4], xv[FG][4];
 #pragma unroll
-      for (int u = 0; u < GU; u++) {
+      for (int u = 0; u < FG; u++) {
         const int row = min(g0 + 64 * u + c16, F.N - 1);
 #pragma unroll
         for (int k4 = 0; k4 < NB / 4; k4++) {
@@ -1390,7 +1394,7 @@ __global__ __launch_bounds__(256) void k_solve_fwd(const int* __restrict__ ids, 
         for (int r = 0; r < 4; r++) xv[u][r] = X[(i64)min(g0 + 64 * u + mrow<T>(q, r), F.N - 1) * RB + c16];
       }
 #pragma unroll
-      for (int u = 0; u < GU; u++) {
+      for (int u = 0; u < FG; u++) {
         V4<T> acc = vzero<T>();
 #pragma unroll
         for (int k4 = 0; k4 < NB / 4; k4++)
@@ -1474,11 +1478,14 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
       zb[c][c16] = ld_or0(X + (i64)(jb + min(c, kb - 1)) * RB + c16, c < kb) - (c < kb ? acc[r] : T(0));
     }
     __syncthreads();
-    {  // x_j = Linv_j^T z: x(i, r) = sum_c Linv(c, i) z(c, r); wave w -> rows 16w..16w+15
+    {  // x_j = Linv_j^T z: x(i, r) = sum_c Linv(c, i) z(c, r); wave w -> rows 16w..16w+15; the
+       // 4-row slices above the wave's rows (Linv lower triangular) and past kb are zero: skipped
       const T* Li = A + F.dinv_off + (i64)bk * NB * NB;
       V4<T> a2 = vzero<T>();
+      const int k4end = (kb + 3) >> 2;
 #pragma unroll
       for (int k4 = 0; k4 < NB / 4; k4++) {
+        if (k4 < 4 * w || k4 >= k4end) continue;
         const int c = 4 * k4 + q;
         a2 = mfma4(Li[(16 * w + c16) * NB + c], zb[c][c16], a2);
       }
@@ -1490,6 +1497,173 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
     }
   }
   __syncthreads();
+#pragma unroll 4
+  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+}
+
+// ---- sweeps of small fronts (at most XLR rows) with the front's right-hand-side block in LDS
+// instead of the global scratch: the only global traffic is the factor, the work block W and
+// (forward) the boundary rows k_sep_acc reads; the steps synchronize with LDS-only barriers,
+// so the next diagonal block's inverse, loaded during a step, stays in flight across them.
+#define XLR 320  // rows of X held in LDS (2D leaf fronts of maxSideLength 8 have N = 285)
+
+// Workgroup barrier for LDS data: LDS writes before it are seen by LDS reads after it; global
+// loads in flight stay in flight (__syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Forward sweep (as k_solve_fwd) of fronts with N <= XLR.
+template <class T>
+__global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                     const T* __restrict__ A, const int* __restrict__ dofs,
+                                                     T* __restrict__ W, T* __restrict__ xs) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  const T* L = A + F.off;
+  const int* di = dofs + F.int_off;
+  const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
+  __shared__ T X[XLR * RB];
+  __shared__ T xb[NB][RB];
+#pragma unroll 4
+  for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
+  for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = T(0);
+  // the wave's slices of a diagonal block's inverse (lower triangular: slices right of the
+  // wave's rows and past kb are zero and not loaded)
+  T li[NB / 4];
+  auto load_li = [&](int jb) {
+    const T* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+    const int k4end = min(4 * (w + 1), (min(NB, F.n - jb) + 3) >> 2);
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++)
+      if (k4 < k4end) li[k4] = Li[(4 * k4 + q) * NB + 16 * w + c16];
+  };
+  load_li(0);
+  for (int jb = 0; jb < F.n; jb += NB) {
+    const int kb = min(NB, F.n - jb);
+    const int k4end = min(4 * (w + 1), (kb + 3) >> 2);
+    lds_barrier();
+    V4<T> acc = vzero<T>();  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      if (k4 >= k4end) break;
+      const int k = 4 * k4 + q;
+      acc = mfma4(li[k4], k < kb ? X[(jb + k) * RB + c16] : T(0), acc);
+    }
+    if (jb + NB < F.n) load_li(jb + NB);
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 16 * w + mrow<T>(q, r);
+      xb[i][c16] = acc[r];
+      if (i < kb) X[(jb + i) * RB + c16] = acc[r];
+    }
+    lds_barrier();
+    // rows below: X(i) -= L(i, block) x_j, one 16-row group per wave at a time (the inverse's
+    // slices in flight take the registers of a second)
+    constexpr int G = 1;
+    for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * G) {
+      T av[G][NB / 4];
+#pragma unroll
+      for (int u = 0; u < G; u++) {
+        const int row = min(g0 + 64 * u + c16, F.N - 1);
+#pragma unroll
+        for (int k4 = 0; k4 < NB / 4; k4++) {
+          const int k = 4 * k4 + q;
+          av[u][k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + row, k < kb);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; u++) {
+        V4<T> a2 = vzero<T>();
+#pragma unroll
+        for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(av[u][k4], xb[4 * k4 + q][c16], a2);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = g0 + 64 * u + mrow<T>(q, r);
+          if (i < F.N) X[i * RB + c16] -= a2[r];
+        }
+      }
+    }
+  }
+  lds_barrier();
+  T* Xg = xs + F.x_off;  // boundary rows for k_sep_acc
+#pragma unroll 4
+  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+  for (int e = F.n * RB + t; e < F.N * RB; e += 256) Xg[e] = X[e];
+}
+
+// Backward sweep (as k_solve_bwd) of fronts with N <= XLR.
+template <class T>
+__global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                     const T* __restrict__ A, const int* __restrict__ dofs,
+                                                     T* __restrict__ W) {
+  const FrontDev F = fr[ids[blockIdx.x]];
+  const T* L = A + F.off;
+  const int* di = dofs + F.int_off;
+  const int* db = dofs + F.bd_off;
+  const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
+  __shared__ T X[XLR * RB];
+  __shared__ T zb[NB][RB];
+#pragma unroll 4
+  for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
+#pragma unroll 4
+  for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = W[(i64)db[(e >> 4) - F.n] * RB + (e & 15)];
+  const int nblk = (F.n + NB - 1) / NB;
+  for (int bk = nblk - 1; bk >= 0; bk--) {
+    const int jb = bk * NB, kb = min(NB, F.n - jb);
+    const int k4lo = 4 * w, k4end = (kb + 3) >> 2;
+    // the wave's slices of Linv_j^T (rows 16w.., columns c >= 16w), loaded first: in flight
+    // during the product below
+    const T* Li = A + F.dinv_off + (i64)bk * NB * NB;
+    T li[NB / 4];
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++)
+      if (k4 >= k4lo && k4 < k4end) li[k4] = Li[(16 * w + c16) * NB + 4 * k4 + q];
+    lds_barrier();
+    // T(c, r) = sum_{i >= jb+kb} L(i, jb+c) X(i, r); wave w owns columns [16w, 16w+16)
+    const int col = 16 * w + c16;
+    const T* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
+    const bool cin = col < kb;
+    V4<T> acc = vzero<T>();
+    for (int i0 = jb + kb; i0 < F.N; i0 += 16 * GU) {
+      T av[GU][4], bv[GU][4];
+#pragma unroll
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) {
+          const int i = i0 + 16 * u + 4 * k4 + q;
+          const int ic = min(i, F.N - 1);
+          av[u][k4] = ld_or0(Lc + ic, cin && i < F.N);
+          bv[u][k4] = X[ic * RB + c16];
+        }
+#pragma unroll
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int c = 16 * w + mrow<T>(q, r);
+      zb[c][c16] = c < kb ? X[(jb + c) * RB + c16] - acc[r] : T(0);
+    }
+    lds_barrier();
+    {  // x_j = Linv_j^T z: wave w -> rows 16w..16w+15
+      V4<T> a2 = vzero<T>();
+#pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        if (k4 < k4lo || k4 >= k4end) continue;
+        a2 = mfma4(li[k4], zb[4 * k4 + q][c16], a2);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + mrow<T>(q, r);
+        if (i < kb) X[(jb + i) * RB + c16] = a2[r];
+      }
+    }
+  }
+  lds_barrier();
 #pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
@@ -2310,26 +2484,27 @@ __device__ __forceinline__ void chunk_reduce(double v, int lcw, int pair, double
 }
 __device__ __forceinline__ double nanmax(double m, double v) { return (v > m || v != v) ? v : m; }
 
-// Second pass: out[pair*16 + c] = sum (or max) over the NCH chunks of part, c < cw (8 interleaved
-// partial results per column, then combined in order); out for c >= cw is zero.
+// Second pass: out[pair*16 + c] = sum (or max) over the NCH chunks of part, one workgroup per
+// (pair, c) output, grid npair x cw: thread t takes chunks t, t + 256, ... in order, then a
+// fixed pairwise tree over the threads.
 template <bool MAX>
-__global__ __launch_bounds__(256) void k_red_final(int npair, int cw, const double* __restrict__ part,
-                                                   double* __restrict__ out) {
-  const int o = threadIdx.x & 31, pr = o >> 4, c = o & 15, s8 = threadIdx.x >> 5;
-  double s = 0.0;
-  if (pr < npair && c < cw)
-    for (int q = s8; q < NCH; q += 8) {
-      const double x = part[((i64)pr * NCH + q) * 16 + c];
-      s = MAX ? nanmax(s, x) : s + x;
-    }
+__global__ __launch_bounds__(256) void k_red_final(const double* __restrict__ part, double* __restrict__ out) {
+  const int c = blockIdx.x, pr = blockIdx.y, t = threadIdx.x;
+  const double* src = part + (i64)pr * NCH * 16 + c;
+  double v[NCH / 256];
+#pragma unroll
+  for (int j = 0; j < NCH / 256; j++) v[j] = src[(i64)(t + 256 * j) * 16];
+  double s = v[0];
+#pragma unroll
+  for (int j = 1; j < NCH / 256; j++) s = MAX ? nanmax(s, v[j]) : s + v[j];
   __shared__ double red[256];
-  red[threadIdx.x] = s;
+  red[t] = s;
   __syncthreads();
-  if (threadIdx.x < 32) {
-    double t = red[threadIdx.x];
-    for (int k = 1; k < 8; k++) t = MAX ? nanmax(t, red[k * 32 + threadIdx.x]) : t + red[k * 32 + threadIdx.x];
-    if (pr < npair) out[o] = t;
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] = MAX ? nanmax(red[t], red[t + h]) : red[t] + red[t + h];
+    __syncthreads();
   }
+  if (t == 0) out[pr * 16 + c] = red[0];
 }
 
 // Start of the refinement: B = R = the right-hand sides (column-major in, columns >= nr zero),

@@ -1043,6 +1043,13 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       };
       // work is attributed to the launches in proportion to their fronts
       auto part = [&](int c, double x) { return x * c / std::max(cnt, 1); };
+      {  // fronts of at most XLR rows: X in LDS
+        const int e = run_end(XLR);
+        if (e > i)
+          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_l<T>, dim3(e - i), li + i, fr,
+                 (const T*)arena, (const int*)p->d_dofs, w, xs);
+        i = e;
+      }
       if (sizeof(T) == 8) {  // (FP32 keeps the streaming kernel: the register form measured slower there)
         int e = run_end(6 * NB);
         if (e > i)
@@ -1082,8 +1089,17 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       work(ell, fl, by);
       g_level = ell;
       if (!blocked(ell)) {
-        launch(p, "k_solve_bwd", fl, by, k_solve_bwd<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
-               (const T*)arena, (const int*)p->d_dofs, w, xs);
+        // fronts of at most XLR rows (a prefix: the lists are sorted by size) with X in LDS
+        const int cnt = A.f_cnt[ell];
+        int e = 0;
+        while (e < cnt && P.fronts[A.ids[A.f_off[ell] + e]].n + P.fronts[A.ids[A.f_off[ell] + e]].m <= XLR) e++;
+        const double fe = cnt ? (double)e / cnt : 0.0;
+        if (e > 0)
+          launch(p, "k_solve_bwd", fl * fe, by * fe, k_solve_bwd_l<T>, dim3(e), ids + A.f_off[ell], fr,
+                 (const T*)arena, (const int*)p->d_dofs, w);
+        if (cnt > e)
+          launch(p, "k_solve_bwd", fl * (1 - fe), by * (1 - fe), k_solve_bwd<T>, dim3(cnt - e), ids + A.f_off[ell] + e,
+                 fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
       } else {
         int nbi, nbt;
         blocks(ell, nbi, nbt);
@@ -1140,7 +1156,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
          p->d_part);
   launch(p, "k_max_final", 0, 0, k_max_final, dim3(1), DOT_BLOCKS, (const double*)p->d_part, p->d_dots + 128);
   launch(p, "k_cg_init", 0, 4.0 * vb, k_cg_init, dim3(gr), n, lcw, nr, in, ld, Bv, R, X, p->d_part);
-  launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(1), 1, cw, (const double*)p->d_part, S2);
+  launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(cw, 1), (const double*)p->d_part, S2);
   double h[32], na;
   HIPCHK(hipMemcpyAsync(&na, p->d_dots + 128, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
@@ -1150,7 +1166,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   if (rc) return rc;
   launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ, Pd,
          p->d_part);
-  launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, rz);
+  launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(cw, 1), (const double*)p->d_part, rz);
   HIPCHK(hipStreamSynchronize(p->stream));
   p->norm_a = na;
   double nb[RB];
@@ -1162,10 +1178,10 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
     it++;
     launch(p, "k_spmv_dot", 0, 12.0 * nnz + 4.0 * (double)n + 3.0 * vb, k_spmv_dot, dim3(gr), n, lcw,
            (const int*)p->d_rp, (const int*)p->d_ci, (const double*)p->d_val, (const double*)Pd, QZ, p->d_part);
-    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, S1);
+    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(cw, 1), (const double*)p->d_part, S1);
     launch(p, "k_cg_xr", 0, 6.5 * vb, k_cg_xr, dim3(gr), n, lcw, (const double*)rz, (const double*)S1, active,
            (const double*)Pd, (const double*)QZ, X, R, w, p->d_part);
-    launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(1), 2, cw, (const double*)p->d_part, S2);
+    launch(p, "k_red_final", 0, 0, k_red_final<true>, dim3(cw, 2), (const double*)p->d_part, S2);
     HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     for (int c = 0; c < nr; c++) {
@@ -1179,7 +1195,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
     if (rc) return rc;
     launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ,
            (double*)nullptr, p->d_part);
-    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(1), 1, cw, (const double*)p->d_part, rzn);
+    launch(p, "k_red_final", 0, 0, k_red_final<false>, dim3(cw, 1), (const double*)p->d_part, rzn);
     launch(p, "k_cg_p", 0, 3.0 * vb, k_cg_p, dim3(ge), n, lcw, (const double*)rzn, (const double*)rz, active,
            (const double*)QZ, Pd);
     std::swap(rz, rzn);
