@@ -453,6 +453,18 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
     // front's arithmetic does not depend on what else shares its launches.
     const std::vector<int>& level_fronts = general_fronts(p, ell);
     const bool left = level_fronts.size() >= 32;
+    // BF16-split GEMMs (split) only on levels whose GEMM work (~ n N^2 per front, over the
+    // plan's whole level) lies mostly in fronts of 1024+ pivots: the split pays where the GEMM
+    // stages are MFMA-bound (3D 127^3: 1259 -> 834 ms per step); on smaller fronts they are bound
+    // by memory and latency (2D 1023^2, fronts of up to 2045 pivots: 21.2 -> 20.9 ms), where the
+    // split only costs accuracy (refinement iterations)
+    double wbig = 0, wall = 0;
+    for (int f : level_fronts) {
+      const double n = P.fronts[f].n, N = n + P.fronts[f].m, wf = n * N * N;
+      wall += wf;
+      if (n >= 1024) wbig += wf;
+    }
+    const bool sp = split && 2 * wbig >= wall && wall > 0;
     // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
     // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
     bool fly = left && !P.rest_off.empty();
@@ -639,14 +651,14 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         i = e;
       }
       if (tin > 0)
-        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(split), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
+        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
                ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
       auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
         if (wide(t128, act))
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(split), dim3(t128, act), ids + L.f_off,
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), dim3(t128, act), ids + L.f_off,
                  (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
         else
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(split), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
+          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
                  ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
       };
       if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
@@ -683,10 +695,10 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
         if (tiles > 0) {
           if (wide(tiles128, c))
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(split), grid(tiles128), ids + L.f_off + i,
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
           else
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(split), grid(tiles), ids + L.f_off + i,
+            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(sp), grid(tiles), ids + L.f_off + i,
                    (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
         }
         i = e;
