@@ -779,9 +779,26 @@ __global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ id
   diag_core(sm.d, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
 }
 
+// Persistent launches over a queue of items (fronts of one level, largest first): a grid of as
+// many workgroups as the chip holds at once, each taking the next item until the queue is empty.
+// Next item of the queue counter q (items 0..cnt-1), or -1 when it is empty.  Every workgroup
+// fetches until its first empty answer, so a launch makes exactly cnt + gridDim.x fetches; the
+// last one resets the counter for the next launch on the stream.
+__device__ __forceinline__ int queue_next(int* __restrict__ q, int cnt) {
+  __shared__ int item;
+  __syncthreads();  // the previous item's LDS work is done
+  if (threadIdx.x == 0) {
+    const int v = atomicAdd(q, 1);
+    if (v == cnt + (int)gridDim.x - 1) atomicExch(q, 0);
+    item = v < cnt ? v : -1;
+  }
+  __syncthreads();
+  return item;
+}
+
 // Small fronts assembled from the operator alone (no child blocks: leaf fronts) with one pivot
-// block (1 <= n <= 64) and m <= LEAF_M boundary rows, one workgroup per front, the whole
-// factorization in LDS: the pivot block and the panel rows are scattered from the front's CSR
+// block (1 <= n <= 64) and m <= LEAF_M boundary rows, one workgroup per front (a persistent
+// launch over the level's queue of them, largest first), the whole factorization in LDS: the pivot block and the panel rows are scattered from the front's CSR
 // triples, the block is factored with its inverse (diag_core: L11 to the front, Linv to its
 // dinv block), the panel becomes L21 = A12 Linv^T, and the boundary block, scattered into the
 // freed diagonal buffer, takes its Schur complement S = A11 - L21 L21^T.  The front is written
@@ -792,14 +809,15 @@ __global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ id
 #define LEAF_M 48
 #define LDP (LEAF_M + 1)
 template <class T>
-__global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                              T* __restrict__ A, const i64* __restrict__ tri_off,
-                                              const Triple* __restrict__ tri, const double* __restrict__ csr_val,
-                                              int* __restrict__ err) {
-  const int fid = ids[blockIdx.x];
-  const FrontDev F = fr[fid];
+__global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, int cnt, int* __restrict__ queue,
+                                              const FrontDev* __restrict__ fr, T* __restrict__ A,
+                                              const i64* __restrict__ tri_off, const Triple* __restrict__ tri,
+                                              const double* __restrict__ csr_val, int* __restrict__ err) {
   __shared__ DiagSmem<T> sd;
   __shared__ T Pn[NB * LDP];  // Pn[c*LDP + r]: A12(r, c), then L21(r, c); r < m, c < 64
+  for (int item; (item = queue_next(queue, cnt)) >= 0;) {
+  const int fid = ids[item];
+  const FrontDev F = fr[fid];
   T* Ts = sd.Ts;
   T* U = Ts + LD;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, c16 = lane & 15, q4 = lane >> 4;
@@ -825,7 +843,7 @@ __global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, const
   __syncthreads();
   diag_core(sd, a, F.N, n, A + F.dinv_off, err, fid, 0);
   __syncthreads();
-  if (m == 0) return;
+  if (m == 0) continue;
   const int nk = (n + 15) >> 4;  // 16-column blocks of the pivot columns
   // panel: wave w forms rows [16w, 16w + 16) of L21 = A12 Linv^T (Linv(r, c) at U[r*LD + c])
   if (16 * w < m) {
@@ -877,6 +895,7 @@ __global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, const
       const int c = 16 * bj + mrow<T>(q4, qq);
       if (r < m && c <= r) a[(i64)(n + c) * F.N + n + r] = Ts[c * LD + r] - acc[qq];
     }
+  }
   }
 }
 
@@ -1501,8 +1520,8 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
 }
 
-// ---- sweeps of small fronts (at most XLR rows) with the front's right-hand-side block in LDS
-// instead of the global scratch: the only global traffic is the factor, the work block W and
+// ---- sweeps of small fronts (at most XLR rows), persistent launches over the level's queue of
+// them, with the front's right-hand-side block in LDS instead of the global scratch: the only global traffic is the factor, the work block W and
 // (forward) the boundary rows k_sep_acc reads; the steps synchronize with LDS-only barriers,
 // so the next diagonal block's inverse, loaded during a step, stays in flight across them.
 #define XLR 320  // rows of X held in LDS (2D leaf fronts of maxSideLength 8 have N = 285)
@@ -1517,15 +1536,17 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Forward sweep (as k_solve_fwd) of fronts with N <= XLR.
 template <class T>
-__global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                     const T* __restrict__ A, const int* __restrict__ dofs,
-                                                     T* __restrict__ W, T* __restrict__ xs) {
-  const FrontDev F = fr[ids[blockIdx.x]];
+__global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids, int cnt, int* __restrict__ queue,
+                                                     const FrontDev* __restrict__ fr, const T* __restrict__ A,
+                                                     const int* __restrict__ dofs, T* __restrict__ W,
+                                                     T* __restrict__ xs) {
+  __shared__ T X[XLR * RB];
+  __shared__ T xb[NB][RB];
+  for (int item; (item = queue_next(queue, cnt)) >= 0;) {
+  const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
   const T* L = A + F.off;
   const int* di = dofs + F.int_off;
   const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
-  __shared__ T X[XLR * RB];
-  __shared__ T xb[NB][RB];
 #pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
   for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = T(0);
@@ -1592,20 +1613,22 @@ __global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids
 #pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
   for (int e = F.n * RB + t; e < F.N * RB; e += 256) Xg[e] = X[e];
+  }
 }
 
 // Backward sweep (as k_solve_bwd) of fronts with N <= XLR.
 template <class T>
-__global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                     const T* __restrict__ A, const int* __restrict__ dofs,
-                                                     T* __restrict__ W) {
-  const FrontDev F = fr[ids[blockIdx.x]];
+__global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids, int cnt, int* __restrict__ queue,
+                                                     const FrontDev* __restrict__ fr, const T* __restrict__ A,
+                                                     const int* __restrict__ dofs, T* __restrict__ W) {
+  __shared__ T X[XLR * RB];
+  __shared__ T zb[NB][RB];
+  for (int item; (item = queue_next(queue, cnt)) >= 0;) {
+  const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
   const T* L = A + F.off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
-  __shared__ T X[XLR * RB];
-  __shared__ T zb[NB][RB];
 #pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
 #pragma unroll 4
@@ -1666,6 +1689,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids
   lds_barrier();
 #pragma unroll 4
   for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+  }
 }
 
 // ---- forward sweep with the right-hand-side rows in registers (fronts of at most 64 G rows):

@@ -130,7 +130,7 @@ struct umf_plan {
   int* d_rp = nullptr;
   int* d_ci = nullptr;
   double* d_val = nullptr;
-  int* d_err = nullptr;
+  int* d_err = nullptr;     // [0..3] error flags, [4] the queue counter of the persistent launches
   int* d_flags = nullptr;  // block flags of the blocked solve sweeps
   int* d_arrive = nullptr;  // per-front arrival counters of k_chol_dpanel (zero between launches)
   int n_flags = 0;
@@ -164,6 +164,8 @@ struct umf_plan {
   int* d_fids = nullptr;
   std::vector<int> ids_h;  // host copy, alive until the stream has consumed it
   std::vector<std::vector<int>> gen_fronts;  // per level: fronts on the general factor path
+  std::map<const void*, int> resident;  // workgroups per CU of the persistent kernels
+  int n_cu = 0;
   // the sweeps' level lists (fixed by the plan): built and uploaded once, then every sweep
   // launches from them without a host rebuild or a wait for the stream
   std::unique_ptr<umf::AllLists> sweep_lists;
@@ -245,6 +247,9 @@ Sched make_sched(const Plan& P, const Plan::Dirty& D, bool small = false) {
                      [&](int a, int b) { return P.fronts[a].n > P.fronts[b].n; });
     L.lf_off = (int)S.ids.size();
     L.lf_cnt = (int)leaves.size();
+    std::stable_sort(leaves.begin(), leaves.end(), [&](int a, int b) {  // k_leaf's queue: largest first
+      return P.fronts[a].n + P.fronts[a].m > P.fronts[b].n + P.fronts[b].m;
+    });
     S.ids.insert(S.ids.end(), leaves.begin(), leaves.end());
     L.s_off = (int)S.ids.size();
     L.max_k = 0;
@@ -355,6 +360,18 @@ auto gemm_fn(bool split) {
 // tens of thousands of small fronts and one large one then launch one workgroup per front.
 inline int per_item(int want, int cnt) { return std::max(1, std::min(want, (32768 + cnt - 1) / std::max(cnt, 1))); }
 
+// Grid of a persistent launch of kern (256 threads) over cnt queue items: as many workgroups as
+// the device holds at once, at most cnt.
+unsigned persistent_grid(umf_plan* p, const void* kern, int cnt) {
+  int& per_cu = p->resident[kern];
+  if (per_cu == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  }
+  if (p->n_cu == 0 && hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+    p->n_cu = 256;
+  return (unsigned)std::max(1, std::min(cnt, per_cu * std::max(p->n_cu, 1)));
+}
+
 // The fronts of level ell that take the general path (not leaf_front), cached per plan: the
 // level-wide algorithm choices of the factorization are made over them.
 const std::vector<int>& general_fronts(umf_plan* p, int ell) {
@@ -435,8 +452,9 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
         fl += 2.0 * n * n * n / 3.0 + m * n * n + n * m * (m + 1);  // factor + inverse, panel, Schur
         by += sizeof(T) * (lowerN(n) + m * n + lowerN(m) + NB * NB) + 20.0 * (P.tri_off[f + 1] - P.tri_off[f]);
       }
-      launch(p, "k_leaf", fl, by, k_leaf<T>, dim3(L.lf_cnt), ids + L.lf_off, (const FrontDev*)p->d_fronts, arena,
-             (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const double*)p->d_val, p->d_err);
+      launch(p, "k_leaf", fl, by, k_leaf<T>, dim3(persistent_grid(p, (const void*)k_leaf<T>, L.lf_cnt)),
+             ids + L.lf_off, L.lf_cnt, p->d_err + 4, (const FrontDev*)p->d_fronts, arena, (const i64*)p->d_tri_off,
+             (const Triple*)p->d_tri, (const double*)p->d_val, p->d_err);
     }
     if (L.f_cnt > 0) [&] {
     // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
@@ -834,6 +852,9 @@ int factor_graph(umf_plan* p, const Plan::Dirty& D) {
     p->fgraph = nullptr;
     int rs = ensure_full_sched(p);  // the Cholesky schedule run_factor replays
     if (rs) return rs;
+    // device queries before the capture: the persistent launches' grids
+    (void)persistent_grid(p, (const void*)k_leaf<double>, 1);
+    (void)persistent_grid(p, (const void*)k_leaf<float>, 1);
     HIPCHK(hipStreamSynchronize(p->stream));
     if (p->prec != UMF_PREC_FP64 && !p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
@@ -1058,7 +1079,8 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       {  // fronts of at most XLR rows: X in LDS
         const int e = run_end(XLR);
         if (e > i)
-          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_l<T>, dim3(e - i), li + i, fr,
+          launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_l<T>,
+                 dim3(persistent_grid(p, (const void*)k_solve_fwd_l<T>, e - i)), li + i, e - i, p->d_err + 4, fr,
                  (const T*)arena, (const int*)p->d_dofs, w, xs);
         i = e;
       }
@@ -1107,7 +1129,8 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
         while (e < cnt && P.fronts[A.ids[A.f_off[ell] + e]].n + P.fronts[A.ids[A.f_off[ell] + e]].m <= XLR) e++;
         const double fe = cnt ? (double)e / cnt : 0.0;
         if (e > 0)
-          launch(p, "k_solve_bwd", fl * fe, by * fe, k_solve_bwd_l<T>, dim3(e), ids + A.f_off[ell], fr,
+          launch(p, "k_solve_bwd", fl * fe, by * fe, k_solve_bwd_l<T>,
+                 dim3(persistent_grid(p, (const void*)k_solve_bwd_l<T>, e)), ids + A.f_off[ell], e, p->d_err + 4, fr,
                  (const T*)arena, (const int*)p->d_dofs, w);
         if (cnt > e)
           launch(p, "k_solve_bwd", fl * (1 - fe), by * (1 - fe), k_solve_bwd<T>, dim3(cnt - e), ids + A.f_off[ell] + e,
@@ -1619,8 +1642,8 @@ int setup_device(umf_plan* p, umf_plan** out) {
   UP(p->d_ci, P.ci);
   UP(p->d_val, p->val_h);
 #undef UP
-  if ((he = hipMalloc((void**)&p->d_err, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
-  if ((he = hipMemset(p->d_err, 0, sizeof(int) * 4)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMalloc((void**)&p->d_err, sizeof(int) * 8)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMemset(p->d_err, 0, sizeof(int) * 8)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMalloc((void**)&p->d_flags, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMemset(p->d_flags, 0, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
@@ -2614,8 +2637,8 @@ int make_slot(umf_plan* b, umf_plan** out) {
   SLOTCHK(hipMalloc((void**)&q->d_seps, sizeof(SepDev) * std::max<size_t>(NS, 1)));
   SLOTCHK(hipMalloc((void**)&q->d_val, sizeof(double) * std::max<size_t>(nnz, 1)));
   SLOTCHK(hipMemcpy(q->d_val, b->d_val, sizeof(double) * nnz, hipMemcpyDeviceToDevice));
-  SLOTCHK(hipMalloc((void**)&q->d_err, sizeof(int) * 4));
-  SLOTCHK(hipMemset(q->d_err, 0, sizeof(int) * 4));
+  SLOTCHK(hipMalloc((void**)&q->d_err, sizeof(int) * 8));
+  SLOTCHK(hipMemset(q->d_err, 0, sizeof(int) * 8));
   SLOTCHK(hipMalloc((void**)&q->d_flags, sizeof(int) * std::max(q->n_flags, 1)));
   SLOTCHK(hipMemset(q->d_flags, 0, sizeof(int) * std::max(q->n_flags, 1)));
   SLOTCHK(hipMalloc((void**)&q->d_arrive, sizeof(int) * std::max<size_t>(NF, 1)));
