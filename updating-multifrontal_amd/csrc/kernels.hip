@@ -816,86 +816,86 @@ __global__ __launch_bounds__(256) void k_leaf(const int* __restrict__ ids, int c
   __shared__ DiagSmem<T> sd;
   __shared__ T Pn[NB * LDP];  // Pn[c*LDP + r]: A12(r, c), then L21(r, c); r < m, c < 64
   for (int item; (item = queue_next(queue, cnt)) >= 0;) {
-  const int fid = ids[item];
-  const FrontDev F = fr[fid];
-  T* Ts = sd.Ts;
-  T* U = Ts + LD;
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63, c16 = lane & 15, q4 = lane >> 4;
-  const int n = F.n, m = F.m;
-  T* a = A + F.off;
-  // pivot block: strict upper triangle zero, identity beyond n; panel rows zero
-  for (int e = t; e < LD * LD; e += 256) {
-    const int c = e / LD, r = e - c * LD;
-    Ts[e] = (r == c && r >= n && r < NB) ? T(1) : T(0);
-  }
-  for (int e = t; e < NB * LDP; e += 256) Pn[e] = T(0);
-  __syncthreads();
-  const i64 t0 = tri_off[fid], t1 = tri_off[fid + 1];
-  for (i64 i = t0 + t; i < t1; i += 256) {
-    const Triple x = tri[i];
-    if (x.col >= n) continue;  // the boundary block comes later, the A21 block is not used
-    if (x.row < n) {
-      if (x.row >= x.col) Ts[x.col * LD + x.row] = (T)csr_val[x.k];
-    } else {
-      Pn[x.col * LDP + (x.row - n)] = (T)csr_val[x.k];
+    const int fid = ids[item];
+    const FrontDev F = fr[fid];
+    T* Ts = sd.Ts;
+    T* U = Ts + LD;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63, c16 = lane & 15, q4 = lane >> 4;
+    const int n = F.n, m = F.m;
+    T* a = A + F.off;
+    // pivot block: strict upper triangle zero, identity beyond n; panel rows zero
+    for (int e = t; e < LD * LD; e += 256) {
+      const int c = e / LD, r = e - c * LD;
+      Ts[e] = (r == c && r >= n && r < NB) ? T(1) : T(0);
     }
-  }
-  __syncthreads();
-  diag_core(sd, a, F.N, n, A + F.dinv_off, err, fid, 0);
-  __syncthreads();
-  if (m == 0) continue;
-  const int nk = (n + 15) >> 4;  // 16-column blocks of the pivot columns
-  // panel: wave w forms rows [16w, 16w + 16) of L21 = A12 Linv^T (Linv(r, c) at U[r*LD + c])
-  if (16 * w < m) {
-    V4<T> acc[4];
-#pragma unroll
-    for (int cj = 0; cj < 4; cj++) {
-      acc[cj] = vzero<T>();
-      if (cj >= nk) continue;
-#pragma unroll
-      for (int kb = 0; kb <= cj; kb++) {
-        if (kb == cj)
-          mma16<0, 1>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
-        else
-          mma16<0, 0>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
+    for (int e = t; e < NB * LDP; e += 256) Pn[e] = T(0);
+    __syncthreads();
+    const i64 t0 = tri_off[fid], t1 = tri_off[fid + 1];
+    for (i64 i = t0 + t; i < t1; i += 256) {
+      const Triple x = tri[i];
+      if (x.col >= n) continue;  // the boundary block comes later, the A21 block is not used
+      if (x.row < n) {
+        if (x.row >= x.col) Ts[x.col * LD + x.row] = (T)csr_val[x.k];
+      } else {
+        Pn[x.col * LDP + (x.row - n)] = (T)csr_val[x.k];
       }
     }
-    const int r = 16 * w + c16;
-#pragma unroll
-    for (int cj = 0; cj < 4; cj++)
-#pragma unroll
+    __syncthreads();
+    diag_core(sd, a, F.N, n, A + F.dinv_off, err, fid, 0);
+    __syncthreads();
+    if (m == 0) continue;
+    const int nk = (n + 15) >> 4;  // 16-column blocks of the pivot columns
+    // panel: wave w forms rows [16w, 16w + 16) of L21 = A12 Linv^T (Linv(r, c) at U[r*LD + c])
+    if (16 * w < m) {
+      V4<T> acc[4];
+  #pragma unroll
+      for (int cj = 0; cj < 4; cj++) {
+        acc[cj] = vzero<T>();
+        if (cj >= nk) continue;
+  #pragma unroll
+        for (int kb = 0; kb <= cj; kb++) {
+          if (kb == cj)
+            mma16<0, 1>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
+          else
+            mma16<0, 0>(acc[cj], Pn + 16 * kb * LDP + 16 * w, 1, LDP, U + 16 * cj * LD + 16 * kb, LD, 1);
+        }
+      }
+      const int r = 16 * w + c16;
+  #pragma unroll
+      for (int cj = 0; cj < 4; cj++)
+  #pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          const int c = 16 * cj + mrow<T>(q4, qq);
+          const T v = c < n ? acc[cj][qq] : T(0);
+          Pn[c * LDP + r] = v;  // this wave's rows only: no other wave reads them before the barrier
+          if (c < n && r < m) a[(i64)c * F.N + n + r] = v;
+        }
+    }
+    __syncthreads();
+    // boundary block A11 (lower triangle) into the diagonal buffer
+    for (int e = t; e < LD * LD; e += 256) Ts[e] = T(0);
+    __syncthreads();
+    for (i64 i = t0 + t; i < t1; i += 256) {
+      const Triple x = tri[i];
+      if (x.col >= n && x.row >= x.col) Ts[(x.col - n) * LD + (x.row - n)] = (T)csr_val[x.k];
+    }
+    __syncthreads();
+    // S = A11 - L21 L21^T, lower 16 x 16 tiles dealt round-robin to the waves
+    const int mt = (m + 15) >> 4;
+    for (int tl = w; tl < mt * (mt + 1) / 2; tl += 4) {
+      int bi = 0;
+      while ((bi + 1) * (bi + 2) / 2 <= tl) bi++;
+      const int bj = tl - bi * (bi + 1) / 2;
+      V4<T> acc = vzero<T>();
+      for (int kb = 0; kb < nk; kb++)
+        mma16(acc, Pn + 16 * kb * LDP + 16 * bi, 1, LDP, Pn + 16 * kb * LDP + 16 * bj, 1, LDP);
+      const int r = 16 * bi + c16;
+  #pragma unroll
       for (int qq = 0; qq < 4; qq++) {
-        const int c = 16 * cj + mrow<T>(q4, qq);
-        const T v = c < n ? acc[cj][qq] : T(0);
-        Pn[c * LDP + r] = v;  // this wave's rows only: no other wave reads them before the barrier
-        if (c < n && r < m) a[(i64)c * F.N + n + r] = v;
+        const int c = 16 * bj + mrow<T>(q4, qq);
+        if (r < m && c <= r) a[(i64)(n + c) * F.N + n + r] = Ts[c * LD + r] - acc[qq];
       }
-  }
-  __syncthreads();
-  // boundary block A11 (lower triangle) into the diagonal buffer
-  for (int e = t; e < LD * LD; e += 256) Ts[e] = T(0);
-  __syncthreads();
-  for (i64 i = t0 + t; i < t1; i += 256) {
-    const Triple x = tri[i];
-    if (x.col >= n && x.row >= x.col) Ts[(x.col - n) * LD + (x.row - n)] = (T)csr_val[x.k];
-  }
-  __syncthreads();
-  // S = A11 - L21 L21^T, lower 16 x 16 tiles dealt round-robin to the waves
-  const int mt = (m + 15) >> 4;
-  for (int tl = w; tl < mt * (mt + 1) / 2; tl += 4) {
-    int bi = 0;
-    while ((bi + 1) * (bi + 2) / 2 <= tl) bi++;
-    const int bj = tl - bi * (bi + 1) / 2;
-    V4<T> acc = vzero<T>();
-    for (int kb = 0; kb < nk; kb++)
-      mma16(acc, Pn + 16 * kb * LDP + 16 * bi, 1, LDP, Pn + 16 * kb * LDP + 16 * bj, 1, LDP);
-    const int r = 16 * bi + c16;
-#pragma unroll
-    for (int qq = 0; qq < 4; qq++) {
-      const int c = 16 * bj + mrow<T>(q4, qq);
-      if (r < m && c <= r) a[(i64)(n + c) * F.N + n + r] = Ts[c * LD + r] - acc[qq];
     }
-  }
   }
 }
 
@@ -1543,76 +1543,76 @@ __global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids
   __shared__ T X[XLR * RB];
   __shared__ T xb[NB][RB];
   for (int item; (item = queue_next(queue, cnt)) >= 0;) {
-  const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
-  const T* L = A + F.off;
-  const int* di = dofs + F.int_off;
-  const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
-#pragma unroll 4
-  for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
-  for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = T(0);
-  // the wave's slices of a diagonal block's inverse (lower triangular: slices right of the
-  // wave's rows and past kb are zero and not loaded)
-  T li[NB / 4];
-  auto load_li = [&](int jb) {
-    const T* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
-    const int k4end = min(4 * (w + 1), (min(NB, F.n - jb) + 3) >> 2);
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++)
-      if (k4 < k4end) li[k4] = Li[(4 * k4 + q) * NB + 16 * w + c16];
-  };
-  load_li(0);
-  for (int jb = 0; jb < F.n; jb += NB) {
-    const int kb = min(NB, F.n - jb);
-    const int k4end = min(4 * (w + 1), (kb + 3) >> 2);
-    lds_barrier();
-    V4<T> acc = vzero<T>();  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {
-      if (k4 >= k4end) break;
-      const int k = 4 * k4 + q;
-      acc = mfma4(li[k4], k < kb ? X[(jb + k) * RB + c16] : T(0), acc);
-    }
-    if (jb + NB < F.n) load_li(jb + NB);
-    lds_barrier();
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int i = 16 * w + mrow<T>(q, r);
-      xb[i][c16] = acc[r];
-      if (i < kb) X[(jb + i) * RB + c16] = acc[r];
-    }
-    lds_barrier();
-    // rows below: X(i) -= L(i, block) x_j, one 16-row group per wave at a time (the inverse's
-    // slices in flight take the registers of a second)
-    constexpr int G = 1;
-    for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * G) {
-      T av[G][NB / 4];
-#pragma unroll
-      for (int u = 0; u < G; u++) {
-        const int row = min(g0 + 64 * u + c16, F.N - 1);
-#pragma unroll
-        for (int k4 = 0; k4 < NB / 4; k4++) {
-          const int k = 4 * k4 + q;
-          av[u][k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + row, k < kb);
+    const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
+    const T* L = A + F.off;
+    const int* di = dofs + F.int_off;
+    const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
+  #pragma unroll 4
+    for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
+    for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = T(0);
+    // the wave's slices of a diagonal block's inverse (lower triangular: slices right of the
+    // wave's rows and past kb are zero and not loaded)
+    T li[NB / 4];
+    auto load_li = [&](int jb) {
+      const T* Li = A + F.dinv_off + (i64)(jb / NB) * NB * NB;
+      const int k4end = min(4 * (w + 1), (min(NB, F.n - jb) + 3) >> 2);
+  #pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++)
+        if (k4 < k4end) li[k4] = Li[(4 * k4 + q) * NB + 16 * w + c16];
+    };
+    load_li(0);
+    for (int jb = 0; jb < F.n; jb += NB) {
+      const int kb = min(NB, F.n - jb);
+      const int k4end = min(4 * (w + 1), (kb + 3) >> 2);
+      lds_barrier();
+      V4<T> acc = vzero<T>();  // x_j = Linv_j b_j: wave w -> rows 16w..16w+15
+  #pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++) {
+        if (k4 >= k4end) break;
+        const int k = 4 * k4 + q;
+        acc = mfma4(li[k4], k < kb ? X[(jb + k) * RB + c16] : T(0), acc);
+      }
+      if (jb + NB < F.n) load_li(jb + NB);
+      lds_barrier();
+  #pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * w + mrow<T>(q, r);
+        xb[i][c16] = acc[r];
+        if (i < kb) X[(jb + i) * RB + c16] = acc[r];
+      }
+      lds_barrier();
+      // rows below: X(i) -= L(i, block) x_j, one 16-row group per wave at a time (the inverse's
+      // slices in flight take the registers of a second)
+      constexpr int G = 1;
+      for (int g0 = jb + kb + 16 * w; g0 < F.N; g0 += 64 * G) {
+        T av[G][NB / 4];
+  #pragma unroll
+        for (int u = 0; u < G; u++) {
+          const int row = min(g0 + 64 * u + c16, F.N - 1);
+  #pragma unroll
+          for (int k4 = 0; k4 < NB / 4; k4++) {
+            const int k = 4 * k4 + q;
+            av[u][k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + row, k < kb);
+          }
+        }
+  #pragma unroll
+        for (int u = 0; u < G; u++) {
+          V4<T> a2 = vzero<T>();
+  #pragma unroll
+          for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(av[u][k4], xb[4 * k4 + q][c16], a2);
+  #pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int i = g0 + 64 * u + mrow<T>(q, r);
+            if (i < F.N) X[i * RB + c16] -= a2[r];
+          }
         }
       }
-#pragma unroll
-      for (int u = 0; u < G; u++) {
-        V4<T> a2 = vzero<T>();
-#pragma unroll
-        for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(av[u][k4], xb[4 * k4 + q][c16], a2);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = g0 + 64 * u + mrow<T>(q, r);
-          if (i < F.N) X[i * RB + c16] -= a2[r];
-        }
-      }
     }
-  }
-  lds_barrier();
-  T* Xg = xs + F.x_off;  // boundary rows for k_sep_acc
-#pragma unroll 4
-  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
-  for (int e = F.n * RB + t; e < F.N * RB; e += 256) Xg[e] = X[e];
+    lds_barrier();
+    T* Xg = xs + F.x_off;  // boundary rows for k_sep_acc
+  #pragma unroll 4
+    for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+    for (int e = F.n * RB + t; e < F.N * RB; e += 256) Xg[e] = X[e];
   }
 }
 
@@ -1624,71 +1624,71 @@ __global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids
   __shared__ T X[XLR * RB];
   __shared__ T zb[NB][RB];
   for (int item; (item = queue_next(queue, cnt)) >= 0;) {
-  const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
-  const T* L = A + F.off;
-  const int* di = dofs + F.int_off;
-  const int* db = dofs + F.bd_off;
-  const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
-#pragma unroll 4
-  for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
-#pragma unroll 4
-  for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = W[(i64)db[(e >> 4) - F.n] * RB + (e & 15)];
-  const int nblk = (F.n + NB - 1) / NB;
-  for (int bk = nblk - 1; bk >= 0; bk--) {
-    const int jb = bk * NB, kb = min(NB, F.n - jb);
-    const int k4lo = 4 * w, k4end = (kb + 3) >> 2;
-    // the wave's slices of Linv_j^T (rows 16w.., columns c >= 16w), loaded first: in flight
-    // during the product below
-    const T* Li = A + F.dinv_off + (i64)bk * NB * NB;
-    T li[NB / 4];
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++)
-      if (k4 >= k4lo && k4 < k4end) li[k4] = Li[(16 * w + c16) * NB + 4 * k4 + q];
-    lds_barrier();
-    // T(c, r) = sum_{i >= jb+kb} L(i, jb+c) X(i, r); wave w owns columns [16w, 16w+16)
-    const int col = 16 * w + c16;
-    const T* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
-    const bool cin = col < kb;
-    V4<T> acc = vzero<T>();
-    for (int i0 = jb + kb; i0 < F.N; i0 += 16 * GU) {
-      T av[GU][4], bv[GU][4];
-#pragma unroll
-      for (int u = 0; u < GU; u++)
-#pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) {
-          const int i = i0 + 16 * u + 4 * k4 + q;
-          const int ic = min(i, F.N - 1);
-          av[u][k4] = ld_or0(Lc + ic, cin && i < F.N);
-          bv[u][k4] = X[ic * RB + c16];
-        }
-#pragma unroll
-      for (int u = 0; u < GU; u++)
-#pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int c = 16 * w + mrow<T>(q, r);
-      zb[c][c16] = c < kb ? X[(jb + c) * RB + c16] - acc[r] : T(0);
-    }
-    lds_barrier();
-    {  // x_j = Linv_j^T z: wave w -> rows 16w..16w+15
-      V4<T> a2 = vzero<T>();
-#pragma unroll
-      for (int k4 = 0; k4 < NB / 4; k4++) {
-        if (k4 < k4lo || k4 >= k4end) continue;
-        a2 = mfma4(li[k4], zb[4 * k4 + q][c16], a2);
+    const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
+    const T* L = A + F.off;
+    const int* di = dofs + F.int_off;
+    const int* db = dofs + F.bd_off;
+    const int t = threadIdx.x, lane = t & 63, w = wave_id(), c16 = lane & 15, q = lane >> 4;
+  #pragma unroll 4
+    for (int e = t; e < F.n * RB; e += 256) X[e] = W[(i64)di[e >> 4] * RB + (e & 15)];
+  #pragma unroll 4
+    for (int e = F.n * RB + t; e < F.N * RB; e += 256) X[e] = W[(i64)db[(e >> 4) - F.n] * RB + (e & 15)];
+    const int nblk = (F.n + NB - 1) / NB;
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+      const int jb = bk * NB, kb = min(NB, F.n - jb);
+      const int k4lo = 4 * w, k4end = (kb + 3) >> 2;
+      // the wave's slices of Linv_j^T (rows 16w.., columns c >= 16w), loaded first: in flight
+      // during the product below
+      const T* Li = A + F.dinv_off + (i64)bk * NB * NB;
+      T li[NB / 4];
+  #pragma unroll
+      for (int k4 = 0; k4 < NB / 4; k4++)
+        if (k4 >= k4lo && k4 < k4end) li[k4] = Li[(16 * w + c16) * NB + 4 * k4 + q];
+      lds_barrier();
+      // T(c, r) = sum_{i >= jb+kb} L(i, jb+c) X(i, r); wave w owns columns [16w, 16w+16)
+      const int col = 16 * w + c16;
+      const T* Lc = L + (i64)(jb + min(col, kb - 1)) * F.N;
+      const bool cin = col < kb;
+      V4<T> acc = vzero<T>();
+      for (int i0 = jb + kb; i0 < F.N; i0 += 16 * GU) {
+        T av[GU][4], bv[GU][4];
+  #pragma unroll
+        for (int u = 0; u < GU; u++)
+  #pragma unroll
+          for (int k4 = 0; k4 < 4; k4++) {
+            const int i = i0 + 16 * u + 4 * k4 + q;
+            const int ic = min(i, F.N - 1);
+            av[u][k4] = ld_or0(Lc + ic, cin && i < F.N);
+            bv[u][k4] = X[ic * RB + c16];
+          }
+  #pragma unroll
+        for (int u = 0; u < GU; u++)
+  #pragma unroll
+          for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
       }
-#pragma unroll
+  #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = 16 * w + mrow<T>(q, r);
-        if (i < kb) X[(jb + i) * RB + c16] = a2[r];
+        const int c = 16 * w + mrow<T>(q, r);
+        zb[c][c16] = c < kb ? X[(jb + c) * RB + c16] - acc[r] : T(0);
+      }
+      lds_barrier();
+      {  // x_j = Linv_j^T z: wave w -> rows 16w..16w+15
+        V4<T> a2 = vzero<T>();
+  #pragma unroll
+        for (int k4 = 0; k4 < NB / 4; k4++) {
+          if (k4 < k4lo || k4 >= k4end) continue;
+          a2 = mfma4(li[k4], zb[4 * k4 + q][c16], a2);
+        }
+  #pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * w + mrow<T>(q, r);
+          if (i < kb) X[(jb + i) * RB + c16] = a2[r];
+        }
       }
     }
-  }
-  lds_barrier();
-#pragma unroll 4
-  for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
+    lds_barrier();
+  #pragma unroll 4
+    for (int e = t; e < F.n * RB; e += 256) W[(i64)di[e >> 4] * RB + (e & 15)] = X[e];
   }
 }
 
