@@ -14,7 +14,8 @@ alternate, so every step really refactors.
 value = the FP32 form's algorithmic TFLOP/s over the timed steps: fronts stored and factored in
 FP32 on FP32 MFMA, the solve refined in FP64 (preconditioned CG on the FP64 operator) until every
 column's infinity-norm backward error ||b - Ax|| / (||A|| ||x|| + ||b||) is <= 1e-10.  The plain
-FP64 form and the BF16-split form run the same steps beside it (objects "fp64" and "bf16x3_ir");
+FP64 form, the BF16-split form and the mixed form (FP64 top three separator levels over FP32
+ones) run the same steps beside it (objects "fp64", "bf16x3_ir", "fp32t64_ir");
 every form reports its backward error and its forward error against the FP64 form's solution.
 TFLOP/s numerator (the same for every form and for the CPU baseline): F = sum over fronts of
 n^3/3 + n^2 m + n m^2 (factor; dirty fronts for the update) + nrhs * sum (2 n^2 + 4 n m) (solve),
@@ -35,14 +36,18 @@ sys.path.insert(0, PKG)
 
 METRIC = "factor+update+solve s & TFLOP/s; FP32-MFMA+IR vs FP64 at 1e-10 backward err"
 # published MI355X dense peaks: FP64 / FP32 MFMA, BF16 MFMA; HBM3E
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+# (the mixed form's dominant GEMMs are its FP64 top levels': FP64 peak)
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "fp32t64": 78.6}
 PEAK_HBM_GBS = 8000.0
 FORM_DTYPE = {
     "fp64": "f64",
     "fp32": "f32 fronts (FP32 MFMA) + f64 residual and refinement",
     "bf16x3": "f32 fronts, Schur/trailing GEMMs on BF16 MFMA over 2 bf16 pieces (3 products, f32 accumulate) "
               "+ f64 residual and refinement",
+    "fp32t64": "f64 fronts on the top 3 separator levels, f32 fronts (FP32 MFMA) below + f64 residual and "
+               "refinement",
 }
+ALL_FORMS = ("fp64", "fp32", "bf16x3", "fp32t64")
 SOURCES = [os.path.join(PKG, "csrc", f) for f in ("kernels.hip", "runtime.hip", "plan.cpp")]
 
 
@@ -452,7 +457,7 @@ def main():
                     help="timed steps of the forms beside the value form (at most --steps)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d127")
     ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (16; 4 for the batch)")
-    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=["fp64", "fp32", "bf16x3"],
+    ap.add_argument("--forms", nargs="+", choices=sorted(FORM_DTYPE), default=list(ALL_FORMS),
                     help="precision forms to time (the value form is always timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 2D secondary workloads")
@@ -500,7 +505,7 @@ def main():
 
     spec = WORKLOADS[args.workload]
     value_form = spec["value_form"]
-    forms = [f for f in ("fp64", "fp32", "bf16x3") if f in args.forms or f == value_form]
+    forms = [f for f in ALL_FORMS if f in args.forms or f == value_form]
     cpu_mt = None
     if not args.no_cpu_baseline:
         progress(f"CPU baseline: reference on sample {spec['cpu']} ({cpu_threads()} threads)")
@@ -609,7 +614,7 @@ def run_batch(args):
     if args.autotune:
         tuned = plan.autotune(reps=2)
     res = {}
-    for f in ("fp64", "fp32", "bf16x3"):
+    for f in ALL_FORMS:
         if f in args.forms:
             res[f] = run_form(run, f, args.steps, args.warmup, args, keep_x64=(f == "fp64"))
     main_ = res.get("fp32") or next(iter(res.values()))

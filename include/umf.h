@@ -204,11 +204,19 @@ int umf_get_stats(const umf_plan* plan, umf_stats* out);
  *     on v_mfma_f32_16x16x32_bf16: each FP32 operand is split into two bfloat16 pieces,
  *     x = x0 + x1, and the three products x0 y0 + x0 y1 + x1 y0 are accumulated in FP32 (about
  *     16 significant bits per product).  Fronts, pivot blocks, panels, solve and refinement are
- *     those of UMF_PREC_FP32.
+ *     those of UMF_PREC_FP32.  The split runs on the levels whose GEMM work lies mostly in fronts
+ *     of 1024+ pivots; the other levels keep FP32 MFMA.
+ *   UMF_PREC_FP32T64: the top three separator levels (0..2; every level of a plan with at most
+ *     three) stored and factored in FP64, pivot blocks, TRSM and Schur updates alike; the levels
+ *     below in FP32 as UMF_PREC_FP32.  The lowest FP64 level assembles from the FP32 blocks of
+ *     the level below, converting them; the solve sweeps the FP32 levels in FP32 and the top
+ *     levels in FP64.  umf_solve refines as UMF_PREC_FP32, umf_apply applies the mixed factors.
+ *     Not served (umf_server_*); umf_plan_save stores such a plan unfactored.
  * Takes effect at the next umf_factor (the plan must be factored again). */
 #define UMF_PREC_FP64 0
 #define UMF_PREC_FP32 1
 #define UMF_PREC_BF16X3 2
+#define UMF_PREC_FP32T64 3
 int umf_set_precision(umf_plan* plan, int precision, double tol, int max_iter);
 
 /* Per-kernel timing (no reference counterpart; the reference's HASCPP11TIMING timers are

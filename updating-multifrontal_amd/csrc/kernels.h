@@ -38,6 +38,7 @@ struct MatDev {
   i64 off;
   int ld;
   int cols;
+  int lev;  // tree level of the matrix (merged level in a batch)
 };
 
 }  // namespace umf

@@ -100,12 +100,15 @@ __global__ __launch_bounds__(256) void k_zero_fly(const int* __restrict__ ids, c
 // CSR scatter triples and 64x64 tiles of block copies into each listed matrix.  All writes of
 // one matrix are to distinct entries (the planner emits disjoint programs), so no ordering or
 // atomics are needed; the matrix must have been zeroed by k_zero_lower.
+// Block sources of level lev32 or deeper are read from the FP32 pool A32 (the FP64 top levels of
+// the mixed form assembling from the FP32 levels below them); lev32 = INT_MAX otherwise.
 template <class T>
 __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ ids, const MatDev* __restrict__ mats,
                                                   T* __restrict__ A, const i64* __restrict__ tri_off,
                                                   const Triple* __restrict__ tri, const i64* __restrict__ tile_off,
                                                   const Tile* __restrict__ tiles, const BlockOp* __restrict__ ops,
-                                                  const double* __restrict__ csr_val) {
+                                                  const double* __restrict__ csr_val, const float* __restrict__ A32,
+                                                  int lev32) {
   const int mid = ids[blockIdx.y];
   const MatDev M = mats[mid];
   T* a = A + M.off;
@@ -119,13 +122,16 @@ __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ ids, c
     const Tile tl = tiles[ti];
     const BlockOp op = ops[tl.op];
     const MatDev S = mats[op.src];
+    const bool src32 = S.lev >= lev32;
     const T* s = A + S.off;
+    const float* s32 = src32 ? A32 + S.off : nullptr;
     const bool trans = op.flags & 1, lower = op.flags & 2;
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
       int r = tl.r0 + (e & 63), c = tl.c0 + (e >> 6);
       if (r >= op.nr || c >= op.nc) continue;
       if (lower && r < c) continue;
-      T v = trans ? s[(i64)(op.sc0 + r) * S.ld + (op.sr0 + c)] : s[(i64)(op.sc0 + c) * S.ld + (op.sr0 + r)];
+      const i64 si = trans ? (i64)(op.sc0 + r) * S.ld + (op.sr0 + c) : (i64)(op.sc0 + c) * S.ld + (op.sr0 + r);
+      const T v = src32 ? (T)s32[si] : s[si];
       a[(i64)(op.dc0 + c) * M.ld + (op.dr0 + r)] = v;
     }
   }

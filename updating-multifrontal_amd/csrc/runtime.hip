@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -30,6 +31,8 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+#define TOP64 3  // the mixed form's (UMF_PREC_FP32T64) FP64 levels: 0..TOP64-1
 
 #define HIPCHK(x)                                                                        \
   do {                                                                                   \
@@ -385,6 +388,356 @@ const std::vector<int>& general_fronts(umf_plan* p, int ell) {
   return p->gen_fronts[ell];
 }
 
+// Start of a factorization pass over schedule S: its id list to the device (unless the caller
+// holds it there: ids non-null), the error flags and the arrival counters cleared (a launch
+// whose wait timed out may leave a count behind, which must not reach the next factorization).
+int factor_begin(umf_plan* p, const Sched& S, const int*& ids) {
+  if (!ids) {
+    int rc = ensure_ids(p, S.ids.size());
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
+    p->ids_h = S.ids;
+    if (!p->ids_h.empty())
+      HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
+                            p->stream));
+    ids = p->d_ids;
+  }
+  const int NF = p->P.nfronts();
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
+  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)std::min((NF + 255) / 256, 1024)), dim3(256), 0, p->stream,
+                     p->d_arrive, NF);
+  return UMF_OK;
+}
+
+// One level of the factorization over the schedule S (its id list on the device at ids), fronts
+// of type T in `arena`.  a32 / lev32: in the mixed form (FP64 top levels over FP32 ones) the FP32
+// pool, whose matrices (MatDev::lev >= lev32) the assembly of the lowest FP64 level reads and
+// converts; otherwise null / INT_MAX.
+template <class T>
+int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena, bool split, const float* a32,
+                 int lev32) {
+  const Plan& P = p->P;
+  auto lowerN = [](double N) { return N * (N + 1) / 2; };
+  const Sched::Lvl& L = S.lv[ell];
+  g_level = ell;
+  // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
+  // then per triple (12 B index + 8 B value read + 8 B write), per copied element 16 B
+  auto asm_work = [&](int off, int cnt, double& zb, double& ab) {
+    zb = ab = 0;
+    for (int i = 0; i < cnt; i++) {
+      int mid = S.ids[off + i];
+      zb += 8.0 * lowerN(P.mats[mid].rows);
+      ab += 28.0 * (P.tri_off[mid + 1] - P.tri_off[mid]);
+      for (i64 t = P.tile_off[mid]; t < P.tile_off[mid + 1]; t++) {
+        const BlockOp& op = P.ops[P.tiles[t].op];
+        int r = std::min(64, op.nr - P.tiles[t].r0), c = std::min(64, op.nc - P.tiles[t].c0);
+        ab += 16.0 * r * c;
+      }
+    }
+  };
+  // 1. separators of this level rebuilt from their children (upper levels) or from the CSR
+  //    operator (leaf level): zero + scatter
+  if (L.sm_cnt > 0) {
+    double zb, ab;
+    asm_work(L.sm_off, L.sm_cnt, zb, ab);
+    launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>,
+           dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+           (const MatDev*)p->d_mats, arena);
+    launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+           (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
+           (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val, a32, lev32);
+  }
+  // 2'. small leaf fronts: assembly, factorization and Schur complement in one launch
+  if (L.lf_cnt > 0) {
+    double fl = 0, by = 0;
+    for (int i = 0; i < L.lf_cnt; i++) {
+      const int f = S.ids[L.lf_off + i];
+      const double n = P.fronts[f].n, m = P.fronts[f].m;
+      fl += 2.0 * n * n * n / 3.0 + m * n * n + n * m * (m + 1);  // factor + inverse, panel, Schur
+      by += sizeof(T) * (lowerN(n) + m * n + lowerN(m) + NB * NB) + 20.0 * (P.tri_off[f + 1] - P.tri_off[f]);
+    }
+    launch(p, "k_leaf", fl, by, k_leaf<T>, dim3(persistent_grid(p, (const void*)k_leaf<T>, L.lf_cnt)),
+           ids + L.lf_off, L.lf_cnt, p->d_err + 4, (const FrontDev*)p->d_fronts, arena, (const i64*)p->d_tri_off,
+           (const Triple*)p->d_tri, (const double*)p->d_val, p->d_err);
+  }
+  if (L.f_cnt > 0) [&] {
+  // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
+  //    64 at a time.  Levels with many fronts go left-looking (each block column is formed
+  //    once by an MFMA GEMM over the finished columns, then factored; no trailing
+  //    read-modify-write).  Levels with few large fronts go right-looking in column groups of
+  //    GW: each block updates only the rest of its group (K = 64), and the group then updates
+  //    the trailing matrix once (K = GW), so the trailing read-modify-write happens once per GW
+  //    columns.
+  // The choice of algorithm (left- or right-looking, fused or lazy column groups) fixes the
+  // order of every front's floating-point operations, so it is made from the plan's whole
+  // level, never from the fronts this pass happens to factor: an Update then computes every
+  // dirty front exactly as a full factorization of the new operator would, bit for bit, and a
+  // front's arithmetic does not depend on what else shares its launches.
+  const std::vector<int>& level_fronts = general_fronts(p, ell);
+  const bool left = level_fronts.size() >= 32;
+  // BF16-split GEMMs (split) only on levels whose GEMM work (~ n N^2 per front, over the
+  // plan's whole level) lies mostly in fronts of 1024+ pivots: the split pays where the GEMM
+  // stages are MFMA-bound (3D 127^3: 1259 -> 834 ms per step); on smaller fronts they are bound
+  // by memory and latency (2D 1023^2, fronts of up to 2045 pivots: 21.2 -> 20.9 ms), where the
+  // split only costs accuracy (refinement iterations)
+  double wbig = 0, wall = 0;
+  for (int f : level_fronts) {
+    const double n = P.fronts[f].n, N = n + P.fronts[f].m, wf = n * N * N;
+    wall += wf;
+    if (n >= 1024) wbig += wf;
+  }
+  const bool sp = split && 2 * wbig >= wall && wall > 0;
+  // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
+  // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
+  bool fly = left && !P.rest_off.empty();
+  for (int i = 0; fly && i < L.f_cnt; i++) fly = P.fly_off[S.ids[L.f_off + i]] >= 0;
+  // 2. front assembly
+  {
+    double zb, ab;
+    asm_work(L.f_off, L.f_cnt, zb, ab);
+    if (fly) {
+      zb = ab = 0;
+      for (int i = 0; i < L.f_cnt; i++) {
+        const int f = S.ids[L.f_off + i];
+        const double n = P.fronts[f].n, m = P.fronts[f].m;
+        zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
+        ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
+      }
+      launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>,
+             dim3(per_item(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt), L.f_cnt),
+             ids + L.f_off, (const FrontDev*)p->d_fronts, arena);
+      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
+             (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
+             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val, a32, lev32);
+    } else {
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
+             ids + L.f_off, (const MatDev*)p->d_mats, arena);
+      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
+             (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
+             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val, a32, lev32);
+    }
+  }
+  const int GW = 256;
+  // 128 x 128 tiles (half the operand traffic) for a stage whose largest front has 16+ of them
+  // and whose fronts have 2048+ in all: 3D 95^3 FP64 trailing update 319 -> 302 ms; fronts of a
+  // few 128-tiles waste the partial tiles (256-problem batch FP64 Schur 13.5 -> 27.3 ms), and
+  // the 2D stages stay below 2048 tiles
+  // The trailing update is blocked on two levels: a finished group of GW columns updates only
+  // the rest of its outer group of GW2 columns (K = GW), and the finished outer group updates
+  // the trailing matrix once with K = GW2.  At K = GW the tile GEMMs are bound by operand and
+  // C traffic: with K = 1024 the 3D 95^3 FP64 trailing update goes from 43 to 52 TFLOP/s, and
+  // the FP32 / BF16-split tiles of a 16k-front microbenchmark from 76 / 103 to 126 / 247;
+  // GW2 = 2048 measured the same as 1024.
+  const int GW2 = p->tune_gw2;
+  // 128 x 128 GEMM tiles for a stage over `fronts` fronts of at most `per_front` such tiles
+  auto wide = [&](i64 per_front, i64 fronts) {
+    return p->tune_tile == 128 || (p->tune_tile == 0 && per_front >= 16 && per_front * fronts >= 2048);
+  };
+  // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
+  // diagonal-block columns); levels of few fronts spread each front over the whole chip
+  bool xcd = L.f_cnt >= 64;
+  int act = L.f_cnt;  // fronts taking part in the current step (a prefix: n > k0)
+  std::vector<int> seg_t, suf;  // per active front: panel row tiles, flops, bytes of the step
+  std::vector<double> seg_f, seg_b;
+  // suf[i] = max t[i..cnt); then, unless the runs would save at least half of the one
+  // launch's workgroups, suf is flattened to one run (each launch costs a few microseconds)
+  auto suffix_max = [&](const std::vector<int>& t, int cnt) {
+    suf.assign(cnt + 1, 0);
+    for (int i = cnt - 1; i >= 0; i--) suf[i] = std::max(suf[i + 1], t[i]);
+    i64 one = (i64)suf[0] * cnt, runs = 0;
+    for (int i = 0; i < cnt;) {
+      int e = i + 1;
+      while (e < cnt && 2 * suf[e] > suf[i]) e++;
+      runs += (i64)suf[i] * (e - i);
+      i = e;
+    }
+    if (2 * runs > one)
+      for (int i = 0; i < cnt; i++) suf[i] = suf[0];
+  };
+  auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((act + 7) / 8) * per_front); };
+  auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
+    const double nc = cend - s, nr = rend - s;
+    return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
+  };
+  // right-looking levels: a column group whose first step runs fused (k_chol_dpanel) applies
+  // its block columns lazily -- each step's panel takes the GEMM over the group's earlier
+  // columns and updates the group's later diagonal blocks -- instead of a group GEMM launch
+  // after every step (one launch fewer on the step chain)
+  bool lazy = false;
+  for (int k0 = 0; k0 < L.maxn; k0 += NB) {
+    while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
+    xcd = act >= 64;
+    const int g0 = k0 / GW * GW, gend = g0 + GW;
+    int rows = 0;
+    for (int i = 0; i < act; i++) {
+      const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+      if (k0 < F.n) rows = std::max(rows, F.n + F.m - k0 - std::min(NB, F.n - k0));
+    }
+    // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
+    // repeats the diagonal factorization in each) run diagonal block and panel in one launch;
+    // decided over the whole level (above), whose fronts bound this pass's, so a fused launch
+    // of this pass fits one round too
+    const int ptd = std::max(1, (rows + NB - 1) / NB);
+    i64 fit = 0;  // workgroups of the step over the whole level
+    if (!left) {
+      int rows_all = 0, act_all = 0;
+      for (int f : level_fronts) {
+        const FrontInfo& F = P.fronts[f];
+        if (k0 >= F.n) continue;
+        act_all++;
+        rows_all = std::max(rows_all, F.n + F.m - k0 - std::min(NB, F.n - k0));
+      }
+      fit = (i64)std::max(1, (rows_all + NB - 1) / NB) * act_all;
+    }
+    if (k0 == g0) lazy = !left && fit <= 512;
+    const bool fused = !left && (lazy || fit <= 512);
+    const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
+    const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
+    const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
+    int tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
+    double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
+    seg_t.assign(act, 0);
+    seg_f.assign(act, 0.0);
+    seg_b.assign(act, 0.0);
+    for (int i = 0; i < act; i++) {
+      const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+      if (k0 >= F.n) continue;
+      int kb = std::min(NB, F.n - k0), N = F.n + F.m;
+      double K = left ? k0 : lazy ? k0 - g0 : 0;
+      double s0 = k0 + kb, nr = N - s0;
+      // the block's factorization (+ left-looking: its update by the K finished columns)
+      fdiag += (double)kb * kb * kb / 3.0 + (double)kb * kb * K * (left ? 1 : 0);
+      // TRSM (substitution count) + left-looking GEMM + (lazy groups) the later diagonal
+      // blocks' update
+      const double dcols = lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
+      seg_t[i] = (N - k0 - kb + NB - 1) / NB;
+      seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + dcols * kb * kb;
+      seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
+      fpan += seg_f[i];
+      bpan += seg_b[i];
+      if (left) continue;
+      const int ce = std::min(F.n, gend);
+      if (!lazy && k0 + kb < ce) {
+        tin = std::max(tin, tiles_of(k0 + kb, N, ce));
+        const double e = lower_elems(s0, N, ce);
+        fin += 2.0 * kb * e;
+        bin += 16.0 * e + 8.0 * kb * (nr + ce - s0);
+      }
+      if (group_done) {  // the group's update of the rest of its outer group
+        const int kg = std::min(GW, F.n - g0), sg = g0 + kg, ce = std::min(F.n, hend);
+        if (sg < ce) {
+          ttr = std::max(ttr, tiles_of(sg, N, ce));
+          ttr128 = std::max(ttr128, tiles_of(sg, N, ce, 128));
+          const double e = lower_elems(sg, N, ce);
+          ftr += 2.0 * kg * e;
+          btr += 16.0 * e + 8.0 * kg * (N - sg + ce - sg);
+        }
+      }
+      if (outer_done) {  // the outer group's trailing update
+        const int kg = std::min(GW2, F.n - h0), sg = h0 + kg;
+        if (sg < F.n) {
+          tou = std::max(tou, tiles_of(sg, N, F.n));
+          tou128 = std::max(tou128, tiles_of(sg, N, F.n, 128));
+          const double e = lower_elems(sg, N, F.n);
+          fou += 2.0 * kg * e;
+          bou += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
+        }
+      }
+    }
+    if (fused)
+      launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, act),
+             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, lazy ? g0 : k0, lazy ? gend : 0,
+             xcd ? ptd : 0, act, p->d_arrive, p->d_err);
+    else
+      launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
+             arena, k0, left ? 1 : 0, p->d_err);
+    // the panel launches over at most a few runs of the active fronts: a run ends where the
+    // largest row-tile count of the fronts after it has halved, so that a level of many small
+    // fronts and a few tall ones does not give every front the tallest one's grid
+    suffix_max(seg_t, act);
+    for (int i = 0; !fused && rows > 0 && i < act;) {
+      const int pt = suf[i];
+      int e = i;
+      double f = 0, b = 0;
+      while (e < act && (e == i || 2 * suf[e] > pt)) {
+        f += seg_f[e];
+        b += seg_b[e];
+        e++;
+      }
+      const int c = e - i;
+      const bool xc = c >= 64;
+      if (pt > 0)
+        launch(p, "k_chol_panel", f, b, k_chol_panel<T>, xc ? dim3(8 * ((c + 7) / 8) * pt) : dim3(pt, c),
+               ids + L.f_off + i, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xc ? pt : 0, c,
+               fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
+      i = e;
+    }
+    if (tin > 0)
+      launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
+             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
+    auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
+      if (wide(t128, act))
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), dim3(t128, act), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
+      else
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
+    };
+    if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
+    if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
+  }
+  // 4. Schur complement into the trailing block (every front of the level)
+  if (L.maxm > 0) {
+    // per front: 64- and 128-tiles of the trailing block, flops, bytes; launched over runs
+    // ending where the largest tile count after them halves, as the panel steps
+    std::vector<int> t64(L.f_cnt), t128(L.f_cnt);
+    seg_f.assign(L.f_cnt, 0.0);
+    seg_b.assign(L.f_cnt, 0.0);
+    for (int i = 0; i < L.f_cnt; i++) {
+      const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
+      t64[i] = tiles_of(F.n, F.n + F.m, F.n + F.m);
+      t128[i] = tiles_of(F.n, F.n + F.m, F.n + F.m, 128);
+      seg_f[i] = (double)F.n * F.m * (F.m + 1);
+      seg_b[i] = 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
+    }
+    suffix_max(t128, L.f_cnt);
+    std::vector<int> suf128 = suf;
+    suffix_max(t64, L.f_cnt);
+    for (int i = 0; i < L.f_cnt;) {
+      const int tiles = suf[i], tiles128 = suf128[i];
+      int e = i;
+      double fl = 0, by = 0;
+      while (e < L.f_cnt && (e == i || 2 * suf[e] > tiles)) {
+        fl += seg_f[e];
+        by += seg_b[e];
+        e++;
+      }
+      const int c = e - i;
+      const bool xc = c >= 64;
+      auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
+      if (tiles > 0) {
+        if (wide(tiles128, c))
+          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
+        else
+          launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(sp), grid(tiles), ids + L.f_off + i,
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
+      }
+      i = e;
+    }
+  }
+  }();
+  // 5. separators of this level receive both neighbours' Schur blocks
+  if (L.s_cnt > 0) {
+    double by = 0;
+    for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
+    launch(p, "k_sep_add", 0, by, k_sep_add<T>,
+           dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.s_cnt), L.s_cnt), ids + L.s_off,
+           (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
+  }
+  return UMF_OK;
+}
+
 // Run the factorization over the dirty schedule on the plan stream, fronts of type T in `arena`.
 // ids_pre: the schedule's id list already on the device (a factorization being captured into a
 // graph: no synchronization, allocation or copy may enter the stream).
@@ -396,343 +749,32 @@ int run_factor(umf_plan* p, const Plan::Dirty& D, T* arena, bool split, const in
   if (!S_pre) S_own = make_sched(P, D, true);
   const Sched& S = S_pre ? *S_pre : S_own;
   const int* ids = ids_pre;
-  if (!ids) {
-    int rc = ensure_ids(p, S.ids.size());
-    if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(p->stream));  // the previous id list may still be in flight
-    p->ids_h = S.ids;
-    if (!p->ids_h.empty())
-      HIPCHK(hipMemcpyAsync(p->d_ids, p->ids_h.data(), sizeof(int) * p->ids_h.size(), hipMemcpyHostToDevice,
-                            p->stream));
-    ids = p->d_ids;
-  }
-  const int NF = P.nfronts();
-  // error flags, and the fused diagonal kernel's arrival counters: a launch whose wait timed out
-  // may leave a count behind, which must not reach the next factorization
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, p->stream, p->d_err, 4);
-  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)std::min((NF + 255) / 256, 1024)), dim3(256), 0, p->stream,
-                     p->d_arrive, NF);
-  auto lowerN = [](double N) { return N * (N + 1) / 2; };
+  int rc = factor_begin(p, S, ids);
+  if (rc) return rc;
   for (int ell = P.L - 1; ell >= 0; ell--) {
-    const Sched::Lvl& L = S.lv[ell];
-    g_level = ell;
-    // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
-    // then per triple (12 B index + 8 B value read + 8 B write), per copied element 16 B
-    auto asm_work = [&](int off, int cnt, double& zb, double& ab) {
-      zb = ab = 0;
-      for (int i = 0; i < cnt; i++) {
-        int mid = S.ids[off + i];
-        zb += 8.0 * lowerN(P.mats[mid].rows);
-        ab += 28.0 * (P.tri_off[mid + 1] - P.tri_off[mid]);
-        for (i64 t = P.tile_off[mid]; t < P.tile_off[mid + 1]; t++) {
-          const BlockOp& op = P.ops[P.tiles[t].op];
-          int r = std::min(64, op.nr - P.tiles[t].r0), c = std::min(64, op.nc - P.tiles[t].c0);
-          ab += 16.0 * r * c;
-        }
-      }
-    };
-    // 1. separators of this level rebuilt from their children (upper levels) or from the CSR
-    //    operator (leaf level): zero + scatter
-    if (L.sm_cnt > 0) {
-      double zb, ab;
-      asm_work(L.sm_off, L.sm_cnt, zb, ab);
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>,
-             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
-             (const MatDev*)p->d_mats, arena);
-      launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
-             (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
-             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
-    }
-    // 2'. small leaf fronts: assembly, factorization and Schur complement in one launch
-    if (L.lf_cnt > 0) {
-      double fl = 0, by = 0;
-      for (int i = 0; i < L.lf_cnt; i++) {
-        const int f = S.ids[L.lf_off + i];
-        const double n = P.fronts[f].n, m = P.fronts[f].m;
-        fl += 2.0 * n * n * n / 3.0 + m * n * n + n * m * (m + 1);  // factor + inverse, panel, Schur
-        by += sizeof(T) * (lowerN(n) + m * n + lowerN(m) + NB * NB) + 20.0 * (P.tri_off[f + 1] - P.tri_off[f]);
-      }
-      launch(p, "k_leaf", fl, by, k_leaf<T>, dim3(persistent_grid(p, (const void*)k_leaf<T>, L.lf_cnt)),
-             ids + L.lf_off, L.lf_cnt, p->d_err + 4, (const FrontDev*)p->d_fronts, arena, (const i64*)p->d_tri_off,
-             (const Triple*)p->d_tri, (const double*)p->d_val, p->d_err);
-    }
-    if (L.f_cnt > 0) [&] {
-    // 3. (decided first: it shapes the assembly) blocked partial Cholesky of the pivot columns,
-    //    64 at a time.  Levels with many fronts go left-looking (each block column is formed
-    //    once by an MFMA GEMM over the finished columns, then factored; no trailing
-    //    read-modify-write).  Levels with few large fronts go right-looking in column groups of
-    //    GW: each block updates only the rest of its group (K = 64), and the group then updates
-    //    the trailing matrix once (K = GW), so the trailing read-modify-write happens once per GW
-    //    columns.
-    // The choice of algorithm (left- or right-looking, fused or lazy column groups) fixes the
-    // order of every front's floating-point operations, so it is made from the plan's whole
-    // level, never from the fronts this pass happens to factor: an Update then computes every
-    // dirty front exactly as a full factorization of the new operator would, bit for bit, and a
-    // front's arithmetic does not depend on what else shares its launches.
-    const std::vector<int>& level_fronts = general_fronts(p, ell);
-    const bool left = level_fronts.size() >= 32;
-    // BF16-split GEMMs (split) only on levels whose GEMM work (~ n N^2 per front, over the
-    // plan's whole level) lies mostly in fronts of 1024+ pivots: the split pays where the GEMM
-    // stages are MFMA-bound (3D 127^3: 1259 -> 834 ms per step); on smaller fronts they are bound
-    // by memory and latency (2D 1023^2, fronts of up to 2045 pivots: 21.2 -> 20.9 ms), where the
-    // split only costs accuracy (refinement iterations)
-    double wbig = 0, wall = 0;
-    for (int f : level_fronts) {
-      const double n = P.fronts[f].n, N = n + P.fronts[f].m, wf = n * N * N;
-      wall += wf;
-      if (n >= 1024) wbig += wf;
-    }
-    const bool sp = split && 2 * wbig >= wall && wall > 0;
-    // left-looking leaf fronts take the entries of their below-diagonal tiles in the panel step
-    // (fly): only the diagonal blocks and the trailing block are zeroed and scattered
-    bool fly = left && !P.rest_off.empty();
-    for (int i = 0; fly && i < L.f_cnt; i++) fly = P.fly_off[S.ids[L.f_off + i]] >= 0;
-    // 2. front assembly
-    {
-      double zb, ab;
-      asm_work(L.f_off, L.f_cnt, zb, ab);
-      if (fly) {
-        zb = ab = 0;
-        for (int i = 0; i < L.f_cnt; i++) {
-          const int f = S.ids[L.f_off + i];
-          const double n = P.fronts[f].n, m = P.fronts[f].m;
-          zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
-          ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
-        }
-        launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>,
-               dim3(per_item(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt), L.f_cnt),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena);
-        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
-               (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
-               (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
-      } else {
-        launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
-               ids + L.f_off, (const MatDev*)p->d_mats, arena);
-        launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
-               (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
-               (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
-      }
-    }
-    const int GW = 256;
-    // 128 x 128 tiles (half the operand traffic) for a stage whose largest front has 16+ of them
-    // and whose fronts have 2048+ in all: 3D 95^3 FP64 trailing update 319 -> 302 ms; fronts of a
-    // few 128-tiles waste the partial tiles (256-problem batch FP64 Schur 13.5 -> 27.3 ms), and
-    // the 2D stages stay below 2048 tiles
-    // The trailing update is blocked on two levels: a finished group of GW columns updates only
-    // the rest of its outer group of GW2 columns (K = GW), and the finished outer group updates
-    // the trailing matrix once with K = GW2.  At K = GW the tile GEMMs are bound by operand and
-    // C traffic: with K = 1024 the 3D 95^3 FP64 trailing update goes from 43 to 52 TFLOP/s, and
-    // the FP32 / BF16-split tiles of a 16k-front microbenchmark from 76 / 103 to 126 / 247;
-    // GW2 = 2048 measured the same as 1024.
-    const int GW2 = p->tune_gw2;
-    // 128 x 128 GEMM tiles for a stage over `fronts` fronts of at most `per_front` such tiles
-    auto wide = [&](i64 per_front, i64 fronts) {
-      return p->tune_tile == 128 || (p->tune_tile == 0 && per_front >= 16 && per_front * fronts >= 2048);
-    };
-    // levels of many fronts: keep all tiles of a front on one XCD (one L2 fetches its shared
-    // diagonal-block columns); levels of few fronts spread each front over the whole chip
-    bool xcd = L.f_cnt >= 64;
-    int act = L.f_cnt;  // fronts taking part in the current step (a prefix: n > k0)
-    std::vector<int> seg_t, suf;  // per active front: panel row tiles, flops, bytes of the step
-    std::vector<double> seg_f, seg_b;
-    // suf[i] = max t[i..cnt); then, unless the runs would save at least half of the one
-    // launch's workgroups, suf is flattened to one run (each launch costs a few microseconds)
-    auto suffix_max = [&](const std::vector<int>& t, int cnt) {
-      suf.assign(cnt + 1, 0);
-      for (int i = cnt - 1; i >= 0; i--) suf[i] = std::max(suf[i + 1], t[i]);
-      i64 one = (i64)suf[0] * cnt, runs = 0;
-      for (int i = 0; i < cnt;) {
-        int e = i + 1;
-        while (e < cnt && 2 * suf[e] > suf[i]) e++;
-        runs += (i64)suf[i] * (e - i);
-        i = e;
-      }
-      if (2 * runs > one)
-        for (int i = 0; i < cnt; i++) suf[i] = suf[0];
-    };
-    auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((act + 7) / 8) * per_front); };
-    auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
-      const double nc = cend - s, nr = rend - s;
-      return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
-    };
-    // right-looking levels: a column group whose first step runs fused (k_chol_dpanel) applies
-    // its block columns lazily -- each step's panel takes the GEMM over the group's earlier
-    // columns and updates the group's later diagonal blocks -- instead of a group GEMM launch
-    // after every step (one launch fewer on the step chain)
-    bool lazy = false;
-    for (int k0 = 0; k0 < L.maxn; k0 += NB) {
-      while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
-      xcd = act >= 64;
-      const int g0 = k0 / GW * GW, gend = g0 + GW;
-      int rows = 0;
-      for (int i = 0; i < act; i++) {
-        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
-        if (k0 < F.n) rows = std::max(rows, F.n + F.m - k0 - std::min(NB, F.n - k0));
-      }
-      // right-looking steps whose panel fits one round of workgroups (2 per CU: the fused kernel
-      // repeats the diagonal factorization in each) run diagonal block and panel in one launch;
-      // decided over the whole level (above), whose fronts bound this pass's, so a fused launch
-      // of this pass fits one round too
-      const int ptd = std::max(1, (rows + NB - 1) / NB);
-      i64 fit = 0;  // workgroups of the step over the whole level
-      if (!left) {
-        int rows_all = 0, act_all = 0;
-        for (int f : level_fronts) {
-          const FrontInfo& F = P.fronts[f];
-          if (k0 >= F.n) continue;
-          act_all++;
-          rows_all = std::max(rows_all, F.n + F.m - k0 - std::min(NB, F.n - k0));
-        }
-        fit = (i64)std::max(1, (rows_all + NB - 1) / NB) * act_all;
-      }
-      if (k0 == g0) lazy = !left && fit <= 512;
-      const bool fused = !left && (lazy || fit <= 512);
-      const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
-      const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
-      const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
-      int tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
-      double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
-      seg_t.assign(act, 0);
-      seg_f.assign(act, 0.0);
-      seg_b.assign(act, 0.0);
-      for (int i = 0; i < act; i++) {
-        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
-        if (k0 >= F.n) continue;
-        int kb = std::min(NB, F.n - k0), N = F.n + F.m;
-        double K = left ? k0 : lazy ? k0 - g0 : 0;
-        double s0 = k0 + kb, nr = N - s0;
-        // the block's factorization (+ left-looking: its update by the K finished columns)
-        fdiag += (double)kb * kb * kb / 3.0 + (double)kb * kb * K * (left ? 1 : 0);
-        // TRSM (substitution count) + left-looking GEMM + (lazy groups) the later diagonal
-        // blocks' update
-        const double dcols = lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
-        seg_t[i] = (N - k0 - kb + NB - 1) / NB;
-        seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + dcols * kb * kb;
-        seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
-        fpan += seg_f[i];
-        bpan += seg_b[i];
-        if (left) continue;
-        const int ce = std::min(F.n, gend);
-        if (!lazy && k0 + kb < ce) {
-          tin = std::max(tin, tiles_of(k0 + kb, N, ce));
-          const double e = lower_elems(s0, N, ce);
-          fin += 2.0 * kb * e;
-          bin += 16.0 * e + 8.0 * kb * (nr + ce - s0);
-        }
-        if (group_done) {  // the group's update of the rest of its outer group
-          const int kg = std::min(GW, F.n - g0), sg = g0 + kg, ce = std::min(F.n, hend);
-          if (sg < ce) {
-            ttr = std::max(ttr, tiles_of(sg, N, ce));
-            ttr128 = std::max(ttr128, tiles_of(sg, N, ce, 128));
-            const double e = lower_elems(sg, N, ce);
-            ftr += 2.0 * kg * e;
-            btr += 16.0 * e + 8.0 * kg * (N - sg + ce - sg);
-          }
-        }
-        if (outer_done) {  // the outer group's trailing update
-          const int kg = std::min(GW2, F.n - h0), sg = h0 + kg;
-          if (sg < F.n) {
-            tou = std::max(tou, tiles_of(sg, N, F.n));
-            tou128 = std::max(tou128, tiles_of(sg, N, F.n, 128));
-            const double e = lower_elems(sg, N, F.n);
-            fou += 2.0 * kg * e;
-            bou += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
-          }
-        }
-      }
-      if (fused)
-        launch(p, "k_chol_dpanel", fdiag + fpan, bpan, k_chol_dpanel<T>, xcd ? dim3(xcd_grid(ptd)) : dim3(ptd, act),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, lazy ? g0 : k0, lazy ? gend : 0,
-               xcd ? ptd : 0, act, p->d_arrive, p->d_err);
-      else
-        launch(p, "k_chol_diag", fdiag, 0, k_chol_diag<T>, dim3(act), ids + L.f_off, (const FrontDev*)p->d_fronts,
-               arena, k0, left ? 1 : 0, p->d_err);
-      // the panel launches over at most a few runs of the active fronts: a run ends where the
-      // largest row-tile count of the fronts after it has halved, so that a level of many small
-      // fronts and a few tall ones does not give every front the tallest one's grid
-      suffix_max(seg_t, act);
-      for (int i = 0; !fused && rows > 0 && i < act;) {
-        const int pt = suf[i];
-        int e = i;
-        double f = 0, b = 0;
-        while (e < act && (e == i || 2 * suf[e] > pt)) {
-          f += seg_f[e];
-          b += seg_b[e];
-          e++;
-        }
-        const int c = e - i;
-        const bool xc = c >= 64;
-        if (pt > 0)
-          launch(p, "k_chol_panel", f, b, k_chol_panel<T>, xc ? dim3(8 * ((c + 7) / 8) * pt) : dim3(pt, c),
-                 ids + L.f_off + i, (const FrontDev*)p->d_fronts, arena, k0, left ? 1 : 0, xc ? pt : 0, c,
-                 fly ? 1 : 0, (const int*)p->d_fly_tab, (const Triple*)p->d_fly_tri, (const double*)p->d_val);
-        i = e;
-      }
-      if (tin > 0)
-        launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
-      auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
-        if (wide(t128, act))
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), dim3(t128, act), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
-        else
-          launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
-      };
-      if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
-      if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
-    }
-    // 4. Schur complement into the trailing block (every front of the level)
-    if (L.maxm > 0) {
-      // per front: 64- and 128-tiles of the trailing block, flops, bytes; launched over runs
-      // ending where the largest tile count after them halves, as the panel steps
-      std::vector<int> t64(L.f_cnt), t128(L.f_cnt);
-      seg_f.assign(L.f_cnt, 0.0);
-      seg_b.assign(L.f_cnt, 0.0);
-      for (int i = 0; i < L.f_cnt; i++) {
-        const FrontInfo& F = P.fronts[S.ids[L.f_off + i]];
-        t64[i] = tiles_of(F.n, F.n + F.m, F.n + F.m);
-        t128[i] = tiles_of(F.n, F.n + F.m, F.n + F.m, 128);
-        seg_f[i] = (double)F.n * F.m * (F.m + 1);
-        seg_b[i] = 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
-      }
-      suffix_max(t128, L.f_cnt);
-      std::vector<int> suf128 = suf;
-      suffix_max(t64, L.f_cnt);
-      for (int i = 0; i < L.f_cnt;) {
-        const int tiles = suf[i], tiles128 = suf128[i];
-        int e = i;
-        double fl = 0, by = 0;
-        while (e < L.f_cnt && (e == i || 2 * suf[e] > tiles)) {
-          fl += seg_f[e];
-          by += seg_b[e];
-          e++;
-        }
-        const int c = e - i;
-        const bool xc = c >= 64;
-        auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
-        if (tiles > 0) {
-          if (wide(tiles128, c))
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
-                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
-          else
-            launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(sp), grid(tiles), ids + L.f_off + i,
-                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
-        }
-        i = e;
-      }
-    }
-    }();
-    // 5. separators of this level receive both neighbours' Schur blocks
-    if (L.s_cnt > 0) {
-      double by = 0;
-      for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
-      launch(p, "k_sep_add", 0, by, k_sep_add<T>,
-             dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.s_cnt), L.s_cnt), ids + L.s_off,
-             (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
-    }
+    rc = factor_level<T>(p, S, ids, ell, arena, split, nullptr, INT_MAX);
+    if (rc) return rc;
   }
-  (void)NF;
+  g_level = -1;
+  HIPCHK(hipGetLastError());
+  return UMF_OK;
+}
+
+// The mixed form (UMF_PREC_FP32T64): levels TOP64 and below in FP32 (arena32), the top levels in
+// FP64 (arena); the lowest FP64 level assembles the FP32 blocks of the level below, converting.
+int run_factor_mixed(umf_plan* p, const Plan::Dirty& D, const int* ids_pre, const Sched* S_pre) {
+  const Plan& P = p->P;
+  Sched S_own;
+  if (!S_pre) S_own = make_sched(P, D, true);
+  const Sched& S = S_pre ? *S_pre : S_own;
+  const int* ids = ids_pre;
+  int rc = factor_begin(p, S, ids);
+  if (rc) return rc;
+  for (int ell = P.L - 1; ell >= 0; ell--) {
+    rc = ell >= TOP64 ? factor_level<float>(p, S, ids, ell, p->arena32, false, nullptr, INT_MAX)
+                      : factor_level<double>(p, S, ids, ell, p->arena, false, p->arena32, TOP64);
+    if (rc) return rc;
+  }
   g_level = -1;
   HIPCHK(hipGetLastError());
   return UMF_OK;
@@ -769,14 +811,16 @@ int run_factor_lu(umf_plan* p, const Plan::Dirty& D) {
              (const MatDev*)p->d_mats, A);
       launch(p, "k_assemble", 0, 0, k_assemble<double>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
              (const MatDev*)p->d_mats, A, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
-             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+             (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val,
+           (const float*)nullptr, INT_MAX);
     }
     if (L.f_cnt == 0) continue;
     launch(p, "k_zero_full", 0, 0, k_zero_full<double>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
            ids + L.f_off, (const MatDev*)p->d_mats, A);
     launch(p, "k_assemble", 0, 0, k_assemble<double>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
            (const MatDev*)p->d_mats, A, (const i64*)p->d_tri_off, (const Triple*)p->d_tri, (const i64*)p->d_tile_off,
-           (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val);
+           (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val,
+           (const float*)nullptr, INT_MAX);
     int act = L.f_cnt;  // fronts sorted by pivot-block size: the prefix with n > k0 takes part
     for (int k0 = 0; k0 < L.maxn; k0 += NB) {
       while (act > 0 && P.fronts[S.ids[L.f_off + act - 1]].n <= k0) act--;
@@ -821,6 +865,7 @@ int factor_any(umf_plan* p, const Plan::Dirty& D, const int* ids_pre = nullptr, 
   if (p->prec != UMF_PREC_FP64) {
     if (!p->arena32)
       HIPCHK(hipMalloc((void**)&p->arena32, sizeof(float) * std::max<i64>(p->P.arena_elems, 1)));
+    if (p->prec == UMF_PREC_FP32T64) return run_factor_mixed(p, D, ids_pre, S_pre);
     return run_factor<float>(p, D, p->arena32, p->prec == UMF_PREC_BF16X3, ids_pre, S_pre);
   }
   return run_factor<double>(p, D, p->arena, false, ids_pre, S_pre);
@@ -878,11 +923,12 @@ int factor_graph(umf_plan* p, const Plan::Dirty& D) {
   return UMF_OK;
 }
 
-// Copy cnt front-pool entries at offset off of the current factors to the host as double.
-int read_pool(umf_plan* p, i64 off, size_t cnt, std::vector<double>& h) {
+// Copy cnt front-pool entries at offset off of the current factors to the host as double; lev:
+// the tree level of the matrix they belong to (the mixed form holds levels < TOP64 in FP64).
+int read_pool(umf_plan* p, i64 off, size_t cnt, std::vector<double>& h, int lev) {
   h.resize(cnt);
   if (!cnt) return UMF_OK;
-  if (p->fact_prec == UMF_PREC_FP64) {
+  if (p->fact_prec == UMF_PREC_FP64 || (p->fact_prec == UMF_PREC_FP32T64 && lev < TOP64)) {
     HIPCHK(hipMemcpy(h.data(), p->arena + off, sizeof(double) * cnt, hipMemcpyDeviceToHost));
     return UMF_OK;
   }
@@ -927,6 +973,10 @@ int ensure_solve_scratch(umf_plan* p) {
     if (!p->d_xs) HIPCHK(hipMalloc((void**)&p->d_xs, sizeof(double) * nx));
     if (!p->d_w) HIPCHK(hipMalloc((void**)&p->d_w, sizeof(double) * nw));
     return UMF_OK;
+  }
+  if (p->fact_prec == UMF_PREC_FP32T64) {  // the top levels sweep in FP64
+    if (!p->d_xs) HIPCHK(hipMalloc((void**)&p->d_xs, sizeof(double) * nx));
+    if (!p->d_w) HIPCHK(hipMalloc((void**)&p->d_w, sizeof(double) * nw));
   }
   if (!p->d_xs32) HIPCHK(hipMalloc((void**)&p->d_xs32, sizeof(float) * nx));
   if (!p->d_w32) {  // zeroed once: columns a narrower refinement never writes hold finite values
@@ -982,9 +1032,14 @@ int next_epoch(umf_plan* p) {
 template <class T>
 // ids_dev: device buffer of at least all_lists(P).ids.size() ints for the id list (the no-alloc
 // solve's scratch), or null for the plan's own (grown on demand).
-int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = nullptr) {
+// lo, hi, phases: the levels [lo, hi) only (hi < 0: all), the upward / forward phase (phases & 1)
+// and the downward / backward phase (phases & 2); the mixed form sweeps its FP32 and FP64 levels
+// in separate calls.
+int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = nullptr, int lo = 0, int hi = -1,
+          int phases = 3) {
   const Plan& P = p->P;
   int L = P.L;
+  if (hi < 0) hi = L;
   if (!p->sweep_lists) p->sweep_lists.reset(new AllLists(all_lists(P)));
   const AllLists& A = *p->sweep_lists;
   if (!ids_dev) {
@@ -1113,13 +1168,15 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
     }
   };
   if (mode == 0) {
-    for (int ell = L - 1; ell >= 1; ell--) {
-      fwd(ell);
-      launch(p, "k_sep_acc", 0, 0, k_sep_acc<T>, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
-             (const int*)p->d_dofs, w, (const T*)xs);
+    if (phases & 1) {
+      for (int ell = hi - 1; ell >= std::max(lo, 1); ell--) {
+        fwd(ell);
+        launch(p, "k_sep_acc", 0, 0, k_sep_acc<T>, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps,
+               fr, (const int*)p->d_dofs, w, (const T*)xs);
+      }
+      if (lo == 0) fwd(0);
     }
-    fwd(0);
-    for (int ell = 0; ell < L; ell++) {
+    for (int ell = lo; (phases & 2) && ell < hi; ell++) {
       work(ell, fl, by);
       g_level = ell;
       if (!blocked(ell)) {
@@ -1144,10 +1201,10 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       }
     }
   } else {
-    for (int ell = L - 1; ell >= 0; ell--)
+    for (int ell = hi - 1; (phases & 1) && ell >= lo; ell--)
       launch(p, "k_apply_up", 0, 0, k_apply_up<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr, (const T*)arena,
              (const int*)p->d_dofs, w, xs, nr);
-    for (int ell = 1; ell < L; ell++) {
+    for (int ell = std::max(lo, 1); (phases & 2) && ell < hi; ell++) {
       launch(p, "k_apply_down", 0, 0, k_apply_down<T>, dim3(A.f_cnt[ell]), ids + A.f_off[ell], fr,
              (const T*)arena, xs, nr);
       launch(p, "k_sep_acc", 0, 0, k_sep_acc<T>, dim3(A.s_cnt[ell]), ids + A.s_off[ell], (const SepDev*)p->d_seps, fr,
@@ -1157,6 +1214,29 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
   g_level = -1;
   HIPCHK(hipGetLastError());
   return UMF_OK;
+}
+
+// Sweep of the mixed form (UMF_PREC_FP32T64) on the FP32 work block w32: the FP32 levels
+// [TOP64, L) in FP32, the work block converted to FP64 (d_w), the top levels in FP64 both ways,
+// converted back, the FP32 levels' downward / backward phase.  Solve (mode 0) and Apply (1).
+int sweep_mixed(umf_plan* p, int mode, int nr, float* w32) {
+  const int L = p->P.L, top = std::min(TOP64, L);
+  const i64 n16 = (i64)p->P.n * RB;
+  const unsigned g = (unsigned)((n16 + 255) / 256);
+  int rc = sweep<float>(p, mode, nr, p->arena32, w32, p->d_xs32, nullptr, top, L, 1);
+  if (rc) return rc;
+  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<double, float>, dim3(g), n16, (const float*)w32, p->d_w);
+  rc = sweep<double>(p, mode, nr, p->arena, p->d_w, p->d_xs, nullptr, 0, top, 3);
+  if (rc) return rc;
+  launch(p, "k_cvt", 0, 12.0 * n16, k_cvt<float, double>, dim3(g), n16, (const double*)p->d_w, w32);
+  return sweep<float>(p, mode, nr, p->arena32, w32, p->d_xs32, nullptr, top, L, 2);
+}
+
+// The preconditioner's sweep on the FP32 work block w (all 16 columns): the FP32 factors, or
+// the mixed form's.
+int precond_sweep(umf_plan* p, float* w) {
+  if (p->fact_prec == UMF_PREC_FP32T64) return sweep_mixed(p, 0, RB, w);
+  return sweep<float>(p, 0, RB, p->arena32, w, p->d_xs32);
 }
 
 // FP32 form of Solve for nr <= RB columns: preconditioned conjugate gradients in FP64 (FP64
@@ -1197,7 +1277,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
   HIPCHK(hipMemcpyAsync(h, S2, sizeof(h), hipMemcpyDeviceToHost, p->stream));
   // z = M^{-1} r, p = z, r.z
   launch(p, "k_pc_in", 0, 1.5 * vb, k_pc_in, dim3(ge), n, lcw, (const double*)R, w);
-  int rc = sweep<float>(p, 0, RB, p->arena32, w, p->d_xs32);
+  int rc = precond_sweep(p, w);
   if (rc) return rc;
   launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ, Pd,
          p->d_part);
@@ -1226,7 +1306,7 @@ int pcg(umf_plan* p, const double* in, double* out, int ld, int nr, int& it_out,
       if (berr[c] <= p->ir_tol) active &= ~(1 << c);
     }
     if (!active) break;
-    rc = sweep<float>(p, 0, RB, p->arena32, w, p->d_xs32);  // w holds (float) r from k_cg_xr
+    rc = precond_sweep(p, w);  // w holds (float) r from k_cg_xr
     if (rc) return rc;
     launch(p, "k_pc_out", 0, 3.5 * vb, k_pc_out, dim3(gr), n, lcw, (const float*)w, (const double*)R, QZ,
            (double*)nullptr, p->d_part);
@@ -1283,7 +1363,8 @@ int solve_apply(umf_plan* p, int mode, const double* in, double* out, int nrhs, 
              nr);
     } else if (mode == 1) {
       launch(p, "k_to_rows", 0, 12.0 * n * nr, k_to_rows<float>, dim3(gb), n, ci, ldd, nr, p->d_w32);
-      rc = sweep<float>(p, mode, nr, p->arena32, p->d_w32, p->d_xs32);
+      rc = p->fact_prec == UMF_PREC_FP32T64 ? sweep_mixed(p, mode, nr, p->d_w32)
+                                            : sweep<float>(p, mode, nr, p->arena32, p->d_w32, p->d_xs32);
       if (rc) return rc;
       launch(p, "k_from_rows", 0, 12.0 * n * nr, k_from_rows<float>, dim3(go), n, (const float*)p->d_w32, co, ldd,
              nr);
@@ -1561,7 +1642,11 @@ int setup_device(umf_plan* p, umf_plan** out) {
   // device descriptors
   std::vector<MatDev>& mats = p->mats_h;
   mats.clear();
-  for (const MatDesc& M : P.mats) mats.push_back(MatDev{M.off, M.rows, M.cols});
+  for (const MatDesc& M : P.mats) mats.push_back(MatDev{M.off, M.rows, M.cols, 0});
+  for (int ell = 0; ell < P.L; ell++) {
+    for (int f : P.box_gid[ell]) mats[f].lev = ell;
+    for (int sp : P.sep_gid[ell]) mats[P.nfronts() + sp].lev = ell;
+  }
   i64 xo = 0;
   for (const FrontInfo& F : P.fronts) {
     FrontDev D{};
@@ -1800,7 +1885,9 @@ int umf_plan_save(umf_plan* p, const char* path) {
   h.ir_tol = p->ir_tol;
   h.arena_elems = P.arena_elems;
   h.elem_size = !p->factored ? 0 : p->fact_prec != UMF_PREC_FP64 ? 4 : 8;
-  if (p->lu) {  // LU factors (pivots, inverted U blocks) are not stored: loaded plans refactor
+  // LU factors (pivots, inverted U blocks) and the mixed form's two pools are not stored: loaded
+  // plans refactor
+  if (p->lu || p->fact_prec == UMF_PREC_FP32T64) {
     h.factored = 0;
     h.elem_size = 0;
   }
@@ -1839,7 +1926,7 @@ int umf_plan_load(const char* path, umf_plan** out) {
   };
   if (std::fread(&h, sizeof h, 1, f) != 1 || !std::equal(kPlanMagic, kPlanMagic + 8, h.magic))
     return bad("not a plan file");
-  auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_BF16X3; };
+  auto prec_ok = [](int v) { return v >= UMF_PREC_FP64 && v <= UMF_PREC_FP32T64; };
   if (h.n <= 0 || h.nnz < 0 || (h.elem_size != 0 && h.elem_size != 4 && h.elem_size != 8) || h.arena_elems < 0 ||
       !prec_ok(h.prec) || !prec_ok(h.fact_prec) || !(h.ir_tol > 0) || !std::isfinite(h.ir_tol) || h.ir_max < 1 ||
       (h.elem_size != 0 && (h.elem_size == 8) != (h.fact_prec == UMF_PREC_FP64)))
@@ -2146,7 +2233,7 @@ int umf_get_front_block(umf_plan* p, int level, int box, int which, double* out,
   const FrontDev& F = p->fronts_h[f];
   int n = F.n, m = F.m, N = F.N;
   std::vector<double> h;
-  int rc = read_pool(p, F.off, (size_t)N * N, h);
+  int rc = read_pool(p, F.off, (size_t)N * N, h, level);
   if (rc) return rc;
   auto at = [&](int i, int j) { return h[(size_t)j * N + i]; };
   std::vector<double> r;
@@ -2291,7 +2378,7 @@ int umf_get_sep_block(umf_plan* p, int level, int sep, double* out, int* k) {
   *k = S.k;
   if (!out) return UMF_OK;
   std::vector<double> h;
-  int rc = read_pool(p, S.off, (size_t)S.k * S.k, h);
+  int rc = read_pool(p, S.off, (size_t)S.k * S.k, h, level);
   if (rc) return rc;
   if (p->lu) {  // full matrix
     std::copy(h.begin(), h.end(), out);
@@ -2315,7 +2402,8 @@ int umf_get_stats(const umf_plan* p, umf_stats* out) {
 int umf_set_precision(umf_plan* p, int precision, double tol, int max_iter) {
   if (!p) return fail(UMF_EINVAL, "null plan");
   if (p->servers > 0) return fail(UMF_ESTATE, "a server answers requests against this plan's factors");
-  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3)
+  if (precision != UMF_PREC_FP64 && precision != UMF_PREC_FP32 && precision != UMF_PREC_BF16X3 &&
+      precision != UMF_PREC_FP32T64)
     return fail(UMF_EINVAL, "unknown precision");
   if (!(tol > 0) || max_iter < 1) return fail(UMF_EINVAL, "tolerance must be positive, max_iter >= 1");
   if (p->lu && precision != UMF_PREC_FP64) return fail(UMF_EINVAL, "LU fronts (unsymmetric operators) are FP64 only");
@@ -2670,6 +2758,7 @@ int umf_server_create(umf_plan* base, int nslots, double pool_bytes, umf_server*
   if (base->host_only || !base->factored) return fail(UMF_ESTATE, "the base plan must be factored");
   if (base->P.forest) return fail(UMF_EINVAL, "serving takes a single-problem plan");
   if (base->lu) return fail(UMF_EINVAL, "serving takes Cholesky fronts (symmetric operators)");
+  if (base->fact_prec == UMF_PREC_FP32T64) return fail(UMF_EINVAL, "serving takes the FP64, FP32 or BF16X3 form");
   if (nslots < 1 || nslots > 32) return fail(UMF_EINVAL, "slots must be 1..32");
   if (!(pool_bytes >= 0)) return fail(UMF_EINVAL, "bad pool size");
   HIPCHK(hipStreamSynchronize(base->stream));

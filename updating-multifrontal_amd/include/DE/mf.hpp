@@ -52,8 +52,14 @@ class Front {
   // Precision form (no reference counterpart): Precision::FP32 keeps and factors the fronts in
   // FP32 on FP32 MFMA, and Solve refines in FP64 to backward error <= tol.  BF16X3 is FP32
   // with the Schur and trailing-update GEMMs on BF16 MFMA over two bfloat16 pieces of each
-  // operand (3 products, FP32 accumulation).  Factor() again after switching.
-  enum class Precision { FP64 = UMF_PREC_FP64, FP32 = UMF_PREC_FP32, BF16X3 = UMF_PREC_BF16X3 };
+  // operand (3 products, FP32 accumulation).  FP32T64 keeps the top three separator levels in
+  // FP64 and the levels below in FP32, refined as FP32.  Factor() again after switching.
+  enum class Precision {
+    FP64 = UMF_PREC_FP64,
+    FP32 = UMF_PREC_FP32,
+    BF16X3 = UMF_PREC_BF16X3,
+    FP32T64 = UMF_PREC_FP32T64
+  };
   void SetPrecision(Precision p, double tol = 1e-10, int maxIter = 50) {
     check(umf_set_precision(plan_, (int)p, tol, maxIter));
   }

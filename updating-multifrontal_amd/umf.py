@@ -26,7 +26,7 @@ _lib = C.CDLL(LIB_PATH)
 UMF_OK = 0
 UMF_EINVAL, UMF_EGEOM, UMF_EDEVICE, UMF_ENOMEM, UMF_EPATTERN, UMF_EUPDATEMAP, UMF_ESTATE, UMF_ENOCONV, UMF_EBUSY = \
     range(-1, -10, -1)
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32t64": 3}  # UMF_PREC_*
 
 
 class RefinementError(ArithmeticError):
@@ -192,7 +192,9 @@ class _MF:
         """'fp64' (default, the reference's arithmetic) or 'fp32': FP32 fronts on FP32 MFMA, Solve
         refined in FP64 (preconditioned CG) to backward error <= tol; 'bf16x3': the fp32 form
         with the Schur and trailing-update GEMMs on BF16 MFMA over two bfloat16 pieces of each
-        FP32 operand (3 piece products, FP32 accumulation).  Factor() again afterwards."""
+        FP32 operand (3 piece products, FP32 accumulation); 'fp32t64': the top three separator
+        levels in FP64, the levels below in FP32, Solve refined as 'fp32'.  Factor() again
+        afterwards."""
         _check(_set_prec(self._h, PRECISIONS[precision], float(tol), int(max_iter)))
 
     def use_graph(self, on=True):
