@@ -57,15 +57,17 @@ __global__ void k_zero_ints(int* __restrict__ p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-// Zero the lower triangle (row >= col) of each listed matrix.
+// Zero the lower triangle (row >= col) of each listed matrix: one wave per column (4 columns per
+// workgroup step), so short columns do not leave most of a 256-thread workgroup idle.
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids, const MatDev* __restrict__ mats,
                                                     T* __restrict__ A) {
   const MatDev M = mats[ids[blockIdx.y]];
   T* a = A + M.off;
-  for (int c = blockIdx.x; c < M.cols; c += gridDim.x) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = 4 * blockIdx.x + w; c < M.cols; c += 4 * gridDim.x) {
     T* col = a + (i64)c * M.ld;
-    for (int r = c + threadIdx.x; r < M.ld; r += 256) col[r] = T(0);
+    for (int r = c + lane; r < M.ld; r += 64) col[r] = T(0);
   }
 }
 
@@ -73,14 +75,14 @@ __global__ __launch_bounds__(256) void k_zero_lower(const int* __restrict__ ids,
 // fly_off): zero only what is read before it is written -- the lower triangles of the 64-column
 // blocks' diagonal blocks and of the trailing block.
 // Work unit u < nb = ceil(n/64): diagonal block u, 64 x 64 per workgroup; u >= nb: trailing
-// column n + (u - nb).
+// columns n + 4(u - nb) .. n + 4(u - nb) + 3, one wave each.
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_fly(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                                   T* __restrict__ A) {
   const FrontDev F = fr[ids[blockIdx.y]];
   T* a = A + F.off;
   const int nb = (F.n + NB - 1) / NB;
-  for (int u = blockIdx.x; u < nb + F.m; u += gridDim.x) {
+  for (int u = blockIdx.x; u < nb + (F.m + 3) / 4; u += gridDim.x) {
     if (u < nb) {
       const int c0 = NB * u, kb = min(NB, F.n - c0);
       T* blk = a + (i64)c0 * F.N + c0;
@@ -89,10 +91,11 @@ __global__ __launch_bounds__(256) void k_zero_fly(const int* __restrict__ ids, c
         const int i = e & (NB - 1), j = e / NB;
         if (i >= j && i < kb && j < kb) blk[(i64)j * F.N + i] = T(0);
       }
-    } else {
-      const int c = F.n + (u - nb);
-      T* col = a + (i64)c * F.N;
-      for (int r = c + threadIdx.x; r < F.N; r += 256) col[r] = T(0);
+    } else {  // trailing columns, one wave each
+      for (int c = F.n + 4 * (u - nb) + (threadIdx.x >> 6); c < min(F.N, F.n + 4 * (u - nb + 1)); c += 4) {
+        T* col = a + (i64)c * F.N;
+        for (int r = c + (threadIdx.x & 63); r < F.N; r += 64) col[r] = T(0);
+      }
     }
   }
 }
@@ -148,8 +151,10 @@ __global__ __launch_bounds__(256) void k_sep_add(const int* __restrict__ ids, co
   T* a = A + S.off;
   const T* s0 = A + B0.off + (i64)(B0.n + S.pos[0]) * B0.N + B0.n + S.pos[0];
   const T* s1 = A + B1.off + (i64)(B1.n + S.pos[1]) * B1.N + B1.n + S.pos[1];
-  for (int c = blockIdx.x; c < S.k; c += gridDim.x)
-    for (int r = c + threadIdx.x; r < S.k; r += 256) {
+  // one wave per column, 4 columns per workgroup step
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = 4 * blockIdx.x + w; c < S.k; c += 4 * gridDim.x)
+    for (int r = c + lane; r < S.k; r += 64) {
       T v = a[(i64)c * S.k + r] + s0[(i64)c * B0.N + r];
       a[(i64)c * S.k + r] = v + s1[(i64)c * B1.N + r];
     }

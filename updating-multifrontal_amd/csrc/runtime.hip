@@ -441,7 +441,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
     double zb, ab;
     asm_work(L.sm_off, L.sm_cnt, zb, ab);
     launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>,
-           dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
+           dim3(per_item(std::min((std::max(L.max_k, 1) + 3) / 4, 256), L.sm_cnt), L.sm_cnt), ids + L.sm_off,
            (const MatDev*)p->d_mats, arena);
     launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.s_grid, L.sm_cnt), L.sm_cnt), ids + L.sm_off,
            (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
@@ -504,13 +504,13 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
         ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
       }
       launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>,
-             dim3(per_item(std::min((L.maxn + NB - 1) / NB + L.maxm, 64), L.f_cnt), L.f_cnt),
+             dim3(per_item(std::min((L.maxn + NB - 1) / NB + (L.maxm + 3) / 4, 64), L.f_cnt), L.f_cnt),
              ids + L.f_off, (const FrontDev*)p->d_fronts, arena);
       launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
              (const MatDev*)p->d_mats, arena, (const i64*)p->d_rest_off, (const Triple*)p->d_rest_tri,
              (const i64*)p->d_tile_off, (const Tile*)p->d_tiles, (const BlockOp*)p->d_ops, (const double*)p->d_val, a32, lev32);
     } else {
-      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(per_item(std::min(L.maxN, 256), L.f_cnt), L.f_cnt),
+      launch(p, "k_zero_lower", 0, zb, k_zero_lower<T>, dim3(per_item(std::min((L.maxN + 3) / 4, 256), L.f_cnt), L.f_cnt),
              ids + L.f_off, (const MatDev*)p->d_mats, arena);
       launch(p, "k_assemble", 0, ab, k_assemble<T>, dim3(per_item(L.f_grid, L.f_cnt), L.f_cnt), ids + L.f_off,
              (const MatDev*)p->d_mats, arena, (const i64*)p->d_tri_off, (const Triple*)p->d_tri,
@@ -732,7 +732,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
     double by = 0;
     for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
     launch(p, "k_sep_add", 0, by, k_sep_add<T>,
-           dim3(per_item(std::min(std::max(L.max_k, 1), 256), L.s_cnt), L.s_cnt), ids + L.s_off,
+           dim3(per_item(std::min((std::max(L.max_k, 1) + 3) / 4, 256), L.s_cnt), L.s_cnt), ids + L.s_off,
            (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
   }
   return UMF_OK;
