@@ -1535,7 +1535,8 @@ __global__ __launch_bounds__(256) void k_solve_bwd(const int* __restrict__ ids, 
 // them, with the front's right-hand-side block in LDS instead of the global scratch: the only global traffic is the factor, the work block W and
 // (forward) the boundary rows k_sep_acc reads; the steps synchronize with LDS-only barriers,
 // so the next diagonal block's inverse, loaded during a step, stays in flight across them.
-#define XLR 320  // rows of X held in LDS (2D leaf fronts of maxSideLength 8 have N = 285)
+#define XLR 320   // rows of X held in LDS (2D leaf fronts of maxSideLength 8 have N = 285)
+#define XLR2 1152  // FP32 fronts up to this many rows: the larger instance (74 KB of LDS)
 
 // Workgroup barrier for LDS data: LDS writes before it are seen by LDS reads after it; global
 // loads in flight stay in flight (__syncthreads would wait for them).
@@ -1545,13 +1546,13 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Forward sweep (as k_solve_fwd) of fronts with N <= XLR.
-template <class T>
+// Forward sweep (as k_solve_fwd) of fronts with N <= XR.
+template <class T, int XR = XLR>
 __global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids, int cnt, int* __restrict__ queue,
                                                      const FrontDev* __restrict__ fr, const T* __restrict__ A,
                                                      const int* __restrict__ dofs, T* __restrict__ W,
                                                      T* __restrict__ xs) {
-  __shared__ T X[XLR * RB];
+  __shared__ T X[XR * RB];
   __shared__ T xb[NB][RB];
   for (int item; (item = queue_next(queue, cnt)) >= 0;) {
     const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first
@@ -1627,12 +1628,12 @@ __global__ __launch_bounds__(256) void k_solve_fwd_l(const int* __restrict__ ids
   }
 }
 
-// Backward sweep (as k_solve_bwd) of fronts with N <= XLR.
-template <class T>
+// Backward sweep (as k_solve_bwd) of fronts with N <= XR.
+template <class T, int XR = XLR>
 __global__ __launch_bounds__(256) void k_solve_bwd_l(const int* __restrict__ ids, int cnt, int* __restrict__ queue,
                                                      const FrontDev* __restrict__ fr, const T* __restrict__ A,
                                                      const int* __restrict__ dofs, T* __restrict__ W) {
-  __shared__ T X[XLR * RB];
+  __shared__ T X[XR * RB];
   __shared__ T zb[NB][RB];
   for (int item; (item = queue_next(queue, cnt)) >= 0;) {
     const FrontDev F = fr[ids[cnt - 1 - item]];  // the list ascends in size: largest first

@@ -1131,13 +1131,21 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       };
       // work is attributed to the launches in proportion to their fronts
       auto part = [&](int c, double x) { return x * c / std::max(cnt, 1); };
-      {  // fronts of at most XLR rows: X in LDS
-        const int e = run_end(XLR);
+      {  // fronts of at most XLR rows (FP32: XLR2) with X in LDS
+        int e = run_end(XLR);
         if (e > i)
           launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_l<T>,
                  dim3(persistent_grid(p, (const void*)k_solve_fwd_l<T>, e - i)), li + i, e - i, p->d_err + 4, fr,
                  (const T*)arena, (const int*)p->d_dofs, w, xs);
         i = e;
+        if constexpr (sizeof(T) == 4) {
+          e = run_end(XLR2);
+          if (e > i)
+            launch(p, "k_solve_fwd", part(e - i, fl), part(e - i, by), k_solve_fwd_l<T, XLR2>,
+                   dim3(persistent_grid(p, (const void*)k_solve_fwd_l<T, XLR2>, e - i)), li + i, e - i, p->d_err + 4,
+                   fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
+          i = e;
+        }
       }
       if (sizeof(T) == 8) {  // (FP32 keeps the streaming kernel: the register form measured slower there)
         int e = run_end(6 * NB);
@@ -1180,18 +1188,29 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
       work(ell, fl, by);
       g_level = ell;
       if (!blocked(ell)) {
-        // fronts of at most XLR rows (a prefix: the lists are sorted by size) with X in LDS
+        // fronts of at most XLR rows (FP32: then XLR2; prefixes: the lists are sorted by size) with X
+        // in LDS
         const int cnt = A.f_cnt[ell];
-        int e = 0;
-        while (e < cnt && P.fronts[A.ids[A.f_off[ell] + e]].n + P.fronts[A.ids[A.f_off[ell] + e]].m <= XLR) e++;
-        const double fe = cnt ? (double)e / cnt : 0.0;
+        auto upto = [&](int from, int rows) {
+          int e = from;
+          while (e < cnt && P.fronts[A.ids[A.f_off[ell] + e]].n + P.fronts[A.ids[A.f_off[ell] + e]].m <= rows) e++;
+          return e;
+        };
+        const int e = upto(0, XLR), e2 = sizeof(T) == 4 ? upto(e, XLR2) : e;
+        auto frac = [&](int a, int b) { return cnt ? (double)(b - a) / cnt : 0.0; };
         if (e > 0)
-          launch(p, "k_solve_bwd", fl * fe, by * fe, k_solve_bwd_l<T>,
+          launch(p, "k_solve_bwd", fl * frac(0, e), by * frac(0, e), k_solve_bwd_l<T>,
                  dim3(persistent_grid(p, (const void*)k_solve_bwd_l<T>, e)), ids + A.f_off[ell], e, p->d_err + 4, fr,
                  (const T*)arena, (const int*)p->d_dofs, w);
-        if (cnt > e)
-          launch(p, "k_solve_bwd", fl * (1 - fe), by * (1 - fe), k_solve_bwd<T>, dim3(cnt - e), ids + A.f_off[ell] + e,
-                 fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
+        if constexpr (sizeof(T) == 4) {
+          if (e2 > e)
+            launch(p, "k_solve_bwd", fl * frac(e, e2), by * frac(e, e2), k_solve_bwd_l<T, XLR2>,
+                   dim3(persistent_grid(p, (const void*)k_solve_bwd_l<T, XLR2>, e2 - e)), ids + A.f_off[ell] + e,
+                   e2 - e, p->d_err + 4, fr, (const T*)arena, (const int*)p->d_dofs, w);
+        }
+        if (cnt > e2)
+          launch(p, "k_solve_bwd", fl * frac(e2, cnt), by * frac(e2, cnt), k_solve_bwd<T>, dim3(cnt - e2),
+                 ids + A.f_off[ell] + e2, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
       } else {
         int nbi, nbt;
         blocks(ell, nbi, nbt);
