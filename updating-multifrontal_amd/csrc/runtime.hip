@@ -418,20 +418,21 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
                  int lev32) {
   const Plan& P = p->P;
   auto lowerN = [](double N) { return N * (N + 1) / 2; };
+  constexpr double E = sizeof(T);  // bytes per front entry
   const Sched::Lvl& L = S.lv[ell];
   g_level = ell;
   // algorithmic traffic of the assembly of a list of matrices: zero the lower triangle,
-  // then per triple (12 B index + 8 B value read + 8 B write), per copied element 16 B
+  // then per triple (12 B index + 8 B value read + one entry written), per copied element two entries
   auto asm_work = [&](int off, int cnt, double& zb, double& ab) {
     zb = ab = 0;
     for (int i = 0; i < cnt; i++) {
       int mid = S.ids[off + i];
-      zb += 8.0 * lowerN(P.mats[mid].rows);
-      ab += 28.0 * (P.tri_off[mid + 1] - P.tri_off[mid]);
+      zb += E * lowerN(P.mats[mid].rows);
+      ab += (20.0 + E) * (P.tri_off[mid + 1] - P.tri_off[mid]);
       for (i64 t = P.tile_off[mid]; t < P.tile_off[mid + 1]; t++) {
         const BlockOp& op = P.ops[P.tiles[t].op];
         int r = std::min(64, op.nr - P.tiles[t].r0), c = std::min(64, op.nc - P.tiles[t].c0);
-        ab += 16.0 * r * c;
+        ab += 2 * E * r * c;
       }
     }
   };
@@ -500,8 +501,8 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       for (int i = 0; i < L.f_cnt; i++) {
         const int f = S.ids[L.f_off + i];
         const double n = P.fronts[f].n, m = P.fronts[f].m;
-        zb += 8.0 * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
-        ab += 28.0 * (P.rest_off[f + 1] - P.rest_off[f]);
+        zb += E * (lowerN(m) + std::floor(n / NB) * lowerN(NB) + lowerN(n - NB * std::floor(n / NB)));
+        ab += (20.0 + E) * (P.rest_off[f + 1] - P.rest_off[f]);
       }
       launch(p, "k_zero_lower", 0, zb, k_zero_fly<T>,
              dim3(per_item(std::min((L.maxn + NB - 1) / NB + (L.maxm + 3) / 4, 64), L.f_cnt), L.f_cnt),
@@ -612,7 +613,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       const double dcols = lazy ? std::max(0.0, std::min(F.n, gend) - s0) : 0.0;
       seg_t[i] = (N - k0 - kb + NB - 1) / NB;
       seg_f[i] = nr * kb * kb + 2.0 * nr * kb * K + dcols * kb * kb;
-      seg_b[i] = (fly ? 8.0 : 16.0) * nr * kb + 8.0 * nr * K;
+      seg_b[i] = (fly ? E : 2 * E) * nr * kb + E * nr * K;
       fpan += seg_f[i];
       bpan += seg_b[i];
       if (left) continue;
@@ -621,7 +622,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
         tin = std::max(tin, tiles_of(k0 + kb, N, ce));
         const double e = lower_elems(s0, N, ce);
         fin += 2.0 * kb * e;
-        bin += 16.0 * e + 8.0 * kb * (nr + ce - s0);
+        bin += 2 * E * e + E * kb * (nr + ce - s0);
       }
       if (group_done) {  // the group's update of the rest of its outer group
         const int kg = std::min(GW, F.n - g0), sg = g0 + kg, ce = std::min(F.n, hend);
@@ -630,7 +631,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
           ttr128 = std::max(ttr128, tiles_of(sg, N, ce, 128));
           const double e = lower_elems(sg, N, ce);
           ftr += 2.0 * kg * e;
-          btr += 16.0 * e + 8.0 * kg * (N - sg + ce - sg);
+          btr += 2 * E * e + E * kg * (N - sg + ce - sg);
         }
       }
       if (outer_done) {  // the outer group's trailing update
@@ -640,7 +641,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
           tou128 = std::max(tou128, tiles_of(sg, N, F.n, 128));
           const double e = lower_elems(sg, N, F.n);
           fou += 2.0 * kg * e;
-          bou += 16.0 * e + 8.0 * kg * (N - sg + F.n - sg);
+          bou += 2 * E * e + E * kg * (N - sg + F.n - sg);
         }
       }
     }
@@ -698,7 +699,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       t64[i] = tiles_of(F.n, F.n + F.m, F.n + F.m);
       t128[i] = tiles_of(F.n, F.n + F.m, F.n + F.m, 128);
       seg_f[i] = (double)F.n * F.m * (F.m + 1);
-      seg_b[i] = 8.0 * F.n * F.m + 16.0 * lowerN(F.m);
+      seg_b[i] = E * F.n * F.m + 2 * E * lowerN(F.m);
     }
     suffix_max(t128, L.f_cnt);
     std::vector<int> suf128 = suf;
@@ -730,7 +731,7 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
   // 5. separators of this level receive both neighbours' Schur blocks
   if (L.s_cnt > 0) {
     double by = 0;
-    for (int i = 0; i < L.s_cnt; i++) by += 32.0 * lowerN(P.seps[S.ids[L.s_off + i]].k);
+    for (int i = 0; i < L.s_cnt; i++) by += 4 * E * lowerN(P.seps[S.ids[L.s_off + i]].k);
     launch(p, "k_sep_add", 0, by, k_sep_add<T>,
            dim3(per_item(std::min((std::max(L.max_k, 1) + 3) / 4, 256), L.s_cnt), L.s_cnt), ids + L.s_off,
            (const SepDev*)p->d_seps, (const FrontDev*)p->d_fronts, arena);
@@ -1093,7 +1094,7 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
     for (int f : P.box_gid[ell]) {
       double nn = P.fronts[f].n, mm = P.fronts[f].m;
       fl += nr * (nn * nn + 2 * nn * mm);
-      by += 8.0 * (nn * (nn + 1) / 2 + nn * mm) + 16.0 * nr * (nn + mm);
+      by += sizeof(T) * (nn * (nn + 1) / 2 + nn * mm + 2.0 * nr * (nn + mm));
     }
   };
   double fl, by;
