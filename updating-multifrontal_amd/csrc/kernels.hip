@@ -253,7 +253,8 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int
 // The MFMA A operand is Q and B is P, so accumulator r of a lane holds C(i, j) with i = 16-block
 // row + lane%16 and j = 16-block col + mrow(lane/16, r), and the 16 lanes of a group cover 16
 // consecutive rows of C.  Per 4-deep k step a wave reads TS/32 + TS/32 operand fragments from
-// LDS for (TS/32)^2 MFMAs.  Two LDS stages: chunk c is computed from stage c&1 while chunk
+// LDS for (TS/32)^2 MFMAs; the next step's fragments are read before this step's MFMAs issue, so
+// their LDS latency hides behind them.  Two LDS stages: chunk c is computed from stage c&1 while chunk
 // c+1 (loaded into registers one chunk earlier) is written to the other stage and chunk c+2 is
 // fetched, one barrier per 16-deep chunk.  P and Q start at the tile's first row; pn, qn >= 1.
 template <class T, int TS>
@@ -314,19 +315,24 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
   // chunk kc from LDS stage b; then the chunk after it (register set `next`) is staged and the
   // chunk DEPTH + 1 ahead is fetched into that set
   auto step = [&](int kc, int b, auto next) {
-#pragma unroll
-    for (int k4 = 0; k4 < 4 && mma; k4++) {
+    T av[2][NBK], bv[2][NBK];
+    auto rd = [&](int k4, int z) {
       const int kr = k4 * 4 + (lane >> 4);
-      T av[NBK], bv[NBK];
 #pragma unroll
       for (int x = 0; x < NBK; x++) {
-        bv[x] = sm.As[b][kr][wr * (TS / 2) + x * 16 + (lane & 15)];
-        av[x] = sm.Bs[b][kr][wc * (TS / 2) + x * 16 + (lane & 15)];
+        bv[z][x] = sm.As[b][kr][wr * (TS / 2) + x * 16 + (lane & 15)];
+        av[z][x] = sm.Bs[b][kr][wc * (TS / 2) + x * 16 + (lane & 15)];
       }
+    };
+    if (mma) rd(0, 0);
+#pragma unroll
+    for (int k4 = 0; k4 < 4 && mma; k4++) {
+      if (k4 < 3) rd(k4 + 1, (k4 + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);  // the next fragments' reads stay ahead of these MFMAs
 #pragma unroll
       for (int bi = 0; bi < NBK; bi++)
 #pragma unroll
-        for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
+        for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[k4 & 1][bj], bv[k4 & 1][bi], acc[bi][bj]);
     }
     stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
     fetch(kc + 16 * (DEPTH + 1), next);
@@ -503,12 +509,13 @@ __device__ __forceinline__ void tile_read(T (&v)[2][2][4], const T* __restrict__
       }
 }
 
-// Workgroup -> (tile, front id index).  xt == 0: 2-D grid (tile = blockIdx.x, front =
-// blockIdx.y).  xt > 0: 1-D grid of 8 * ceil(nf / 8) * xt workgroups laid out so that the xt
-// tiles of a front get workgroup ids congruent mod 8; ids are dealt to the 8 XCDs round-robin,
-// so a front's tiles run on one XCD and the operands they share are fetched into one L2.
+// Workgroup -> (tile, front id index).  xt <= 0: 2-D grid (tile = blockIdx.x, front =
+// blockIdx.y; xt < 0 is the GEMMs' band order, see gemm_body).  xt > 0: 1-D grid of
+// 8 * ceil(nf / 8) * xt workgroups laid out so that the xt tiles of a front get workgroup ids
+// congruent mod 8; ids are dealt to the 8 XCDs round-robin, so a front's tiles run on one XCD
+// and the operands they share are fetched into one L2.
 __device__ __forceinline__ bool wg_tile(int xt, int nf, int& tile, int& f) {
-  if (xt == 0) {
+  if (xt <= 0) {
     tile = blockIdx.x;
     f = blockIdx.y;
     return true;
@@ -1208,6 +1215,45 @@ __device__ __forceinline__ bool tile_of(int x, int s, int rend, int cend, int& r
   return true;
 }
 
+// The same tile set as tile_of in band order: bands of 8 tile rows, each band column by column
+// (the band's tiles of one tile column are consecutive).  A run of 64 consecutive indices then
+// covers about 8 x 8 tiles, which share 8 row and 8 column operand panels.
+template <int TS = 64>
+__device__ __forceinline__ bool tile_of_band(int x, int s, int rend, int cend, int& row0, int& col0) {
+  const int T = (cend - s + TS - 1) / TS, RT = (rend - s + TS - 1) / TS;
+  const int tri = T * (T + 1) / 2;
+  if (x >= tri + (RT - T) * T) return false;
+  int R;  // tile rows wholly before x
+  if (x < tri) {
+    R = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+    while (R * (R + 1) / 2 > x) R--;
+    while ((R + 1) * (R + 2) / 2 <= x) R++;
+  } else {
+    R = T + (x - tri) / T;
+  }
+  const int r0 = R & ~7, r1 = min(r0 + 8, RT) - 1, h = r1 - r0 + 1;
+  int y = x - (r0 <= T ? r0 * (r0 + 1) / 2 : tri + (r0 - T) * T);
+  const int F = min(r0 + 1, T);  // columns that every row of the band reaches
+  int rt, ct;
+  if (y < F * h) {
+    ct = y / h;
+    rt = r0 + y - ct * h;
+  } else {  // the band's triangle: column c holds rows c..r1
+    y -= F * h;
+    int c = r0 + 1, cnt = r1 - c + 1;
+    while (y >= cnt) {
+      y -= cnt;
+      c++;
+      cnt--;
+    }
+    ct = c;
+    rt = c + y;
+  }
+  row0 = s + TS * rt;
+  col0 = s + TS * ct;
+  return true;
+}
+
 // C -= P Q^T on the lower part of a region, one 64x64 tile per workgroup.
 // MODE 0: right-looking update by the finished columns [k0, k0+K), K = min(kw, n - k0):
 //         P = F[rows, k0:k0+K], Q = F[cols, k0:k0+K], C = F[s:N, s:min(n, ccap)] lower,
@@ -1237,7 +1283,17 @@ __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const Fro
     s = F.n; K = F.n; kbase = 0; rend = F.N; cend = F.N;
   }
   int row0, col0;
-  if (!tile_of<TS>(tile, s, rend, cend, row0, col0)) return;
+  if (xt < 0) {
+    // band order, the front's tiles dealt to the 8 XCDs in contiguous eighths: the grid's x
+    // dimension is a multiple of 8, so workgroup (x, y) runs on XCD x mod 8, and the tiles an
+    // XCD holds at one time lie in one or two bands and share their operand panels in its L2
+    const int TC = (cend - s + TS - 1) / TS, TR = (rend - s + TS - 1) / TS;
+    const int nt = TC * (TC + 1) / 2 + (TR - TC) * TC, chunk = (nt + 7) >> 3, j = tile >> 3;
+    if (j >= chunk) return;
+    if (!tile_of_band<TS>((tile & 7) * chunk + j, s, rend, cend, row0, col0)) return;
+  } else if (!tile_of<TS>(tile, s, rend, cend, row0, col0)) {
+    return;
+  }
   row0 = __builtin_amdgcn_readfirstlane(row0);  // wave-uniform: scalar operand addresses
   col0 = __builtin_amdgcn_readfirstlane(col0);
   constexpr int NBK = TS / 32;

@@ -556,6 +556,9 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       for (int i = 0; i < cnt; i++) suf[i] = suf[0];
   };
   auto xcd_grid = [&](int per_front) { return (unsigned)(8 * ((act + 7) / 8) * per_front); };
+  // GEMM launches over fewer fronts: a 2-D grid (tiles rounded up to a multiple of 8, front)
+  // whose kernels take each front's tiles in band order, an eighth per XCD (xt = -1)
+  auto band_grid = [](int per_front, int fronts) { return dim3((unsigned)((per_front + 7) / 8 * 8), fronts); };
   auto lower_elems = [](double s, double rend, double cend) {  // lower part of [s,rend) x [s,cend)
     const double nc = cend - s, nr = rend - s;
     return nc > 0 ? nc * nr - nc * (nc - 1) / 2 : 0.0;
@@ -674,15 +677,15 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       i = e;
     }
     if (tin > 0)
-      launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : dim3(tin, act),
-             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : 0, act);
+      launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : band_grid(tin, act),
+             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : -1, act);
     auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
       if (wide(t128, act))
-        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), dim3(t128, act), ids + L.f_off,
-               (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, 0, act);
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), band_grid(t128, act), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act);
       else
-        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : dim3(t64, act),
-               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : 0, act);
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : band_grid(t64, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : -1, act);
     };
     if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
     if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
@@ -715,14 +718,14 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       }
       const int c = e - i;
       const bool xc = c >= 64;
-      auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : dim3(t, c); };
+      auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : band_grid(t, c); };
       if (tiles > 0) {
         if (wide(tiles128, c))
           launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
-                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : 0, c);
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c);
         else
           launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(sp), grid(tiles), ids + L.f_off + i,
-                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : 0, c);
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : -1, c);
       }
       i = e;
     }
