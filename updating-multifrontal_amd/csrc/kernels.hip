@@ -257,10 +257,19 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int
 // their LDS latency hides behind them.  Two LDS stages: chunk c is computed from stage c&1 while chunk
 // c+1 (loaded into registers one chunk earlier) is written to the other stage and chunk c+2 is
 // fetched, one barrier per 16-deep chunk.  P and Q start at the tile's first row; pn, qn >= 1.
+// FP32 stage rows are padded by 16 floats: the fragment reads of one half-wave take rows kr and
+// kr + 1 (lane groups q = 0, 1) at the same columns, which then fall on the other 16 of the 32
+// LDS banks instead of the same ones (FP64 reads go 16 lanes at a time, one row: no conflict).
+template <class T>
+constexpr int kStagePad = sizeof(T) == 4 ? 16 : 0;
+// column stride of the 64-column X blocks of the panel kernels, read as X[k * stride + row] with
+// k = 4 k4 + lane / 16: FP32 strides of 16 mod 32 banks keep a half-wave's two k rows apart
+template <class T>
+constexpr int kXs = sizeof(T) == 4 ? NB + 16 : NB + 1;
 template <class T, int TS>
 struct TileSmemT {
-  T As[2][16][TS];
-  T Bs[2][16][TS];
+  T As[2][16][TS + kStagePad<T>];
+  T Bs[2][16][TS + kStagePad<T>];
 };
 
 // LOWER (P = Q, only the lower triangle of the product wanted): the wave above the diagonal
@@ -946,7 +955,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
     TileSmemT<T, 64> g;  // the in-group GEMM's stages, before the block is loaded
     struct {
       DiagSmem<T> sd;
-      T X[NB * (NB + 1)];
+      T X[NB * kXs<T>];
     } f;
   } sm;
   DiagSmem<T>& sd = sm.f.sd;
@@ -978,7 +987,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
+        X[j * kXs<T> + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
         T v = T(0);  // strict upper part: the inverse, initially zero; identity padding
         if (i >= j) v = (i < kb && j < kb) ? bv[bi][bj][r] : (i == j ? T(1) : T(0));
         sd.Ts[j * LD + i] = v;
@@ -1013,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
     const int k = k4 * 4 + q;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++) {
-      const T bv = X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
+      const T bv = X[k * kXs<T> + wr * 32 + bi * 16 + c16];
 #pragma unroll
       for (int bj = 0; bj < 2; bj++) {
         const int j = wc * 32 + bj * 16 + c16;
@@ -1043,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        X[j * (NB + 1) + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
+        X[j * kXs<T> + i] = (i < nrow && j < kb) ? acc[bi][bj][r] : T(0);
       }
   __syncthreads();
   if (wr < wc) return;
@@ -1057,11 +1066,11 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
       const int k = k4 * 4 + q;
 #pragma unroll
       for (int bi = 0; bi < 2; bi++) {
-        const T bv = X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
+        const T bv = X[k * kXs<T> + wr * 32 + bi * 16 + c16];
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) {
           if (decltype(diag_wave)::value && bj > bi) continue;
-          const T av = X[k * (NB + 1) + wc * 32 + bj * 16 + c16];
+          const T av = X[k * kXs<T> + wc * 32 + bj * 16 + c16];
           acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
         }
       }
@@ -1105,7 +1114,7 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
   __shared__ union {
     TileSmemT<T, 64> g;
     struct {
-      T X[NB * (NB + 1)];
+      T X[NB * kXs<T>];
     } c;
   } sm;
   const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
@@ -1132,7 +1141,7 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
-        sm.c.X[j * (NB + 1) + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
+        sm.c.X[j * kXs<T> + i] = (i < nrow && j < kb) ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
       }
   const T* dinv = A + F.dinv_off + (i64)(jb / NB) * NB * NB;  // Linv(j, k) at dinv[k*NB + j]
   __syncthreads();
@@ -1140,7 +1149,7 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
     const int* tb = fly_tab + F.fly_off + (jb / NB) * ((F.N + NB - 1) / NB + 1) + tile;
     for (int e = tb[0] + t; e < tb[1]; e += 256) {
       const Triple tr = fly_tri[e];
-      sm.c.X[(tr.col - jb) * (NB + 1) + (tr.row - row0)] += T(csr_val[tr.k]);
+      sm.c.X[(tr.col - jb) * kXs<T> + (tr.row - row0)] += T(csr_val[tr.k]);
     }
     __syncthreads();
   }
@@ -1158,7 +1167,7 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
       const int k = k4 * 4 + q;
 #pragma unroll
       for (int bi = 0; bi < 2; bi++) {
-        const T bv = sm.c.X[k * (NB + 1) + wr * 32 + bi * 16 + c16];
+        const T bv = sm.c.X[k * kXs<T> + wr * 32 + bi * 16 + c16];
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) {
           if (k4 >= 8 * WC + 4 * bj + 4) continue;
