@@ -414,6 +414,33 @@ def run_form(run, prec, steps, warmup, args, keep_x64=False):
             "kernels_by_level": {k: round(v["ms"], 4) for k, v in sorted(prof_lv.items())}}
 
 
+def gemm_probe(reps=10):
+    """The vendor library's GEMM (torch.matmul: hipBLASLt / rocBLAS) on this GPU, FP32 without TF32-like
+    reduced precision and FP64, at m = n = 16384, k = 1024 (one outer group of the 3D trailing update)
+    and at 8192^3: mean TFLOP/s of `reps` calls after 3 untimed ones.  A reference point for the
+    attainable MFMA rate; no part of the factorization uses it."""
+    import torch
+    torch.backends.cuda.matmul.allow_tf32 = False
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        for (m, n, k) in ((16384, 16384, 1024), (8192, 8192, 8192)):
+            a = torch.randn(m, k, device="cuda", dtype=dt)
+            b = torch.randn(k, n, device="cuda", dtype=dt)
+            c = torch.empty(m, n, device="cuda", dtype=dt)
+            for _ in range(3):
+                torch.matmul(a, b, out=c)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                torch.matmul(a, b, out=c)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"{str(dt).split('.')[-1]} {m}x{n}x{k}"] = round(2.0 * m * n * k / (ms * 1e-3) / 1e12, 1)
+            del a, b, c
+    print(json.dumps({"gemm_probe_tflops": out}), flush=True)
+
+
 def factor_graph_timing(plan, torch, reps=2):
     """Factor() alone, direct launches against the captured HIP graph (umf_set_graph): mean ms
     of `reps` calls each after one untimed call (the graph's capture)."""
@@ -467,6 +494,9 @@ def main():
     ap.add_argument("--cpu-sample", default=None, help=argparse.SUPPRESS)  # child of the CPU baseline
     ap.add_argument("--cpu-only", default=None, choices=sorted(CPU_SAMPLES),
                     help="time only the reference on the host (threaded MKL) on this sample and exit")
+    ap.add_argument("--gemm-probe", action="store_true",
+                    help="time the vendor library's FP32 and FP64 GEMM (torch.matmul) at the trailing update's "
+                         "shape on this GPU, as the attainable MFMA rate beside the published peaks, and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 1 GiB once on the GPU and exit (the HBM counters' calibration run)")
     ap.add_argument("--autotune", action="store_true",
@@ -489,6 +519,8 @@ def main():
         out = cpu_sample_threaded(args.cpu_only, args.nrhs or 16)
         print(json.dumps({"cpu_only": out, "host": host_info()}), flush=True)
         return
+    if args.gemm_probe:
+        return gemm_probe()
     if args.copy_probe:
         import torch
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
