@@ -1171,9 +1171,13 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) {
           if (k4 >= 8 * WC + 4 * bj + 4) continue;
+          // partial tiles: a product outside the block is multiplied by zero instead of skipped
+          // by a branch, so the inverse's loads stay unconditional and are issued ahead together
+          // (X is zero there and the inverse finite, and adding +-0 to an accumulator that
+          // started at +0 changes no bit)
           const T av = dinv[k * NB + WC * 32 + bj * 16 + c16];
-          if (!decltype(guard)::value || (k4 < k4end && wr * 32 + bi * 16 < nrow))
-            acc[bi][bj] = mfma4(av, bv, acc[bi][bj]);
+          const bool on = !decltype(guard)::value || (k4 < k4end && wr * 32 + bi * 16 < nrow);
+          acc[bi][bj] = mfma4(av, bv * (on ? T(1) : T(0)), acc[bi][bj]);
         }
       }
     }
