@@ -364,6 +364,94 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
   }
 }
 
+// tile_gemm_nt_t for a 128 x 128 tile on 8 waves (512 threads): waves as 2 (rows) x 4 (columns),
+// wave w holding rows 64 (w / 4) .. + 64 and columns 32 (w % 4) .. + 32 (acc[bi][bj]: 16-row
+// block bi, 16-column block bj).  The same stages, chunks, operand loads and fragment prefetch;
+// per 4-deep step a wave reads 4 P and 2 Q fragments for 8 MFMAs.  Twice the waves per tile at
+// half the accumulators each, so a SIMD holds four waves of two tiles instead of two.
+template <class T, int DEPTH = 1>
+__device__ __forceinline__ void tile_gemm_nt_8(V4<T> (&acc)[4][2], const T* __restrict__ P, int ldp, int pn,
+                                               const T* __restrict__ Q, int ldq, int qn, int K,
+                                               TileSmemT<T, 128>& sm) {
+  static_assert(DEPTH == 1 || DEPTH == 2, "one or two chunks fetched ahead");
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wr = w >> 2, wc = w & 3;
+  T pa[DEPTH][2][2], qb[DEPTH][2][2];
+  int po[2][2], qo[2][2];  // byte offsets of this lane's elements within a chunk: k = w + 8q, row = lane + 64u
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      po[q][u] = ((w + 8 * q) * ldp + min(lane + 64 * u, pn - 1)) * (int)sizeof(T);
+      qo[q][u] = ((w + 8 * q) * ldq + min(lane + 64 * u, qn - 1)) * (int)sizeof(T);
+    }
+  auto fetch = [&](int kc, auto set) {
+    constexpr int S = decltype(set)::value;
+    const int nk = max(min(16, K - kc), 0);
+    const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * (int)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * (int)sizeof(T));
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        pa[S][q][u] = buf_load<T>(rp, po[q][u]);
+        qb[S][q][u] = buf_load<T>(rq, qo[q][u]);
+      }
+  };
+  auto stage = [&](int b, auto set) {
+    constexpr int S = decltype(set)::value;
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        sm.As[b][w + 8 * q][lane + 64 * u] = pa[S][q][u];
+        sm.Bs[b][w + 8 * q][lane + 64 * u] = qb[S][q][u];
+      }
+  };
+  if (K <= 0) return;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, DEPTH - 1>;
+  auto step = [&](int kc, int b, auto next) {
+    T av[2][2], bv[2][4];
+    auto rd = [&](int k4, int z) {
+      const int kr = k4 * 4 + (lane >> 4);
+#pragma unroll
+      for (int x = 0; x < 4; x++) bv[z][x] = sm.As[b][kr][wr * 64 + x * 16 + (lane & 15)];
+#pragma unroll
+      for (int y = 0; y < 2; y++) av[z][y] = sm.Bs[b][kr][wc * 32 + y * 16 + (lane & 15)];
+    };
+    rd(0, 0);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+      if (k4 < 3) rd(k4 + 1, (k4 + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);  // the next fragments' reads stay ahead of these MFMAs
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = mfma4(av[k4 & 1][bj], bv[k4 & 1][bi], acc[bi][bj]);
+    }
+    stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 16 * (DEPTH + 1), next);
+    __syncthreads();
+  };
+  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
+  fetch(0, S0{});
+  if constexpr (DEPTH == 2) fetch(16, S1{});
+  stage(0, S0{});
+  fetch(16 * DEPTH, S0{});
+  __syncthreads();
+  if constexpr (DEPTH == 1) {
+    int b = 0;
+    for (int kc = 0; kc < K; kc += 16, b ^= 1) step(kc, b, S0{});
+  } else {
+    for (int kc = 0; kc < K; kc += 32) {
+      step(kc, 0, S1{});  // the next chunk sits in set 1
+      if (kc + 16 < K) step(kc + 16, 1, S0{});
+    }
+  }
+}
+
 // BF16-split variant of tile_gemm_nt_t for FP32 operands (the split precision form).  Each FP32
 // value x is written to LDS as two bfloat16 pieces, x ~ x_0 + x_1 (x_0 = bf16(x),
 // x_1 = bf16(x - x_0); the difference is exact in FP32), and P Q^T is formed on
@@ -1276,10 +1364,12 @@ __device__ __forceinline__ bool tile_of_band(int x, int s, int rend, int cend, i
 //         A11 + S, except on the diagonal separator blocks, which get S alone.
 // SPLIT = false: the native MFMA of T; true (T = float): the product on BF16 pieces
 // (tile_gemm_nt_bf).
-template <class T, int MODE, int TS, bool SPLIT>
+// NW = 8 (TS = 128, native MFMA): the tile on 8 waves (tile_gemm_nt_8).
+template <class T, int MODE, int TS, bool SPLIT, int NW = 4>
 __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
                                           T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
   static_assert(!SPLIT || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
+  static_assert(NW == 4 || (NW == 8 && TS == 128 && !SPLIT), "8 waves: 128-tiles on the native MFMA");
   int tile, fi;
   if (!wg_tile(xt, nf, tile, fi)) return;
   const FrontDev F = fr[ids[fi]];
@@ -1309,35 +1399,42 @@ __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const Fro
   }
   row0 = __builtin_amdgcn_readfirstlane(row0);  // wave-uniform: scalar operand addresses
   col0 = __builtin_amdgcn_readfirstlane(col0);
-  constexpr int NBK = TS / 32;
-  V4<T> acc[NBK][NBK];
+  // a wave's sub-tile: NBR x NBC blocks of 16 x 16 at (WR, WC) in the tile
+  constexpr int NBR = TS / 32, NBC = NW == 8 ? 2 : TS / 32;
+  V4<T> acc[NBR][NBC];
 #pragma unroll
-  for (int i = 0; i < NBK; i++)
+  for (int i = 0; i < NBR; i++)
 #pragma unroll
-    for (int j = 0; j < NBK; j++) acc[i][j] = vzero<T>();
+    for (int j = 0; j < NBC; j++) acc[i][j] = vzero<T>();
   // The tile's current values are read with unconditional loads at clamped (always valid)
   // addresses -- entries outside the region are read and discarded -- so that the loads are in
   // flight together instead of one guarded load and wait per entry.  64 x 64 tiles read theirs
   // before the GEMM (its operands are other columns), 128 x 128 tiles 16 per lane after it.
   T* C = A + F.off;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  constexpr int BR = TS == 64 ? NBK : 1;  // 16-row blocks read per batch
-  T cv[BR][NBK][4];
+  const int WR = NW == 8 ? (w >> 2) * (TS / 2) : (w >> 1) * (TS / 2);
+  const int WC = NW == 8 ? (w & 3) * (TS / 4) : (w & 1) * (TS / 2);
+  constexpr int BR = TS == 64 ? NBR : 1;  // 16-row blocks read per batch
+  T cv[BR][NBC][4];
   auto read_c = [&](int bi0) {
 #pragma unroll
     for (int b = 0; b < BR; b++)
 #pragma unroll
-      for (int bj = 0; bj < NBK; bj++)
+      for (int bj = 0; bj < NBC; bj++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = min(row0 + (w >> 1) * (TS / 2) + (bi0 + b) * 16 + (lane & 15), rend - 1);
-          const int j = min(col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r), cend - 1);
+          const int i = min(row0 + WR + (bi0 + b) * 16 + (lane & 15), rend - 1);
+          const int j = min(col0 + WC + bj * 16 + mrow<T>(lane >> 4, r), cend - 1);
           cv[b][bj][r] = C[(i64)j * F.N + i];
         }
   };
   if constexpr (TS == 64) read_c(0);
   const T* base = A + F.off + (i64)kbase * F.N;
-  if constexpr (!SPLIT) {
+  if constexpr (NW == 8) {
+    constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
+    __shared__ TileSmemT<T, 128> sm;
+    tile_gemm_nt_8<T, DEPTH>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
+  } else if constexpr (!SPLIT) {
     // FP32 fronts fetch two chunks ahead (FP64 tiles would lose occupancy to the registers)
     constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
     __shared__ TileSmemT<T, TS> sm;
@@ -1347,17 +1444,17 @@ __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const Fro
     tile_gemm_nt_bf<TS>(acc, base + row0, F.N, rend - row0, base + col0, F.N, cend - col0, K, sm);
   }
 #pragma unroll
-  for (int bi0 = 0; bi0 < NBK; bi0 += BR) {
+  for (int bi0 = 0; bi0 < NBR; bi0 += BR) {
     if constexpr (TS != 64) read_c(bi0);
 #pragma unroll
     for (int b = 0; b < BR; b++)
 #pragma unroll
-      for (int bj = 0; bj < NBK; bj++)
+      for (int bj = 0; bj < NBC; bj++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int bi = bi0 + b;
-          const int i = row0 + (w >> 1) * (TS / 2) + bi * 16 + (lane & 15);
-          const int j = col0 + (w & 1) * (TS / 2) + bj * 16 + mrow<T>(lane >> 4, r);
+          const int i = row0 + WR + bi * 16 + (lane & 15);
+          const int j = col0 + WC + bj * 16 + mrow<T>(lane >> 4, r);
           if (i >= rend || j >= cend || i < j) continue;
           T* c = C + (i64)j * F.N + i;
           const T sv = -acc[bi][bj][r];
@@ -1388,17 +1485,18 @@ __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_group(const int
                                                                    int k0, int kw, int ccap, int xt, int nf) {
   gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
-template <class T, int TS = 64, bool SPLIT = false>
-__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_trail(const int* __restrict__ ids,
-                                                                   const FrontDev* __restrict__ fr, T* __restrict__ A,
-                                                                   int k0, int kw, int ccap, int xt, int nf) {
-  gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
+// NW = 8: the 128-tile on 8 waves, four waves per SIMD (two tiles per CU)
+template <class T, int TS = 64, bool SPLIT = false, int NW = 4>
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : TS == 128 ? 2 : 1) void k_gemm_trail(
+    const int* __restrict__ ids, const FrontDev* __restrict__ fr, T* __restrict__ A, int k0, int kw, int ccap, int xt,
+    int nf) {
+  gemm_body<T, 0, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
-template <class T, int TS = 64, bool SPLIT = false>
-__global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_schur(const int* __restrict__ ids,
-                                                                   const FrontDev* __restrict__ fr, T* __restrict__ A,
-                                                                   int k0, int kw, int ccap, int xt, int nf) {
-  gemm_body<T, 1, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
+template <class T, int TS = 64, bool SPLIT = false, int NW = 4>
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : TS == 128 ? 2 : 1) void k_gemm_schur(
+    const int* __restrict__ ids, const FrontDev* __restrict__ fr, T* __restrict__ A, int k0, int kw, int ccap, int xt,
+    int nf) {
+  gemm_body<T, 1, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
 
 

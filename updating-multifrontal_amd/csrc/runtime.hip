@@ -290,7 +290,8 @@ int ensure_ids(umf_plan* p, size_t n) {
 thread_local int g_level = -1;
 
 template <class K, class... Args>
-void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, Args... args) {
+void launch_b(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, dim3 block,
+              Args... args) {
   Prof& pr = p->prof;
   hipEvent_t a = nullptr, b = nullptr;
   char tagged[64];
@@ -313,8 +314,12 @@ void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, d
       (void)hipEventRecord(a, p->stream);
     }
   }
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, p->stream, args...);
+  hipLaunchKernelGGL(kern, grid, block, 0, p->stream, args...);
   if (b) (void)hipEventRecord(b, p->stream);
+}
+template <class K, class... Args>
+void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, Args... args) {
+  launch_b(p, name, flops, bytes, kern, grid, dim3(256), args...);
 }
 
 // After the stream has been synchronized: fold the pending event pairs into the totals.
@@ -356,6 +361,17 @@ auto gemm_fn(bool split) {
   }
   return pick(std::false_type{});
 }
+// the native-MFMA 128-tile trailing (STAGE 1) or Schur (STAGE 2) GEMM on 8 waves, used for FP64
+// fronts: four waves per SIMD instead of two took the 3D 127^3 FP64 trailing update from 1604 to
+// 1504 ms and the Schur update from 423 to 386 ms; FP32 fronts measured slower on 8 waves
+// (728 -> 748 ms) and keep 4
+template <class T, int STAGE>
+auto gemm_fn8() {
+  if constexpr (STAGE == 1) return k_gemm_trail<T, 128, false, 8>;
+  else return k_gemm_schur<T, 128, false, 8>;
+}
+template <class T>
+constexpr bool kGemm8 = sizeof(T) == 8;
 
 // Workgroups per item (front, separator) of a level stage whose largest item can use `want`
 // of them: enough in all to fill the chip (about 32k), no more.  Every stage loops over its
@@ -680,7 +696,10 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : band_grid(tin, act),
              ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : -1, act);
     auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
-      if (wide(t128, act))
+      if (wide(t128, act) && kGemm8<T>)
+        launch_b(p, "k_gemm_trail", fl, by, gemm_fn8<T, 1>(), band_grid(t128, act), dim3(512), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act);
+      else if (wide(t128, act))
         launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), band_grid(t128, act), ids + L.f_off,
                (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act);
       else
@@ -720,7 +739,10 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       const bool xc = c >= 64;
       auto grid = [&](int t) { return xc ? dim3(8 * ((c + 7) / 8) * t) : band_grid(t, c); };
       if (tiles > 0) {
-        if (wide(tiles128, c))
+        if (wide(tiles128, c) && kGemm8<T>)
+          launch_b(p, "k_gemm_schur", fl, by, gemm_fn8<T, 2>(), grid(tiles128), dim3(512), ids + L.f_off + i,
+                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c);
+        else if (wide(tiles128, c))
           launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
                  (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c);
         else
