@@ -1,11 +1,24 @@
 """GPU: plan files (umf_plan_save / umf_plan_load).  A restored plan holds the same factors, so
 Solve / Apply / factor blocks are bit-identical to the saved plan's, after Factor, after an
 Update (the file carries the current operator values), in the FP32 form, and for a plan saved
-before factoring (restored unfactored, then factored)."""
+before factoring (restored unfactored, then factored).  The restored plan's solves are also
+checked against the oracle's factorization of the same operator (relative 2-norm <= 1e-10, the
+FP64 form's parity bound), so the round trip is pinned to the reference's arithmetic, not only
+to the plan it came from."""
 import numpy as np
 import pytest
 
-from oracle_py import coeff_field, laplacian, moddofs
+from oracle_py import Factorization, coeff_field, laplacian, moddofs
+
+
+def _oracle_solve(A, dims, maxs, B):
+    O = Factorization(A, dims, 1, maxs, "orc")
+    O.factor()
+    return np.stack([O.solve(B[:, j]) for j in range(B.shape[1])], axis=1)
+
+
+def _rel(x, y):
+    return np.linalg.norm(x - y) / np.linalg.norm(y)
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +42,7 @@ def test_saved_plan_solves_identically(tmp_path, dims, maxs):
     H = umf.load_plan(p)
     assert H.dims == G.dims and H.n == G.n
     assert np.array_equal(H.Solve(B), G.Solve(B))
+    assert _rel(H.Solve(B), _oracle_solve(A, dims, maxs, B)) <= 1e-10
     assert np.array_equal(H.Apply(B), G.Apply(B))
     for blk in ("A22", "XR", "S"):
         assert np.array_equal(H.front_block(1, 0, blk), G.front_block(1, 0, blk)), blk
@@ -41,6 +55,7 @@ def test_saved_plan_solves_identically(tmp_path, dims, maxs):
     G.save(p)
     H2 = umf.load_plan(p)
     assert np.array_equal(H2.Solve(B), G.Solve(B))
+    assert _rel(H2.Solve(B), _oracle_solve(A2, dims, maxs, B)) <= 1e-10
     # and the restored plan updates like the original
     A3 = laplacian(dims, co)
     G.Update(A3, mod)
