@@ -854,19 +854,19 @@ __device__ __forceinline__ void diag_core(DiagSmem<T>& sm, T* __restrict__ blk, 
 }
 
 template <class T>
-// left = 1 (left-looking levels): the block first takes the update by every finished column,
+union DiagBlockSmem {
+  TileSmemT<T, 64> g;  // the update's stages, before the block is loaded
+  DiagSmem<T> d;
+};
+
+// The 64 x 64 diagonal block of block column jb of front F (fid in error reports): left = 1
+// (left-looking levels) first takes the update by every finished column,
 // D = A(jb.., jb..) - L(jb.., 0:jb) L(jb.., 0:jb)^T, on MFMA (the panel steps leave the later
-// diagonal blocks untouched).
-__global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                   T* __restrict__ A, int jb, int left, int* __restrict__ err) {
-  const int fid = ids[blockIdx.x];
-  const FrontDev F = fr[fid];
-  if (jb >= F.n) return;
+// diagonal blocks untouched); then L_jj and Linv_jj (diag_core).
+template <class T>
+__device__ __forceinline__ void diag_block(const FrontDev& F, T* __restrict__ A, int jb, int left,
+                                           int* __restrict__ err, int fid, DiagBlockSmem<T>& sm) {
   const int kb = min(NB, F.n - jb);
-  __shared__ union {
-    TileSmemT<T, 64> g;  // the update's stages, before the block is loaded
-    DiagSmem<T> d;
-  } sm;
   T* blk = A + F.off + (i64)jb * F.N + jb;
   V4<T> acc[2][2];
 #pragma unroll
@@ -892,6 +892,16 @@ __global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ id
         sm.d.Ts[j * LD + i] = v;
       }
   diag_core(sm.d, blk, F.N, kb, A + F.dinv_off + (i64)(jb / NB) * NB * NB, err, fid, jb);
+}
+
+template <class T>
+__global__ __launch_bounds__(256, 4) void k_chol_diag(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                   T* __restrict__ A, int jb, int left, int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  if (jb >= F.n) return;
+  __shared__ DiagBlockSmem<T> sm;
+  diag_block<T>(F, A, jb, left, err, fid, sm);
 }
 
 // Persistent launches over a queue of items (fronts of one level, largest first): a grid of as
@@ -1183,30 +1193,75 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
       }
 }
 
-// The left-looking GEMM runs on the native MFMA of T in every precision form: BF16 pieces
-// (tile_gemm_nt_bf) measured slower here, where the GEMMs are short and the 40 KB of split
-// stages halve the workgroups per CU.
+// Y = X Linv^T for one 64 x 64 tile on 4 waves as 2 x 2 (acc_row / acc_col layout): X in LDS
+// (X(i, k) at X[k * kXs + i], zero outside the tile), Linv(j, k) at dinv[k * NB + j] (zero above
+// the diagonal).  Linv(j, k) = 0 for k > j and X(i, k) = 0 for k >= kb: block column bj of a
+// wave needs k < min(32 wc + 16 bj + 16, kb).  Partial tiles (kb or nrow < 64): a product outside
+// the block is multiplied by zero instead of skipped by a branch, so the inverse's loads stay
+// unconditional and are issued ahead together (X is zero there and the inverse finite, and adding
+// +-0 to an accumulator that started at +0 changes no bit).
 template <class T>
-__global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                    T* __restrict__ A, int jb, int left, int xt, int nf, int fly,
-                                                    const int* __restrict__ fly_tab,
-                                                    const Triple* __restrict__ fly_tri,
-                                                    const double* __restrict__ csr_val) {
-  int tile, fi;
-  if (!wg_tile(xt, nf, tile, fi)) return;
-  const FrontDev F = fr[ids[fi]];
-  if (jb >= F.n) return;
+__device__ __forceinline__ void trsm64(V4<T> (&acc)[2][2], const T* __restrict__ X, const T* __restrict__ dinv,
+                                       int kb, int nrow) {
+  const int lane = threadIdx.x & 63, c16 = lane & 15, q = lane >> 4;
+  const int w = wave_id(), wr = w >> 1, wc = w & 1;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
+  auto trsm = [&](auto half, auto guard) {
+    constexpr int WC = decltype(half)::value;
+    const int k4end = (kb + 3) / 4;
+#pragma unroll
+    for (int k4 = 0; k4 < 8 * WC + 8; k4++) {
+      const int k = k4 * 4 + q;
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        const T bv = X[k * kXs<T> + wr * 32 + bi * 16 + c16];
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) {
+          if (k4 >= 8 * WC + 4 * bj + 4) continue;
+          const T av = dinv[k * NB + WC * 32 + bj * 16 + c16];
+          const bool on = !decltype(guard)::value || (k4 < k4end && wr * 32 + bi * 16 < nrow);
+          acc[bi][bj] = mfma4(av, bv * (on ? T(1) : T(0)), acc[bi][bj]);
+        }
+      }
+    }
+  };
+  const bool full = kb == NB && nrow == NB;
+  if (wc == 0) {
+    if (full)
+      trsm(std::integral_constant<int, 0>{}, std::false_type{});
+    else
+      trsm(std::integral_constant<int, 0>{}, std::true_type{});
+  } else {
+    if (full)
+      trsm(std::integral_constant<int, 1>{}, std::false_type{});
+    else
+      trsm(std::integral_constant<int, 1>{}, std::true_type{});
+  }
+}
+
+template <class T>
+union PanelSmem64 {
+  TileSmemT<T, 64> g;
+  struct {
+    T X[NB * kXs<T>];
+  } c;
+};
+
+// One panel row tile of block column jb (rows row0 = jb + kb + 64 tile .. + 64 of front F):
+// X = A(rows, jb block) - L(rows, 0:K) L(jb block, 0:K)^T (K = jb left-looking, 0
+// right-looking), CSR entries added for leaf fronts taking them in the panel step (fly), then
+// L(rows, jb block) = X Linv_jj^T.  Returns without work for a tile past the front.
+template <class T>
+__device__ __forceinline__ void panel_tile(const FrontDev& F, T* __restrict__ A, int jb, int left, int tile, int fly,
+                                           const int* __restrict__ fly_tab, const Triple* __restrict__ fly_tri,
+                                           const double* __restrict__ csr_val, PanelSmem64<T>& sm) {
   const int kb = min(NB, F.n - jb);
   const int s0 = jb + kb;  // first row below the block
   if (s0 + NB * tile >= F.N) return;
-  __shared__ union {
-    TileSmemT<T, 64> g;
-    struct {
-      T X[NB * kXs<T>];
-    } c;
-  } sm;
-  const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
-  const int w = wave_id(), wr = w >> 1, wc = w & 1;
+  const int t = threadIdx.x;
   V4<T> acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -1241,47 +1296,7 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
     }
     __syncthreads();
   }
-  // Y = X Linv^T: Y(i,j) = sum_k X(i,k) Linv(j,k), where Linv(j, k) = 0 for k > j and X(i, k) = 0
-  // for k >= kb: block column bj of the wave needs k < min(32 wc + 16 bj + 16, kb)
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = vzero<T>();
-  auto trsm = [&](auto half, auto guard) {
-    constexpr int WC = decltype(half)::value;
-    const int k4end = (kb + 3) / 4;
-#pragma unroll
-    for (int k4 = 0; k4 < 8 * WC + 8; k4++) {
-      const int k = k4 * 4 + q;
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) {
-        const T bv = sm.c.X[k * kXs<T> + wr * 32 + bi * 16 + c16];
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
-          if (k4 >= 8 * WC + 4 * bj + 4) continue;
-          // partial tiles: a product outside the block is multiplied by zero instead of skipped
-          // by a branch, so the inverse's loads stay unconditional and are issued ahead together
-          // (X is zero there and the inverse finite, and adding +-0 to an accumulator that
-          // started at +0 changes no bit)
-          const T av = dinv[k * NB + WC * 32 + bj * 16 + c16];
-          const bool on = !decltype(guard)::value || (k4 < k4end && wr * 32 + bi * 16 < nrow);
-          acc[bi][bj] = mfma4(av, bv * (on ? T(1) : T(0)), acc[bi][bj]);
-        }
-      }
-    }
-  };
-  const bool full = kb == NB && nrow == NB;
-  if (wc == 0) {
-    if (full)
-      trsm(std::integral_constant<int, 0>{}, std::false_type{});
-    else
-      trsm(std::integral_constant<int, 0>{}, std::true_type{});
-  } else {
-    if (full)
-      trsm(std::integral_constant<int, 1>{}, std::false_type{});
-    else
-      trsm(std::integral_constant<int, 1>{}, std::true_type{});
-  }
+  trsm64<T>(acc, sm.c.X, dinv, kb, nrow);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -1290,6 +1305,367 @@ __global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ i
       for (int r = 0; r < 4; r++) {
         const int i = acc_row(bi), j = acc_col<T>(bj, r);
         if (i < nrow && j < kb) pan[(i64)j * F.N + i] = acc[bi][bj][r];
+      }
+}
+
+// The left-looking GEMM runs on the native MFMA of T in every precision form: BF16 pieces
+// (tile_gemm_nt_bf) measured slower here, where the GEMMs are short and the 40 KB of split
+// stages halve the workgroups per CU.
+template <class T>
+__global__ __launch_bounds__(256, 4) void k_chol_panel(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
+                                                    T* __restrict__ A, int jb, int left, int xt, int nf, int fly,
+                                                    const int* __restrict__ fly_tab,
+                                                    const Triple* __restrict__ fly_tri,
+                                                    const double* __restrict__ csr_val) {
+  int tile, fi;
+  if (!wg_tile(xt, nf, tile, fi)) return;
+  const FrontDev F = fr[ids[fi]];
+  if (jb >= F.n) return;
+  __shared__ PanelSmem64<T> sm;
+  panel_tile<T>(F, A, jb, left, tile, fly, fly_tab, fly_tri, csr_val, sm);
+}
+
+// ---- paired left-looking steps: block columns jb and jb + 64 (both full) in three launches,
+// k_chol_diag (jb), k_chol_diag2 (jb), k_chol_panel2 (jb), instead of four.  The panel below both
+// blocks reads its rows' finished columns L(rows, 0:jb) once for 128 columns instead of once per
+// 64 (the left-looking panel's dominant traffic).  Every entry gets the same operations in the
+// same order as in two single steps, so the factors are bit-identical.
+
+// Stages of the 64 x 128 panel GEMM (P: 64 rows, Q: 128 rows), padded as TileSmemT.
+template <class T>
+struct TileSmemR {
+  T As[2][16][64 + kStagePad<T>];
+  T Bs[2][16][128 + kStagePad<T>];
+};
+
+// acc += P Q^T for a 64-row x 128-column tile, k in [0, K): the 4 waves as 2 x 2, wave (wr, wc)
+// holds rows 32 wr .. + 32 and columns 64 wc .. + 64 (acc[bi][bj]: 16-row block bi, 16-column
+// block bj).  Stages, operand loads and fragment prefetch as tile_gemm_nt_t; per 4-deep step a
+// wave reads 2 P and 4 Q fragments for 8 MFMAs.
+template <class T>
+__device__ __forceinline__ void tile_gemm_nt_r(V4<T> (&acc)[2][4], const T* __restrict__ P, int ldp, int pn,
+                                               const T* __restrict__ Q, int ldq, int qn, int K, TileSmemR<T>& sm) {
+  if (K <= 0) return;
+  const int lane = threadIdx.x & 63, w = wave_id(), wr = w >> 1, wc = w & 1;
+  T pa[4], qb[4][2];
+  int po[4], qo[4][2];  // byte offsets of this lane's elements within a chunk: k = w + 4q
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    po[q] = ((w + 4 * q) * ldp + min(lane, pn - 1)) * (int)sizeof(T);
+#pragma unroll
+    for (int u = 0; u < 2; u++) qo[q][u] = ((w + 4 * q) * ldq + min(lane + 64 * u, qn - 1)) * (int)sizeof(T);
+  }
+  auto fetch = [&](int kc) {
+    const int nk = max(min(16, K - kc), 0);
+    const __amdgpu_buffer_rsrc_t rp = chunk_rsrc(P + (i64)kc * ldp, nk * ldp * (int)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rq = chunk_rsrc(Q + (i64)kc * ldq, nk * ldq * (int)sizeof(T));
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      pa[q] = buf_load<T>(rp, po[q]);
+#pragma unroll
+      for (int u = 0; u < 2; u++) qb[q][u] = buf_load<T>(rq, qo[q][u]);
+    }
+  };
+  auto stage = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sm.As[b][w + 4 * q][lane] = pa[q];
+#pragma unroll
+      for (int u = 0; u < 2; u++) sm.Bs[b][w + 4 * q][lane + 64 * u] = qb[q][u];
+    }
+  };
+  auto step = [&](int b) {
+    T av[2][4], bv[2][2];
+    auto rd = [&](int k4, int z) {
+      const int kr = k4 * 4 + (lane >> 4);
+#pragma unroll
+      for (int x = 0; x < 2; x++) bv[z][x] = sm.As[b][kr][wr * 32 + x * 16 + (lane & 15)];
+#pragma unroll
+      for (int y = 0; y < 4; y++) av[z][y] = sm.Bs[b][kr][wc * 64 + y * 16 + (lane & 15)];
+    };
+    rd(0, 0);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+      if (k4 < 3) rd(k4 + 1, (k4 + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);  // the next fragments' reads stay ahead of these MFMAs
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = mfma4(av[k4 & 1][bj], bv[k4 & 1][bi], acc[bi][bj]);
+    }
+  };
+  __syncthreads();  // the stages may still be read by an earlier use of the shared memory
+  fetch(0);
+  stage(0);
+  fetch(16);
+  __syncthreads();
+  int b = 0;
+  for (int kc = 0; kc < K; kc += 16, b ^= 1) {
+    step(b);
+    stage(b ^ 1);  // past the last chunk: a harmless write of zeros to the unused stage
+    fetch(kc + 32);
+    __syncthreads();
+  }
+}
+
+// k_chol_diag2: one workgroup per front: the first panel tile of block jb (rows jb2 = jb + 64 ..
+// jb + 128: T0 = L(jb2 block, jb block)) and the diagonal block of jb2, from one 64 x 128 GEMM
+// over the finished columns [0, jb) (P = L(jb2 block, 0:jb), Q = L(jb:jb + 128, 0:jb): the T0
+// tile's update and the jb2 block's update, both rows read once); T0 = X Linv_1^T; the jb2
+// block's update continues over k in [jb, jb2) with T0 as both operands; then its L and Linv
+// (diag_core).  The same operations in the same order as k_chol_panel's tile 0 and k_chol_diag.
+template <class T>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_chol_diag2(
+    const int* __restrict__ ids, const FrontDev* __restrict__ fr, T* __restrict__ A, int jb, int fly,
+    const int* __restrict__ fly_tab, const Triple* __restrict__ fly_tri, const double* __restrict__ csr_val,
+    int* __restrict__ err) {
+  const int fid = ids[blockIdx.x];
+  const FrontDev F = fr[fid];
+  if (jb + 2 * NB > F.n) return;
+  const int jb2 = jb + NB;
+  __shared__ union {
+    TileSmemR<T> g;
+    struct {
+      T X[NB * kXs<T>];
+    } c;
+    DiagSmem<T> d;
+  } sm;
+  const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
+  const int w = wave_id(), wr = w >> 1, wc = w & 1;
+  V4<T> acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = vzero<T>();
+  const bool from_csr = fly && F.fly_off >= 0;
+  T* pan = A + F.off + (i64)jb * F.N + jb2;  // the T0 tile; the jb2 block at pan + 64 F.N
+  tile_gemm_nt_r<T>(acc, A + F.off + jb2, F.N, NB, A + F.off + jb, F.N, 2 * NB, jb, sm.g);
+  // T0: X = A(jb2 block, jb block) - acc (waves wc = 0; the stages are free after the GEMM)
+  if (wc == 0) {
+    T pv[2][4][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          pv[bi][bj][r] = from_csr ? T(0) : pan[(i64)j * F.N + i];
+        }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          sm.c.X[j * kXs<T> + i] = pv[bi][bj][r] - acc[bi][bj][r];
+        }
+  }
+  __syncthreads();
+  if (from_csr) {  // CSR entries of the 64-row tile 0 below jb's block
+    const int* tb = fly_tab + F.fly_off + (jb / NB) * ((F.N + NB - 1) / NB + 1);
+    for (int e = tb[0] + t; e < tb[1]; e += 256) {
+      const Triple tr = fly_tri[e];
+      sm.c.X[(tr.col - jb) * kXs<T> + (tr.row - jb2)] += T(csr_val[tr.k]);
+    }
+    __syncthreads();
+  }
+  V4<T> y[2][2];
+  trsm64<T>(y, sm.c.X, A + F.dinv_off + (i64)(jb / NB) * NB * NB, NB, NB);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        pan[(i64)j * F.N + i] = y[bi][bj][r];
+      }
+  __syncthreads();  // every wave is done reading X
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        sm.c.X[j * kXs<T> + i] = y[bi][bj][r];  // T0 in the X layout
+      }
+  __syncthreads();
+  // the jb2 block (waves wc = 1, rows 32 wr .. + 32, all 64 columns): its update continues over k
+  // in [jb, jb2) with P(i, k) = T0(i, k), Q(j, k) = T0(j, k); the block's current values read
+  T dv[2][4][4];
+  if (wc == 1) {
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = k4 * 4 + q;
+      T av[4], bv[2];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) av[bj] = sm.c.X[k * kXs<T> + bj * 16 + c16];
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) bv[bi] = sm.c.X[k * kXs<T> + wr * 32 + bi * 16 + c16];
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          dv[bi][bj][r] = pan[(i64)(NB + j) * F.N + i];  // A(jb2 + i, jb2 + j)
+        }
+  }
+  __syncthreads();  // every wave is done reading T0
+  if (wc == 1) {  // the lower triangle of D into the diagonal factorization's LDS block
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          sm.d.Ts[j * LD + i] = i >= j ? dv[bi][bj][r] - acc[bi][bj][r] : T(0);
+        }
+  }
+  diag_core(sm.d, pan + (i64)NB * F.N, F.N, NB, A + F.dinv_off + (i64)(jb2 / NB) * NB * NB, err, fid, jb2);
+}
+
+// k_chol_panel2: the panel rows below both blocks (row0 = jb + 128 + 64 tile), one 64-row tile per
+// workgroup.  X = A(rows, jb:jb + 128) - L(rows, 0:jb) L(jb:jb + 128, 0:jb)^T (one GEMM for both
+// blocks); block jb: Y1 = X1 Linv_1^T; block jb2: the accumulation continues over k in [jb, jb2)
+// with Y1 and T0 = L(jb2 block, jb block) as operands, X2 = A2 - acc, Y2 = X2 Linv_2^T.
+template <class T>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_chol_panel2(const int* __restrict__ ids,
+                                                                            const FrontDev* __restrict__ fr,
+                                                     T* __restrict__ A, int jb, int xt, int nf, int fly,
+                                                     const int* __restrict__ fly_tab,
+                                                     const Triple* __restrict__ fly_tri,
+                                                     const double* __restrict__ csr_val) {
+  int tile, fi;
+  if (!wg_tile(xt, nf, tile, fi)) return;
+  const FrontDev F = fr[ids[fi]];
+  if (jb + 2 * NB > F.n) return;
+  const int jb2 = jb + NB, s0 = jb + 2 * NB;
+  if (s0 + NB * tile >= F.N) return;
+  __shared__ union {
+    TileSmemR<T> g;
+    struct {
+      T X[NB * kXs<T>];
+    } c;
+  } sm;
+  const int t = threadIdx.x, lane = t & 63, c16 = lane & 15, q = lane >> 4;
+  const int w = wave_id(), wr = w >> 1, wc = w & 1;
+  V4<T> acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = vzero<T>();
+  const int row0 = __builtin_amdgcn_readfirstlane(s0 + NB * tile);
+  const int nrow = min(NB, F.N - row0);
+  const bool from_csr = fly && F.fly_off >= 0;
+  T* pan = A + F.off + (i64)jb * F.N + row0;  // X1 / Y1; X2 / Y2 at pan + 64 F.N
+  tile_gemm_nt_r<T>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, 2 * NB, jb, sm.g);
+  // X = (the wave's 32 rows x 64 columns of the tile's current values, block wc, read at clamped
+  // addresses) - acc, into the X block; read only when needed (registers)
+  auto x_out = [&]() {
+    T pv[2][4][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          const T x = from_csr ? T(0) : pan[(i64)(wc * NB + j) * F.N + min(i, nrow - 1)];
+          pv[bi][bj][r] = i < nrow ? x : T(0);
+        }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr * 32 + bi * 16 + c16, j = bj * 16 + mrow<T>(q, r);
+          sm.c.X[j * kXs<T> + i] = i < nrow ? pv[bi][bj][r] - acc[bi][bj][r] : T(0);
+        }
+  };
+  const int NT = (F.N + NB - 1) / NB;
+  auto add_csr = [&](int cb, int t64) {  // CSR entries of 64-row tile t64 of block column cb
+    const int* tb = fly_tab + F.fly_off + cb * (NT + 1);
+    for (int e = tb[t64] + t; e < tb[t64 + 1]; e += 256) {
+      const Triple tr = fly_tri[e];
+      sm.c.X[(tr.col - cb * NB) * kXs<T> + (tr.row - row0)] += T(csr_val[tr.k]);
+    }
+  };
+  // block jb: X1 (waves wc = 0), its CSR entries (64-row tile `tile + 1` below jb's block), Y1
+  // (the GEMM ends on a barrier: the stages are free)
+  if (wc == 0) x_out();
+  __syncthreads();
+  if (from_csr) {
+    add_csr(jb / NB, tile + 1);
+    __syncthreads();
+  }
+  V4<T> y[2][2];
+  trsm64<T>(y, sm.c.X, A + F.dinv_off + (i64)(jb / NB) * NB * NB, NB, nrow);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        if (i < nrow) pan[(i64)j * F.N + i] = y[bi][bj][r];
+      }
+  __syncthreads();  // every wave is done reading X1
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        sm.c.X[j * kXs<T> + i] = i < nrow ? y[bi][bj][r] : T(0);  // Y1 in the X layout
+      }
+  __syncthreads();
+  // block jb2 (waves wc = 1): the accumulation continues over k in [jb, jb2): P(i, k) = Y1(i, k),
+  // Q(j, k) = T0(j, k) = L(jb2 + j, jb + k)
+  if (wc == 1) {
+    const T* T0 = A + F.off + (i64)jb * F.N + jb2;
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; k4++) {
+      const int k = k4 * 4 + q;
+      T av[4], bv[2];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) av[bj] = T0[(i64)k * F.N + bj * 16 + c16];
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) bv[bi] = sm.c.X[k * kXs<T> + wr * 32 + bi * 16 + c16];
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) acc[bi][bj] = mfma4(av[bj], bv[bi], acc[bi][bj]);
+    }
+  }
+  __syncthreads();  // every wave is done reading Y1
+  if (wc == 1) x_out();
+  __syncthreads();
+  if (from_csr) {
+    add_csr(jb2 / NB, tile);
+    __syncthreads();
+  }
+  trsm64<T>(y, sm.c.X, A + F.dinv_off + (i64)(jb2 / NB) * NB * NB, NB, nrow);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = acc_row(bi), j = acc_col<T>(bj, r);
+        if (i < nrow) pan[(i64)(NB + j) * F.N + i] = y[bi][bj][r];
       }
 }
 
