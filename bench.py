@@ -95,6 +95,7 @@ WORKLOADS = {
 # bounded samples of the same generators for the reference's CPU run (about 10-30 s of CPU work)
 CPU_SAMPLES = {
     "3d47": dict(dims=(47, 47, 47), max_side=8, box=8, scale=None, seed=3),
+    "3d79": dict(dims=(79, 79, 79), max_side=8, box=8, scale=None, seed=3),
     "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3),
     "2d511": dict(dims=(511, 511), max_side=16, box=32, scale=100.0, seed=2),
     "2d127": WORKLOADS["2d127"],
@@ -247,11 +248,14 @@ def cpu_sample(name, nrhs):
     t0 = time.perf_counter()
     F.factor(trace=False)
     t1 = time.perf_counter()
+    progress(f"reference CPU {name}: Factor {t1 - t0:.1f} s")
     F.update(oracle_py.Csr(rp, ci, v2), mod, trace=False)
     t2 = time.perf_counter()
+    progress(f"reference CPU {name}: Update {t2 - t1:.1f} s")
     for r in range(nrhs):
         F.solve(B[r])
     t3 = time.perf_counter()
+    progress(f"reference CPU {name}: {nrhs} Solve calls {t3 - t2:.1f} s")
     tot = t3 - t0
     return {"value": (f_factor + f_update + f_solve) / tot / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
             "sample": describe(spec, nrhs, P.levels, P.leaf_width(0)) + "; the reference's LU form, MKL sequential",
@@ -271,12 +275,13 @@ def cpu_sample_threaded(name, nrhs, threads=None):
     t = threads or cpu_threads()
     env = dict(os.environ, UMF_REF_SO=lib, MKL_NUM_THREADS=str(t), OMP_NUM_THREADS=str(t))
     try:
+        # the child's progress lines (stderr) pass through; its result line is on stdout
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", name, "--nrhs", str(nrhs)],
-                           env=env, capture_output=True, text=True, timeout=1800)
+                           env=env, stdout=subprocess.PIPE, stderr=None, text=True, timeout=1800)
     except subprocess.TimeoutExpired:
         return None
     if r.returncode != 0 or not r.stdout.strip():
-        progress(f"threaded CPU baseline failed: {r.stderr.strip()[-300:]}")
+        progress(f"threaded CPU baseline failed (exit {r.returncode})")
         return None
     out = json.loads(r.stdout.strip().splitlines()[-1])
     out["cores"] = t
