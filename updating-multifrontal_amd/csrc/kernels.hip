@@ -1741,11 +1741,9 @@ __device__ __forceinline__ bool tile_of_band(int x, int s, int rend, int cend, i
 // SPLIT = false: the native MFMA of T; true (T = float): the product on BF16 pieces
 // (tile_gemm_nt_bf).
 // NW = 8 (TS = 128, native MFMA): the tile on 8 waves (tile_gemm_nt_8).
-// sfix >= 0 (MODE 0): the region starts at row / column sfix instead of k0 + K (the part of a
-// trailing update beyond the next column group, run beside that group's factorization).
 template <class T, int MODE, int TS, bool SPLIT, int NW = 4>
 __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                          T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf, int sfix) {
+                                          T* __restrict__ A, int k0, int kw, int ccap, int xt, int nf) {
   static_assert(!SPLIT || std::is_same<T, float>::value, "the BF16 split takes FP32 fronts");
   static_assert(NW == 4 || (NW == 8 && TS == 128 && !SPLIT), "8 waves: 128-tiles on the native MFMA");
   int tile, fi;
@@ -1755,7 +1753,7 @@ __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const Fro
   if (MODE == 0) {  // columns [k0, k0 + kw) applied to rows [s, N) x columns [s, min(n, ccap))
     if (k0 >= F.n) return;
     K = min(kw, F.n - k0);
-    s = sfix >= 0 ? sfix : k0 + K;
+    s = k0 + K;
     cend = min(F.n, ccap);
     if (s >= cend) return;
     kbase = k0; rend = F.N;
@@ -1860,22 +1858,21 @@ __device__ __forceinline__ void gemm_body(const int* __restrict__ ids, const Fro
 template <class T, int TS = 64, bool SPLIT = false>
 __global__ __launch_bounds__(256, TS == 128 ? 2 : 1) void k_gemm_group(const int* __restrict__ ids,
                                                                    const FrontDev* __restrict__ fr, T* __restrict__ A,
-                                                                   int k0, int kw, int ccap, int xt, int nf,
-                                                                   int sfix) {
-  gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf, sfix);
+                                                                   int k0, int kw, int ccap, int xt, int nf) {
+  gemm_body<T, 0, TS, SPLIT>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
 // NW = 8: the 128-tile on 8 waves, four waves per SIMD (two tiles per CU)
 template <class T, int TS = 64, bool SPLIT = false, int NW = 4>
 __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : TS == 128 ? 2 : 1) void k_gemm_trail(
     const int* __restrict__ ids, const FrontDev* __restrict__ fr, T* __restrict__ A, int k0, int kw, int ccap, int xt,
-    int nf, int sfix) {
-  gemm_body<T, 0, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf, sfix);
+    int nf) {
+  gemm_body<T, 0, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
 template <class T, int TS = 64, bool SPLIT = false, int NW = 4>
 __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : TS == 128 ? 2 : 1) void k_gemm_schur(
     const int* __restrict__ ids, const FrontDev* __restrict__ fr, T* __restrict__ A, int k0, int kw, int ccap, int xt,
-    int nf, int sfix) {
-  gemm_body<T, 1, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf, sfix);
+    int nf) {
+  gemm_body<T, 1, TS, SPLIT, NW>(ids, fr, A, k0, kw, ccap, xt, nf);
 }
 
 

@@ -116,9 +116,6 @@ struct umf_plan {
   Prof prof;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // the side stream of the look-ahead trailing updates (factor_level) and its fork / join events
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* arena = nullptr;
   int* d_dofs = nullptr;
   MatDev* d_mats = nullptr;
@@ -293,8 +290,8 @@ int ensure_ids(umf_plan* p, size_t n) {
 thread_local int g_level = -1;
 
 template <class K, class... Args>
-void launch_s(umf_plan* p, hipStream_t st, const char* name, double flops, double bytes, K kern, dim3 grid,
-              dim3 block, Args... args) {
+void launch_b(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, dim3 block,
+              Args... args) {
   Prof& pr = p->prof;
   hipEvent_t a = nullptr, b = nullptr;
   char tagged[64];
@@ -314,20 +311,15 @@ void launch_s(umf_plan* p, hipStream_t st, const char* name, double flops, doubl
       b = pr.pool[pr.used + 1];
       pr.used += 2;
       pr.pend.push_back(Prof::Pending{pr.find(name), flops, bytes});
-      (void)hipEventRecord(a, st);
+      (void)hipEventRecord(a, p->stream);
     }
   }
-  hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
-  if (b) (void)hipEventRecord(b, st);
-}
-template <class K, class... Args>
-void launch_b(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, dim3 block,
-              Args... args) {
-  launch_s(p, p->stream, name, flops, bytes, kern, grid, block, args...);
+  hipLaunchKernelGGL(kern, grid, block, 0, p->stream, args...);
+  if (b) (void)hipEventRecord(b, p->stream);
 }
 template <class K, class... Args>
 void launch(umf_plan* p, const char* name, double flops, double bytes, K kern, dim3 grid, Args... args) {
-  launch_s(p, p->stream, name, flops, bytes, kern, grid, dim3(256), args...);
+  launch_b(p, name, flops, bytes, kern, grid, dim3(256), args...);
 }
 
 // After the stream has been synchronized: fold the pending event pairs into the totals.
@@ -592,12 +584,6 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
   // columns and updates the group's later diagonal blocks -- instead of a group GEMM launch
   // after every step (one launch fewer on the step chain)
   bool lazy = false;
-  // Look-ahead on FP32 fronts: each outer group's update of the columns beyond the next outer
-  // group runs on the side stream beside that group's factorization (3D 127^3 FP32 step
-  // 1250 -> 1233 ms, 95^3 236.9 -> 233.2 ms).  FP64 fronts do not take it: their 8-wave GEMM
-  // tiles fill the CUs' LDS, so the next group's steps wait for CUs (FP64 step 2362 -> 2372 ms).
-  const bool ahead = sizeof(T) == 4;
-  bool side_pending = false;  // a look-ahead trailing update is queued on the side stream
   // FP32 left-looking levels take block columns in pairs (k_chol_diag, k_chol_diag2,
   // k_chol_panel2) where every front of the plan's level reaching k0 has two full blocks left:
   // the panel below both blocks then reads its rows' finished columns once per 128 columns, and
@@ -688,8 +674,8 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
     const int h0 = k0 / GW2 * GW2, hend = h0 + GW2;
     const bool group_done = k0 + NB >= gend || k0 + NB >= L.maxn;
     const bool outer_done = k0 + NB >= hend || k0 + NB >= L.maxn;
-    int tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0, tsb = 0, tsb128 = 0;
-    double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0, fsb = 0, bsb = 0;
+    int tin = 0, ttr = 0, ttr128 = 0, tou = 0, tou128 = 0;
+    double fdiag = 0, fpan = 0, bpan = 0, fin = 0, bin = 0, ftr = 0, btr = 0, fou = 0, bou = 0;
     seg_t.assign(act, 0);
     seg_f.assign(act, 0.0);
     seg_b.assign(act, 0.0);
@@ -727,24 +713,14 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
           btr += 2 * E * e + E * kg * (N - sg + ce - sg);
         }
       }
-      if (outer_done) {  // the outer group's trailing update; with the look-ahead, the next outer
-        // group's columns [sg, sg + GW2) on the plan stream, the columns beyond them (sb) on the
-        // side stream
+      if (outer_done) {  // the outer group's trailing update
         const int kg = std::min(GW2, F.n - h0), sg = h0 + kg;
-        const int ca = ahead ? std::min(F.n, sg + GW2) : F.n, sb = ahead ? sg + GW2 : F.n;
         if (sg < F.n) {
-          tou = std::max(tou, tiles_of(sg, N, ca));
-          tou128 = std::max(tou128, tiles_of(sg, N, ca, 128));
-          const double e = lower_elems(sg, N, ca);
+          tou = std::max(tou, tiles_of(sg, N, F.n));
+          tou128 = std::max(tou128, tiles_of(sg, N, F.n, 128));
+          const double e = lower_elems(sg, N, F.n);
           fou += 2.0 * kg * e;
-          bou += 2 * E * e + E * kg * (N - sg + ca - sg);
-        }
-        if (sb < F.n) {
-          tsb = std::max(tsb, tiles_of(sb, N, F.n));
-          tsb128 = std::max(tsb128, tiles_of(sb, N, F.n, 128));
-          const double e = lower_elems(sb, N, F.n);
-          fsb += 2.0 * kg * e;
-          bsb += 2 * E * e + E * kg * (N - sb + F.n - sb);
+          bou += 2 * E * e + E * kg * (N - sg + F.n - sg);
         }
       }
     }
@@ -778,42 +754,21 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
     }
     if (tin > 0)
       launch(p, "k_gemm_group", fin, bin, gemm_fn<T, 0, 64>(sp), xcd ? dim3(xcd_grid(tin)) : band_grid(tin, act),
-             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : -1, act, -1);
-    auto trail = [&](hipStream_t st, int t64, int t128, double fl, double by, int kstart, int kw, int ccap,
-                     int sfix) {
+             ids + L.f_off, (const FrontDev*)p->d_fronts, arena, k0, NB, gend, xcd ? tin : -1, act);
+    auto trail = [&](int t64, int t128, double fl, double by, int kstart, int kw, int ccap) {
       if (wide(t128, act) && kGemm8<T>)
-        launch_s(p, st, "k_gemm_trail", fl, by, gemm_fn8<T, 1>(), band_grid(t128, act), dim3(512), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act, sfix);
+        launch_b(p, "k_gemm_trail", fl, by, gemm_fn8<T, 1>(), band_grid(t128, act), dim3(512), ids + L.f_off,
+                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act);
       else if (wide(t128, act))
-        launch_s(p, st, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), band_grid(t128, act), dim3(256),
-                 ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act, sfix);
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 128>(sp), band_grid(t128, act), ids + L.f_off,
+               (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, -1, act);
       else
-        launch_s(p, st, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp),
-                 xcd ? dim3(xcd_grid(t64)) : band_grid(t64, act), dim3(256), ids + L.f_off,
-                 (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : -1, act, sfix);
+        launch(p, "k_gemm_trail", fl, by, gemm_fn<T, 1, 64>(sp), xcd ? dim3(xcd_grid(t64)) : band_grid(t64, act),
+               ids + L.f_off, (const FrontDev*)p->d_fronts, arena, kstart, kw, ccap, xcd ? t64 : -1, act);
     };
-    if (ttr > 0) trail(p->stream, ttr, ttr128, ftr, btr, g0, GW, hend, -1);
-    if (tou > 0) {
-      // Look-ahead: the outer group's update of the columns beyond the next outer group runs on
-      // the side stream, beside the next group's factorization (which touches only its own
-      // columns); the update of the next group's columns stays on the plan stream.  Every entry
-      // still receives the same updates in the same order: the plan stream waits for the
-      // previous side update before this group's update of its columns, and the side stream
-      // runs its updates in order.
-      if (side_pending) HIPCHK(hipStreamWaitEvent(p->stream, p->ev_join, 0));
-      side_pending = false;
-      if (tsb > 0) {
-        HIPCHK(hipEventRecord(p->ev_fork, p->stream));
-        HIPCHK(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-        trail(p->side, tsb, tsb128, fsb, bsb, h0, GW2, 0x7fffffff, hend + GW2);
-        HIPCHK(hipEventRecord(p->ev_join, p->side));
-        side_pending = true;
-      }
-      trail(p->stream, tou, tou128, fou, bou, h0, GW2, ahead ? hend + GW2 : 0x7fffffff, -1);
-    }
+    if (ttr > 0) trail(ttr, ttr128, ftr, btr, g0, GW, hend);
+    if (tou > 0) trail(tou, tou128, fou, bou, h0, GW2, 0x7fffffff);
   }
-  if (side_pending) HIPCHK(hipStreamWaitEvent(p->stream, p->ev_join, 0));  // before the Schur update
-  side_pending = false;
   // 4. Schur complement into the trailing block (every front of the level)
   if (L.maxm > 0) {
     // per front: 64- and 128-tiles of the trailing block, flops, bytes; launched over runs
@@ -846,13 +801,13 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
       if (tiles > 0) {
         if (wide(tiles128, c) && kGemm8<T>)
           launch_b(p, "k_gemm_schur", fl, by, gemm_fn8<T, 2>(), grid(tiles128), dim3(512), ids + L.f_off + i,
-                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c, -1);
+                   (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c);
         else if (wide(tiles128, c))
           launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 128>(sp), grid(tiles128), ids + L.f_off + i,
-                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c, -1);
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles128 : -1, c);
         else
           launch(p, "k_gemm_schur", fl, by, gemm_fn<T, 2, 64>(sp), grid(tiles), ids + L.f_off + i,
-                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : -1, c, -1);
+                 (const FrontDev*)p->d_fronts, arena, 0, 0, 0, xc ? tiles : -1, c);
       }
       i = e;
     }
@@ -1701,9 +1656,6 @@ void free_plan(umf_plan* p) {
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
-  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->stream) (void)hipStreamDestroy(p->stream);
 }
 
@@ -1849,13 +1801,8 @@ int setup_device(umf_plan* p, umf_plan** out) {
     return bail(he == hipErrorOutOfMemory ? UMF_ENOMEM : UMF_EDEVICE, std::string("upload: ") + hipGetErrorString(he));
   if ((he = hipGetDevice(&p->device)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipStreamCreate(&p->stream)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
-  if ((he = hipStreamCreate(&p->side)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipEventCreate(&p->ev0)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipEventCreate(&p->ev1)) != hipSuccess) return bail(UMF_EDEVICE, hipGetErrorString(he));
-  if ((he = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess)
-    return bail(UMF_EDEVICE, hipGetErrorString(he));
-  if ((he = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming)) != hipSuccess)
-    return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMalloc((void**)&p->arena, sizeof(double) * std::max<i64>(P.arena_elems, 1))) != hipSuccess)
     return bail(UMF_ENOMEM, std::string("front pool of ") + std::to_string(P.arena_elems * 8.0 / 1e9) +
                                 " GB: " + hipGetErrorString(he));
@@ -2878,11 +2825,8 @@ int make_slot(umf_plan* b, umf_plan** out) {
 #define SLOTCHK(x) if ((e = (x)) != hipSuccess) return bail(e)
   // non-blocking: a slot's work never waits on the null stream's, so slots overlap
   SLOTCHK(hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking));
-  SLOTCHK(hipStreamCreateWithFlags(&q->side, hipStreamNonBlocking));
   SLOTCHK(hipEventCreate(&q->ev0));
   SLOTCHK(hipEventCreate(&q->ev1));
-  SLOTCHK(hipEventCreateWithFlags(&q->ev_fork, hipEventDisableTiming));
-  SLOTCHK(hipEventCreateWithFlags(&q->ev_join, hipEventDisableTiming));
   SLOTCHK(hipMalloc((void**)&q->d_mats, sizeof(MatDev) * std::max<size_t>(q->mats_h.size(), 1)));
   SLOTCHK(hipMalloc((void**)&q->d_fronts, sizeof(FrontDev) * std::max<size_t>(NF, 1)));
   SLOTCHK(hipMalloc((void**)&q->d_seps, sizeof(SepDev) * std::max<size_t>(NS, 1)));
