@@ -84,8 +84,8 @@ def measured_traffic(kernel, workload, prec):
 # None), multiplied by `scale` (one LN(0,2) draw when None) or set to `setval`), coefficient
 # field, right-hand sides, seed, the form whose rate is `value`, and the CPU baseline's sample
 WORKLOADS = {
-    "3d127": dict(dims=(127, 127, 127), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d47"),
-    "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d47"),
+    "3d127": dict(dims=(127, 127, 127), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d63"),
+    "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3, value_form="fp32", cpu="3d63"),
     "2d1023": dict(dims=(1023, 1023), max_side=16, box=32, scale=100.0, seed=2, value_form="fp32", cpu="2d511"),
     "2d127": dict(dims=(127, 127), max_side=8, box=8, lo=(60, 60), setval=10.0, coeff="uniform", rhs="uniform",
                   nrhs=1, seed=1, value_form="fp64", cpu="2d127"),
@@ -95,6 +95,7 @@ WORKLOADS = {
 # bounded samples of the same generators for the reference's CPU run (about 10-30 s of CPU work)
 CPU_SAMPLES = {
     "3d47": dict(dims=(47, 47, 47), max_side=8, box=8, scale=None, seed=3),
+    "3d63": dict(dims=(63, 63, 63), max_side=8, box=8, scale=None, seed=3),
     "3d79": dict(dims=(79, 79, 79), max_side=8, box=8, scale=None, seed=3),
     "3d95": dict(dims=(95, 95, 95), max_side=8, box=8, scale=None, seed=3),
     "2d511": dict(dims=(511, 511), max_side=16, box=32, scale=100.0, seed=2),
