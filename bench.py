@@ -398,9 +398,12 @@ def run_form(run, prec, steps, warmup, args, keep_x64=False):
             d = fold(plan.profile_read()).get(dom, {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
         plan.profile_reset()
         launches = max(d["launches"], 1)
-        if d["flops"] > 0 and d["ms"] > 0:
+        # the bound is the ceiling the kernel's algorithmic intensity meets first: MFMA above the
+        # ridge point (peak flop/s over 8 TB/s), HBM below it (the sweeps, small-front panels)
+        peak = PEAK_TFLOPS[prec]
+        ridge = peak * 1e12 / (PEAK_HBM_GBS * 1e9)
+        if d["flops"] > 0 and d["ms"] > 0 and d["flops"] >= ridge * d["bytes"]:
             ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
-            peak = PEAK_TFLOPS[prec]
             roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
         else:
             ach = d["bytes"] / (max(d["ms"], 1e-9) * 1e-3) / 1e9
