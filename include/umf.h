@@ -245,10 +245,11 @@ int umf_set_lu(umf_plan* plan, int on);
 int umf_get_lu(const umf_plan* plan);
 
 /* Publication between the workgroups of the blocked solve sweeps (no reference counterpart):
- * fenced = 0 (default) relies on gfx950's write-through agent-scope atomics, with no cache
- * maintenance; fenced = 1 adds agent-scope release / acquire fences around the block flags,
- * which the HIP memory model orders formally (slower: L2 write-back / invalidate per block).
- * Results are identical. */
+ * fenced = 0 (default) publishes each block as 64-bit words that carry the launch's epoch beside
+ * the data (relaxed agent-scope atomics, polled by the readers: no flag, no cache maintenance);
+ * fenced = 1 writes the block, then sets a per-block flag between agent-scope release / acquire
+ * fences (slower: L2 write-back / invalidate per block).  Both are ordered by the HIP memory
+ * model; results are identical. */
 int umf_set_sweep_sync(umf_plan* plan, int fenced);
 
 /* hipStream_t of the plan (for callers ordering their own work), returned as void*. */

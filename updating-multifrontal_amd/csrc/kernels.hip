@@ -2323,27 +2323,78 @@ __global__ __launch_bounds__(256, 2) void k_solve_fwd_r(const int* __restrict__ 
 }
 
 // ---- blocked sweeps for large fronts: one workgroup per 64-row block, blocks of a front
-// hand their results to one another through per-block flags (set to the launch's epoch).
-// A workgroup only waits for blocks owned by workgroups with smaller ids, which the hardware
-// dispatches first, and every wait is bounded: past the limit err[3] is set and the
-// workgroup goes on, so the launch always drains.
-// The rows a block publishes to the other blocks (X) are written and read with relaxed
-// agent-scope atomics, which go through to memory instead of a CU's L1 or an XCD's L2, so no
-// cache maintenance is needed: the flag is polled with relaxed agent-scope loads, and the
-// publisher waits for its stores to complete (s_waitcnt) before the barrier, after which one
-// relaxed store sets the flag.  (An agent-scope release / acquire fence instead writes back and
-// invalidates the XCD's L2 at every block, which also evicts the factor rows the other
-// workgroups are streaming.)
-// This relies on how gfx950 code is generated for these operations, not on the HIP memory
-// model, which orders nothing between the relaxed data stores and the relaxed flag store: the
-// agent-scope atomic stores are written through (sc1) and complete in order before the
-// s_waitcnt, and the loads after the flag are agent-scope atomic loads that bypass the
-// non-coherent caches.  FENCED = true adds the agent-scope release fence before the flag store
-// and the acquire fence after the flag is seen (formally ordered under the memory model, at
-// the cost of the L2 write-back / invalidate); the runtime selects it per plan
-// (umf_set_sweep_sync) and the parity tests compare the two.
-template <bool FENCED>
-__device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err) {
+// hand their results to one another inside the launch.
+// The grid is one-dimensional and interleaved over the launch's fronts: workgroup id
+// s * nf + y owns slot s of front y (forward: interior slots first, then the boundary slots),
+// so every front's chain starts in the first wave of workgroups.  A workgroup only waits for
+// workgroups with smaller ids, which the hardware dispatches first, and every wait is bounded:
+// past the limit err[3] is set and the workgroup goes on, so the launch always drains.
+//
+// Publication, SYNC = 0 (the default): flag-in-data.  A block's result x_b (64 rows x 16
+// columns) is written as 64-bit words, each holding one 32-bit piece of a value in its low
+// half and the launch's epoch in its high half (FP64 values take two words), with relaxed
+// agent-scope atomic stores; a consumer polls the words themselves with relaxed agent-scope
+// atomic loads until every word it needs carries the epoch.  A 64-bit atomic access is
+// single-copy atomic, so a word that carries the epoch carries this launch's value: the
+// handoff is ordered by the memory model alone, needs no flag, no store completion wait and
+// no workgroup barrier on either side, and costs one store-to-load latency per block.
+// SYNC = 1: the result goes to X with relaxed atomic stores, then a per-block flag is set
+// after an agent-scope release fence and read before an agent-scope acquire fence (the L2
+// write-back / invalidate they imply makes it slower); the parity tests compare the two.
+template <class T>
+constexpr int kLLW = (int)sizeof(T) / 4;  // 64-bit words per published value
+constexpr int kLLBlock = NB * RB * 2;      // words per published block (room for FP64)
+
+__device__ __forceinline__ void ll_put32(unsigned long long* p, unsigned v, int epoch) {
+  __hip_atomic_store(p, ((unsigned long long)(unsigned)epoch << 32) | v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void ll_put(unsigned long long* p, T v, int epoch) {
+  if constexpr (sizeof(T) == 4) {
+    ll_put32(p, __builtin_bit_cast(unsigned, v), epoch);
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    ll_put32(p, (unsigned)b, epoch);
+    ll_put32(p + 1, (unsigned)(b >> 32), epoch);
+  }
+}
+// this lane's 4 values of wave w's quarter of a published 64 x 16 block (row 16 w + 4 s + q,
+// column c16; rows >= kn read as 0), polled until every word carries `epoch`.  Each poll
+// issues all of the lane's loads together (rows past kn load row kn - 1 and are ignored).
+template <class T>
+__device__ __forceinline__ void ll_quarter(const unsigned long long* blk, int w, int kn, int q, int c16, int epoch,
+                                           T (&bv)[4], int* err) {
+  unsigned long long* base = const_cast<unsigned long long*>(blk) + c16 * kLLW<T>;
+  for (int spins = 0;;) {
+    unsigned long long w0[4], w1[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      unsigned long long* a = base + min(16 * w + 4 * s4 + q, kn - 1) * (RB * kLLW<T>);
+      w0[s4] = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if constexpr (sizeof(T) == 8) w1[s4] = __hip_atomic_load(a + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      const bool live = 16 * w + 4 * s4 + q < kn;
+      if constexpr (sizeof(T) == 4) {
+        ok = ok && (!live || (int)(w0[s4] >> 32) == epoch);
+        bv[s4] = live ? __builtin_bit_cast(T, (unsigned)w0[s4]) : T(0);
+      } else {
+        ok = ok && (!live || ((int)(w0[s4] >> 32) == epoch && (int)(w1[s4] >> 32) == epoch));
+        bv[s4] = live ? __builtin_bit_cast(T, (w0[s4] & 0xffffffffull) | (w1[s4] << 32)) : T(0);
+      }
+    }
+    if (ok) break;
+    if (++spins > SPIN_LIMIT) {
+      atomicCAS(err + 3, 0, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+__device__ __forceinline__ void wait_block_fenced(const int* flag, int epoch, int* err) {
   if (threadIdx.x == 0) {
     int spins = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
@@ -2353,20 +2404,17 @@ __device__ __forceinline__ void wait_block(const int* flag, int epoch, int* err)
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    if (FENCED) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
-template <bool FENCED>
-__device__ __forceinline__ void post_block(int* flag, int epoch) {
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's published stores have completed
+__device__ __forceinline__ void post_block_fenced(int* flag, int epoch) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (FENCED) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
-// published rows: relaxed agent-scope atomic accesses
 template <class T>
 __device__ __forceinline__ T ld_pub(const T* p) {
   return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2375,129 +2423,217 @@ template <class T>
 __device__ __forceinline__ void st_pub(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// acc[rb] += (the factor tile av: 64 output rows x this wave's 16 of block j's rows) x (those
+// published rows: 16 w + 4 s + q from k0, column c16).  Each wave takes a quarter of the block's
+// rows for all 64 output rows, so a workgroup reads each published block once.
+template <class T, int SYNC>
+__device__ __forceinline__ void acc_block(const unsigned long long* ll, const T* X, const int* fl, int j, int k0,
+                                          int kn, int w, int q, int c16, int epoch, const T (&av)[4][4],
+                                          V4<T> (&acc)[4], int* err) {
+  T bv[4];
+  if constexpr (SYNC == 0) {
+    ll_quarter<T>(ll + (i64)j * kLLBlock, w, kn, q, c16, epoch, bv, err);
+  } else {
+    wait_block_fenced(fl + j, epoch, err);
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      const int r = 16 * w + 4 * s4 + q;
+      const T v = ld_pub(X + (i64)(k0 + min(r, kn - 1)) * RB + c16);
+      bv[s4] = r < kn ? v : T(0);
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) acc[rb] = mfma4(av[rb][s4], bv[s4], acc[rb]);
+}
+// sum of the four waves' partial 64 x 16 blocks (red: [wave][row][column]); wave w gets rows
+// 16 w + mrow(q, r) in acc[w]'s layout.  Every wave of the workgroup calls it.
+template <class T>
+__device__ __forceinline__ V4<T> reduce_waves(T (*red)[NB][RB], const V4<T> (&acc)[4], int w, int q, int c16) {
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[w][16 * rb + mrow<T>(q, r)][c16] = acc[rb][r];
+  __syncthreads();
+  V4<T> s = vzero<T>();
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int i = 16 * w + mrow<T>(q, r);
+    s[r] = ((red[0][i][c16] + red[1][i][c16]) + red[2][i][c16]) + red[3][i][c16];
+  }
+  return s;
+}
 
-// Forward sweep of one front by blocks (blockIdx.x = block, blockIdx.y = front): interior block
-// b waits for the blocks j < b, forms v = b_b - sum_j L_bj x_j and publishes x_b = Linv_b v
-// (to X and to W[int]); boundary block b accumulates over every interior block and leaves
-// X[rows] = -sum_j L_bj x_j, the contribution gathered by k_sep_acc.
-template <class T, bool FENCED>
-__global__ __launch_bounds__(256) void k_fwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                 const T* __restrict__ A, const int* __restrict__ dofs,
-                                                 T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
-                                                 int* err) {
-  const FrontDev F = fr[ids[blockIdx.y]];
-  const int nbi = (F.n + NB - 1) / NB, nbt = nbi + (F.m + NB - 1) / NB;
-  const int b = blockIdx.x;
-  if (b >= nbt) return;
-  const bool interior = b < nbi;
-  const int r0 = interior ? NB * b : F.n + NB * (b - nbi);
+// Forward sweep of one front by blocks: interior block b waits for the blocks j < b, forms
+// v = b_b - sum_j L_bj x_j and publishes x_b = Linv_b v (and writes it to W[int]); boundary
+// block b accumulates over every interior block and leaves X[rows] = -sum_j L_bj x_j, the
+// contribution gathered by k_sep_acc.  nbi_g: the most interior blocks of a front of the
+// launch (the grid holds nf x (nbi_g + the most boundary blocks) workgroups); ll: the launch's
+// published blocks, kLLBlock words per (front, block).
+template <class T, int SYNC>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_fwd_blk(const int* __restrict__ ids, int nf, int nbi_g,
+                                                 const FrontDev* __restrict__ fr, const T* __restrict__ A,
+                                                 const int* __restrict__ dofs, T* __restrict__ W, T* __restrict__ xs,
+                                                 int* flags, unsigned long long* ll, int epoch, int* err) {
+  const int id = blockIdx.x;
+  const bool interior = id < nf * nbi_g;
+  const int rid = interior ? id : id - nf * nbi_g;
+  const int slot = rid / nf, y = rid - slot * nf;
+  const FrontDev F = fr[ids[y]];
+  const int nbi = (F.n + NB - 1) / NB, nbb = (F.m + NB - 1) / NB;
+  if (interior ? slot >= nbi : slot >= nbb) return;
+  const int b = interior ? slot : nbi + slot;
+  const int r0 = interior ? NB * b : F.n + NB * slot;
   const int rend = min(interior ? F.n : F.N, r0 + NB);
   const T* L = A + F.off;
   T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   int* fl = flags + F.flag_off;
+  unsigned long long* llf = ll + (i64)y * nbi_g * kLLBlock;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  __shared__ T red[4][NB][RB];
   __shared__ T vb[NB][RB];
-  const int row = r0 + 16 * w + c16;  // L row supplied by this lane
-  const bool rok = row < rend;
-  const int rowc = rok ? row : r0;
-  V4<T> acc = vzero<T>();
+  int rowc[4];  // L rows supplied by this lane, one per 16-row block (clamped; `rok` masks)
+  bool rok[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int row = r0 + 16 * rb + c16;
+    rok[rb] = row < rend;
+    rowc[rb] = rok[rb] ? row : r0;
+  }
+  V4<T> acc[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) acc[rb] = vzero<T>();
   const int jmax = interior ? b : nbi;
-  // this block's inverse (factor data, final before the launch) is read before any wait
+  // the factor data (this block's inverse) and this block's right-hand-side rows (written by no
+  // other block of the launch; kept in vb, each lane its own entries) are read before any wait
   T li[NB / 4];
   if (interior) {
     const T* Li = A + F.dinv_off + (i64)b * NB * NB;  // Linv(i, c) at Li[c*NB + i]
 #pragma unroll
     for (int k4 = 0; k4 < NB / 4; k4++) li[k4] = Li[(4 * k4 + q) * NB + 16 * w + c16];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      vb[16 * w + mrow<T>(q, r)][c16] = W[(i64)di[min(r0 + 16 * w + mrow<T>(q, r), rend - 1)] * RB + c16];
   }
   for (int j = 0; j < jmax; j++) {
     const int jb = NB * j, kb = min(NB, F.n - jb);
-    T av[NB / 4], bv[NB / 4];
+    T av[4][4];
 #pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block j's wait
-      const int k = 4 * k4 + q;
-      av[k4] = ld_or0(L + (i64)(jb + min(k, kb - 1)) * F.N + rowc, rok && k < kb);
+    for (int s4 = 0; s4 < 4; s4++) {  // the factor tile: issued before block j's wait
+      const int k = 16 * w + 4 * s4 + q;
+      const T* Lk = L + (i64)(jb + min(k, kb - 1)) * F.N;
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) av[rb][s4] = ld_or0(Lk + rowc[rb], rok[rb] && k < kb);
     }
-    wait_block<FENCED>(fl + j, epoch, err);
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {
-      const int k = 4 * k4 + q;
-      const T v = ld_pub(X + (i64)(jb + min(k, kb - 1)) * RB + c16);
-      bv[k4] = k < kb ? v : T(0);
-    }
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
+    acc_block<T, SYNC>(llf, X, fl, j, jb, kb, w, q, c16, epoch, av, acc, err);
   }
+  const V4<T> tot = reduce_waves<T>(red, acc, w, q, c16);
   if (!interior) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = r0 + 16 * w + mrow<T>(q, r);
-      if (i < rend) X[(i64)i * RB + c16] = -acc[r];
+      if (i < rend) X[(i64)i * RB + c16] = -tot[r];
     }
     return;
   }
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = 16 * w + mrow<T>(q, r);
-    const bool ok = r0 + i < rend;
-    vb[i][c16] = ld_or0(W + (i64)di[min(r0 + i, rend - 1)] * RB + c16, ok) - (ok ? acc[r] : T(0));
+    vb[i][c16] = r0 + i < rend ? vb[i][c16] - tot[r] : T(0);
   }
   __syncthreads();
   V4<T> a2 = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(li[k4], vb[4 * k4 + q][c16], a2);
+  unsigned long long* dst = llf + (i64)b * kLLBlock;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int i = r0 + 16 * w + mrow<T>(q, r);
-    if (i < rend) {
-      st_pub(X + (i64)i * RB + c16, a2[r]);
-      W[(i64)di[i] * RB + c16] = a2[r];
+    const int i = 16 * w + mrow<T>(q, r);
+    if (r0 + i < rend) {
+      if constexpr (SYNC == 0)
+        ll_put<T>(dst + (i * RB + c16) * kLLW<T>, a2[r], epoch);
+      else
+        st_pub(X + (i64)(r0 + i) * RB + c16, a2[r]);
     }
   }
-  post_block<FENCED>(fl + b, epoch);
+  if constexpr (SYNC == 1) post_block_fenced(fl + b, epoch);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {  // after the publication: W is read by later launches only
+    const int i = 16 * w + mrow<T>(q, r);
+    if (r0 + i < rend) W[(i64)di[r0 + i] * RB + c16] = a2[r];
+  }
 }
 
-// Backward sweep of one front by blocks, last block first (workgroup b owns block
-// nbi-1-b): z = y_i - L(rows below, i)^T x(rows below), boundary rows (read from W[bd]) first,
-// then the interior blocks below as they are published; x_i = Linv_i^T z goes to X and W[int].
-template <class T, bool FENCED>
-__global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, const FrontDev* __restrict__ fr,
-                                                 const T* __restrict__ A, const int* __restrict__ dofs,
-                                                 T* __restrict__ W, T* __restrict__ xs, int* flags, int epoch,
-                                                 int* err) {
-  const FrontDev F = fr[ids[blockIdx.y]];
+// Backward sweep of one front by blocks, last block first (slot s owns block nbi-1-s):
+// z = y_i - L(rows below, i)^T x(rows below), boundary rows (read from W[bd]) first, then the
+// interior blocks below as they are published; x_i = Linv_i^T z is published and written to
+// W[int].  Over the boundary rows each wave takes 16 of the block's columns; over the interior
+// blocks below, as in the forward sweep, each wave takes a quarter of every published block for
+// all 64 columns, and the waves' partial sums are added at the end.
+template <class T, int SYNC>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_bwd_blk(const int* __restrict__ ids, int nf, int nbi_g,
+                                                 const FrontDev* __restrict__ fr, const T* __restrict__ A,
+                                                 const int* __restrict__ dofs, T* __restrict__ W, T* __restrict__ xs,
+                                                 int* flags, unsigned long long* ll, int epoch, int* err) {
+  const int slot = blockIdx.x / nf, y = blockIdx.x - slot * nf;
+  const FrontDev F = fr[ids[y]];
   const int nbi = (F.n + NB - 1) / NB;
-  if ((int)blockIdx.x >= nbi) return;
-  const int ib = nbi - 1 - blockIdx.x;
+  if (slot >= nbi) return;
+  const int ib = nbi - 1 - slot;
   const int jb = NB * ib, kb = min(NB, F.n - jb);
   const T* L = A + F.off;
   T* X = xs + F.x_off;
   const int* di = dofs + F.int_off;
   const int* db = dofs + F.bd_off;
   int* fl = flags + F.flag_off;
+  unsigned long long* llf = ll + (i64)y * nbi_g * kLLBlock;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, c16 = lane & 15, q = lane >> 4;
+  __shared__ T red[4][NB][RB];
   __shared__ T zb[NB][RB];
-  const int col = 16 * w + c16;  // column of the block supplied by this lane
-  const bool cin = col < kb;
-  const T* Lc = L + (i64)(jb + (cin ? col : 0)) * F.N;
-  V4<T> acc = vzero<T>();
-  for (int i0 = F.n; i0 < F.N; i0 += 16 * GU) {  // boundary rows: x = W[bd]
-    T av[GU][4], bv[GU][4];
+  const T* Lc[4];  // the block's columns supplied by this lane, one per 16-column block
+  bool cin[4];
 #pragma unroll
-    for (int u = 0; u < GU; u++)
-#pragma unroll
-      for (int k4 = 0; k4 < 4; k4++) {
-        const int i = i0 + 16 * u + 4 * k4 + q;
-        const bool ok = i < F.N;
-        const int ic = min(i, F.N - 1);
-        av[u][k4] = ld_or0(Lc + ic, ok && cin);
-        bv[u][k4] = ld_or0(W + (i64)db[ic - F.n] * RB + c16, ok);
-      }
-#pragma unroll
-    for (int u = 0; u < GU; u++)
-#pragma unroll
-      for (int k4 = 0; k4 < 4; k4++) acc = mfma4(av[u][k4], bv[u][k4], acc);
+  for (int rb = 0; rb < 4; rb++) {
+    const int col = 16 * rb + c16;
+    cin[rb] = col < kb;
+    Lc[rb] = L + (i64)(jb + (cin[rb] ? col : 0)) * F.N;
   }
+  // this block's right-hand-side rows (written by no other block of the launch), kept in zb
+  // (each lane its own entries) until the end
+  T wv[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) wv[r] = W[(i64)di[jb + min(16 * w + mrow<T>(q, r), kb - 1)] * RB + c16];
+  // boundary rows (x = W[bd]): wave w takes the block's columns 16 w .. 16 w + 15 over all rows
+  V4<T> accb = vzero<T>();
+  {
+    const T* Lw = Lc[0] + (i64)16 * w * F.N;  // column jb + 16 w + c16 (clamped to jb when past kb)
+    const bool cw = 16 * w + c16 < kb;
+    if (!cw) Lw = L + (i64)jb * F.N;
+    for (int i0 = F.n; i0 < F.N; i0 += 16 * GU) {
+      T av[GU][4], bv[GU][4];
+#pragma unroll
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) {
+          const int i = i0 + 16 * u + 4 * k4 + q;
+          const bool ok = i < F.N;
+          const int ic = min(i, F.N - 1);
+          av[u][k4] = ld_or0(Lw + ic, ok && cw);
+          bv[u][k4] = ld_or0(W + (i64)db[ic - F.n] * RB + c16, ok);
+        }
+#pragma unroll
+      for (int u = 0; u < GU; u++)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) accb = mfma4(av[u][k4], bv[u][k4], accb);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) zb[16 * w + mrow<T>(q, r)][c16] = wv[r] - accb[r];
+  V4<T> acc[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) acc[rb] = vzero<T>();
   // this block's inverse (factor data, final before the launch) is read before any wait
   T li[NB / 4];
   {
@@ -2507,40 +2643,42 @@ __global__ __launch_bounds__(256) void k_bwd_blk(const int* __restrict__ ids, co
   }
   for (int kbk = nbi - 1; kbk > ib; kbk--) {  // interior blocks below, in publication order
     const int k0 = NB * kbk, kn = min(NB, F.n - k0);
-    T av[NB / 4], bv[NB / 4];
+    T av[4][4];
 #pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {  // the factor tile: issued before block kbk's wait
-      const int k = 4 * k4 + q;
-      av[k4] = ld_or0(Lc + k0 + min(k, kn - 1), k < kn && cin);
+    for (int s4 = 0; s4 < 4; s4++) {  // the factor tile: issued before block kbk's wait
+      const int k = 16 * w + 4 * s4 + q;
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) av[rb][s4] = ld_or0(Lc[rb] + k0 + min(k, kn - 1), k < kn && cin[rb]);
     }
-    wait_block<FENCED>(fl + kbk, epoch, err);
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) {
-      const int k = 4 * k4 + q;
-      const T v = ld_pub(X + (i64)(k0 + min(k, kn - 1)) * RB + c16);
-      bv[k4] = k < kn ? v : T(0);
-    }
-#pragma unroll
-    for (int k4 = 0; k4 < NB / 4; k4++) acc = mfma4(av[k4], bv[k4], acc);
+    acc_block<T, SYNC>(llf, X, fl, kbk, k0, kn, w, q, c16, epoch, av, acc, err);
   }
+  const V4<T> tot = reduce_waves<T>(red, acc, w, q, c16);
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int c = 16 * w + mrow<T>(q, r);
-    zb[c][c16] = ld_or0(W + (i64)di[jb + min(c, kb - 1)] * RB + c16, c < kb) - (c < kb ? acc[r] : T(0));
+    zb[c][c16] = c < kb ? zb[c][c16] - tot[r] : T(0);
   }
   __syncthreads();
   V4<T> a2 = vzero<T>();
 #pragma unroll
   for (int k4 = 0; k4 < NB / 4; k4++) a2 = mfma4(li[k4], zb[4 * k4 + q][c16], a2);
+  unsigned long long* dst = llf + (i64)ib * kLLBlock;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int i = 16 * w + mrow<T>(q, r);
     if (i < kb) {
-      st_pub(X + (i64)(jb + i) * RB + c16, a2[r]);
-      W[(i64)di[jb + i] * RB + c16] = a2[r];
+      if constexpr (SYNC == 0)
+        ll_put<T>(dst + (i * RB + c16) * kLLW<T>, a2[r], epoch);
+      else
+        st_pub(X + (i64)(jb + i) * RB + c16, a2[r]);
     }
   }
-  post_block<FENCED>(fl + ib, epoch);
+  if constexpr (SYNC == 1) post_block_fenced(fl + ib, epoch);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {  // after the publication: W is read by later launches only
+    const int i = 16 * w + mrow<T>(q, r);
+    if (i < kb) W[(i64)di[jb + i] * RB + c16] = a2[r];
+  }
 }
 
 // Column-major user block (n x nr, ld) <-> row-major work block (n x 16); unused columns zero.

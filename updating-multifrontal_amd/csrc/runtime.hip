@@ -134,7 +134,9 @@ struct umf_plan {
   int* d_ci = nullptr;
   double* d_val = nullptr;
   int* d_err = nullptr;     // [0..3] error flags, [4] the queue counter of the persistent launches
-  int* d_flags = nullptr;  // block flags of the blocked solve sweeps
+  int* d_flags = nullptr;  // block flags of the blocked solve sweeps (fenced publication)
+  unsigned long long* d_ll = nullptr;  // published blocks of a blocked sweep launch (flag-in-data)
+  i64 ll_blocks = 0;                   // (front, block) capacity of d_ll: kLLBlock words each
   int* d_arrive = nullptr;  // per-front arrival counters of k_chol_dpanel (zero between launches)
   int n_flags = 0;
   // coefficient-change requests: per-DOF change flags (epoch-tagged), the collected list
@@ -1194,13 +1196,18 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
   for (int ell = 0; ell < L; ell++)
     for (int f : P.box_gid[ell]) maxN[ell] = std::max(maxN[ell], P.fronts[f].n + P.fronts[f].m);
   auto blocked = [&](int ell) { return A.f_cnt[ell] <= 16 || (A.f_cnt[ell] <= 128 && maxN[ell] >= 2048); };
-  auto blocks = [&](int ell, int& nbi, int& nbt) {
-    nbi = nbt = 1;
+  // the most interior / boundary 64-row blocks of a front of level ell; a blocked launch
+  // publishes at most f_cnt x nbi blocks, which the plan's buffer holds by construction
+  int blk_rc = UMF_OK;
+  auto blocks = [&](int ell, int& nbi, int& nbb) {
+    nbi = 1;
+    nbb = 0;
     for (int f : P.box_gid[ell]) {
       const FrontInfo& F = P.fronts[f];
       nbi = std::max(nbi, (F.n + NB - 1) / NB);
-      nbt = std::max(nbt, (F.n + NB - 1) / NB + (F.m + NB - 1) / NB);
+      nbb = std::max(nbb, (F.m + NB - 1) / NB);
     }
+    if ((i64)A.f_cnt[ell] * nbi > p->ll_blocks) blk_rc = UMF_EINVAL;
   };
   // FP64 levels of many fronts: fronts of at most 64 G rows keep the right-hand-side rows in
   // registers in the forward sweep (G = 6, 12 or 18 chunks of 64 rows), larger ones stream.
@@ -1256,11 +1263,13 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
         launch(p, "k_solve_fwd", part(cnt - i, fl), part(cnt - i, by), k_solve_fwd<T>, dim3(cnt - i), li + i, fr,
                (const T*)arena, (const int*)p->d_dofs, w, xs, (const int*)nullptr);
     } else {
-      int nbi, nbt;
-      blocks(ell, nbi, nbt);
-      launch(p, "k_fwd_blk", fl, by, p->fenced ? k_fwd_blk<T, true> : k_fwd_blk<T, false>, dim3(nbt, A.f_cnt[ell]),
-             ids + A.f_off[ell], fr,
-             (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
+      int nbi, nbb;
+      blocks(ell, nbi, nbb);
+      if (blk_rc) return;
+      const int nf = A.f_cnt[ell];
+      launch(p, "k_fwd_blk", fl, by, p->fenced ? k_fwd_blk<T, 1> : k_fwd_blk<T, 0>, dim3(nf * (nbi + nbb)),
+             ids + A.f_off[ell], nf, nbi, fr, (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, p->d_ll,
+             next_epoch(p), p->d_err);
     }
   };
   if (mode == 0) {
@@ -1300,11 +1309,13 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
           launch(p, "k_solve_bwd", fl * frac(e2, cnt), by * frac(e2, cnt), k_solve_bwd<T>, dim3(cnt - e2),
                  ids + A.f_off[ell] + e2, fr, (const T*)arena, (const int*)p->d_dofs, w, xs);
       } else {
-        int nbi, nbt;
-        blocks(ell, nbi, nbt);
-        launch(p, "k_bwd_blk", fl, by, p->fenced ? k_bwd_blk<T, true> : k_bwd_blk<T, false>, dim3(nbi, A.f_cnt[ell]),
-               ids + A.f_off[ell], fr,
-               (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, next_epoch(p), p->d_err);
+        int nbi, nbb;
+        blocks(ell, nbi, nbb);
+        if (blk_rc) break;
+        const int nf = A.f_cnt[ell];
+        launch(p, "k_bwd_blk", fl, by, p->fenced ? k_bwd_blk<T, 1> : k_bwd_blk<T, 0>, dim3(nf * nbi),
+               ids + A.f_off[ell], nf, nbi, fr, (const T*)arena, (const int*)p->d_dofs, w, xs, p->d_flags, p->d_ll,
+               next_epoch(p), p->d_err);
       }
     }
   } else {
@@ -1319,6 +1330,7 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
     }
   }
   g_level = -1;
+  if (blk_rc) return fail(blk_rc, "blocked sweep: more published blocks than the plan's buffer holds");
   HIPCHK(hipGetLastError());
   return UMF_OK;
 }
@@ -1647,7 +1659,7 @@ void free_plan(umf_plan* p) {
                   p->d_tiles, p->d_tile_off, p->d_rp, p->d_ci, p->d_val, p->d_err, p->d_flags, p->d_ids, p->d_xs, p->d_w, p->d_stage,
                   p->arena32, p->d_xs32, p->d_w32, p->d_cg, p->d_part, p->d_dots, p->d_bflag, p->d_blist,
                   p->d_fly_tab, p->d_fly_tri, p->d_rest_tri, p->d_rest_off, p->d_arrive, p->d_ipiv, p->d_perm,
-                  p->d_uinv, p->d_ut};
+                  p->d_uinv, p->d_ut, p->d_ll};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->fgraph) (void)hipGraphExecDestroy(p->fgraph);
@@ -1777,6 +1789,14 @@ int setup_device(umf_plan* p, umf_plan** out) {
     p->fronts_h.push_back(D);
   }
   p->xs_elems = xo;
+  // a blocked sweep launch holds at most 128 fronts of one level (16 when they are small), each
+  // publishing at most the level's largest block count
+  p->ll_blocks = 0;
+  for (size_t ell = 0; ell < P.box_gid.size(); ell++) {
+    i64 nbmax = 0;
+    for (int f : P.box_gid[ell]) nbmax = std::max<i64>(nbmax, (P.fronts[f].n + NB - 1) / NB);
+    p->ll_blocks = std::max<i64>(p->ll_blocks, std::min<i64>((i64)P.box_gid[ell].size(), 128) * nbmax);
+  }
   for (const SepInfo& S : P.seps) {
     SepDev D{};
     D.off = P.mats[P.nfronts() + (&S - &P.seps[0])].off;
@@ -1839,6 +1859,12 @@ int setup_device(umf_plan* p, umf_plan** out) {
   if ((he = hipMalloc((void**)&p->d_flags, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMemset(p->d_flags, 0, sizeof(int) * std::max(p->n_flags, 1))) != hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMalloc((void**)&p->d_ll, sizeof(unsigned long long) * kLLBlock * std::max<i64>(p->ll_blocks, 1))) !=
+      hipSuccess)
+    return bail(UMF_EDEVICE, hipGetErrorString(he));
+  if ((he = hipMemset(p->d_ll, 0, sizeof(unsigned long long) * kLLBlock * std::max<i64>(p->ll_blocks, 1))) !=
+      hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
   if ((he = hipMalloc((void**)&p->d_arrive, sizeof(int) * std::max(P.nfronts(), 1))) != hipSuccess)
     return bail(UMF_EDEVICE, hipGetErrorString(he));
@@ -2795,6 +2821,7 @@ int make_slot(umf_plan* b, umf_plan** out) {
   q->device = b->device;
   q->xs_elems = b->xs_elems;
   q->n_flags = b->n_flags;
+  q->ll_blocks = b->ll_blocks;
   q->max_tiles_per_mat = b->max_tiles_per_mat;
   q->fronts_h = b->fronts_h;
   q->seps_h = b->seps_h;
@@ -2836,6 +2863,8 @@ int make_slot(umf_plan* b, umf_plan** out) {
   SLOTCHK(hipMemset(q->d_err, 0, sizeof(int) * 8));
   SLOTCHK(hipMalloc((void**)&q->d_flags, sizeof(int) * std::max(q->n_flags, 1)));
   SLOTCHK(hipMemset(q->d_flags, 0, sizeof(int) * std::max(q->n_flags, 1)));
+  SLOTCHK(hipMalloc((void**)&q->d_ll, sizeof(unsigned long long) * kLLBlock * std::max<i64>(q->ll_blocks, 1)));
+  SLOTCHK(hipMemset(q->d_ll, 0, sizeof(unsigned long long) * kLLBlock * std::max<i64>(q->ll_blocks, 1)));
   SLOTCHK(hipMalloc((void**)&q->d_arrive, sizeof(int) * std::max<size_t>(NF, 1)));
   SLOTCHK(hipMemset(q->d_arrive, 0, sizeof(int) * std::max<size_t>(NF, 1)));
   q->ids_cap = NF + 2 * NS;
