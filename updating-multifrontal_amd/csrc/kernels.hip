@@ -274,7 +274,7 @@ struct TileSmemT {
 
 // LOWER (P = Q, only the lower triangle of the product wanted): the wave above the diagonal
 // (wr < wc) only stages operands.
-template <class T, int TS, bool LOWER = false, int DEPTH = 1>
+template <class T, int TS, bool LOWER = false, int DEPTH = 1, bool PRIO = false>
 __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], const T* __restrict__ P, int ldp,
                                                int pn, const T* __restrict__ Q, int ldq, int qn, int K,
                                                TileSmemT<T, TS>& sm) {
@@ -334,10 +334,10 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
       }
     };
     if (mma) rd(0, 0);
-    // 64-tiles (the panels' GEMMs, four tiles per CU): issue priority to the wave in its MFMA
-    // phase (FP64 3D 127^3 panels 306 -> 296 ms per step; the 128-tile GEMMs and the diagonal
+    // PRIO (the panels' 64-tile GEMMs, four tiles per CU): issue priority to the wave in its MFMA
+    // phase (FP64 3D 127^3 panels 306 -> 296 ms per step; the stage GEMMs' tiles and the diagonal
     // blocks' SYRK measured slower with it)
-    if constexpr (TS == 64 && !LOWER) __builtin_amdgcn_s_setprio(1);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int k4 = 0; k4 < 4 && mma; k4++) {
       if (k4 < 3) rd(k4 + 1, (k4 + 1) & 1);
@@ -347,7 +347,7 @@ __device__ __forceinline__ void tile_gemm_nt_t(V4<T> (&acc)[TS / 32][TS / 32], c
 #pragma unroll
         for (int bj = 0; bj < NBK; bj++) acc[bi][bj] = mfma4(av[k4 & 1][bj], bv[k4 & 1][bi], acc[bi][bj]);
     }
-    if constexpr (TS == 64 && !LOWER) __builtin_amdgcn_s_setprio(0);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
     stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
     fetch(kc + 16 * (DEPTH + 1), next);
     __syncthreads();
@@ -436,8 +436,13 @@ __device__ __forceinline__ void tile_gemm_nt_8(V4<T> (&acc)[4][2], const T* __re
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) acc[bi][bj] = mfma4(av[k4 & 1][bj], bv[k4 & 1][bi], acc[bi][bj]);
     }
+    // issue priority to the wave staging and fetching the next chunks (four waves per SIMD: FP64
+    // 3D 127^3 trailing update 1504 -> 1482 ms per step; the 4-wave FP32 128-tile and the 64 x 128
+    // paired-step tiles measured slower with it, in either phase)
+    __builtin_amdgcn_s_setprio(1);
     stage(b ^ 1, next);  // past the last chunk: a harmless write of zeros to the unused stage
     fetch(kc + 16 * (DEPTH + 1), next);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
   };
   __syncthreads();  // the stages may still be read by an earlier use of the shared memory
@@ -1080,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_chol_dpanel(const int* __restrict__ ids
   const int K = nrow > 0 ? jb - kstart : 0;
   if (K > 0) {
     const T* G = A + F.off + (i64)kstart * F.N;
-    tile_gemm_nt_t<T, 64>(acc, G + row0, F.N, nrow, G + jb, F.N, kb, K, sm.g);
+    tile_gemm_nt_t<T, 64, false, 1, true>(acc, G + row0, F.N, nrow, G + jb, F.N, kb, K, sm.g);
   }
   // X = the tile minus the GEMM (its stages are done: the GEMM ends on a barrier)
 #pragma unroll
@@ -1281,7 +1286,7 @@ __device__ __forceinline__ void panel_tile(const FrontDev& F, T* __restrict__ A,
   const bool from_csr = fly && F.fly_off >= 0;
   T pv[2][2][4];
   tile_read(pv, pan, F.N, nrow, kb, !from_csr);
-  tile_gemm_nt_t<T, 64>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
+  tile_gemm_nt_t<T, 64, false, 1, true>(acc, A + F.off + row0, F.N, nrow, A + F.off + jb, F.N, kb, K, sm.g);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
