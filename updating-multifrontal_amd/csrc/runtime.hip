@@ -1117,6 +1117,13 @@ int next_epoch(umf_plan* p) {
   return p->epoch;
 }
 
+// Levels of few (<= 16) fronts, or of at most 128 fronts of 2048+ rows (max_rows: the level's
+// largest n + m), are swept by 64-row blocks, one workgroup each (k_fwd_blk / k_bwd_blk): one
+// workgroup per front would leave most of the 256 CUs idle while it streams its factor (3D 127^3:
+// 64 fronts of 8557 rows at 0.7 TB/s).  Levels of many fronts keep one workgroup per front
+// (blocks on the 3D 127^3 leaf level, 512 fronts of 4725 rows, measured slower).
+bool sweep_blocked(i64 nfronts, int max_rows) { return nfronts <= 16 || (nfronts <= 128 && max_rows >= 2048); }
+
 // Solve (mode 0) or Apply (mode 1) of nr <= RB columns held in the row-major work block w
 // (n x RB), fronts of type T in `arena`, per-front scratch xs.
 template <class T>
@@ -1187,15 +1194,10 @@ int sweep(umf_plan* p, int mode, int nr, T* arena, T* w, T* xs, int* ids_dev = n
     }
   };
   double fl, by;
-  // Levels of few (<= 16) fronts, or of at most 128 fronts of 2048+ rows, are swept by 64-row
-  // blocks, one workgroup each (k_fwd_blk / k_bwd_blk): one workgroup per front would leave
-  // most of the 256 CUs idle while it streams its factor (3D 127^3: 64 fronts of 8557 rows at
-  // 0.7 TB/s).  Levels of many fronts keep one workgroup per front (blocks on the 3D 127^3 leaf
-  // level, 512 fronts of 4725 rows, measured slower: FP32 sweeps 40.6 -> 47.8 ms per solve).
   std::vector<int> maxN(L, 0);
   for (int ell = 0; ell < L; ell++)
     for (int f : P.box_gid[ell]) maxN[ell] = std::max(maxN[ell], P.fronts[f].n + P.fronts[f].m);
-  auto blocked = [&](int ell) { return A.f_cnt[ell] <= 16 || (A.f_cnt[ell] <= 128 && maxN[ell] >= 2048); };
+  auto blocked = [&](int ell) { return sweep_blocked(A.f_cnt[ell], maxN[ell]); };
   // the most interior / boundary 64-row blocks of a front of level ell; a blocked launch
   // publishes at most f_cnt x nbi blocks, which the plan's buffer holds by construction
   int blk_rc = UMF_OK;
@@ -1789,13 +1791,18 @@ int setup_device(umf_plan* p, umf_plan** out) {
     p->fronts_h.push_back(D);
   }
   p->xs_elems = xo;
-  // a blocked sweep launch holds at most 128 fronts of one level (16 when they are small), each
-  // publishing at most the level's largest block count
+  // published blocks of the largest blocked sweep launch: a blocked level's fronts times its
+  // largest block count
   p->ll_blocks = 0;
   for (size_t ell = 0; ell < P.box_gid.size(); ell++) {
     i64 nbmax = 0;
-    for (int f : P.box_gid[ell]) nbmax = std::max<i64>(nbmax, (P.fronts[f].n + NB - 1) / NB);
-    p->ll_blocks = std::max<i64>(p->ll_blocks, std::min<i64>((i64)P.box_gid[ell].size(), 128) * nbmax);
+    int rows = 0;
+    for (int f : P.box_gid[ell]) {
+      nbmax = std::max<i64>(nbmax, (P.fronts[f].n + NB - 1) / NB);
+      rows = std::max(rows, P.fronts[f].n + P.fronts[f].m);
+    }
+    const i64 nf = (i64)P.box_gid[ell].size();
+    if (sweep_blocked(nf, rows)) p->ll_blocks = std::max<i64>(p->ll_blocks, nf * nbmax);
   }
   for (const SepInfo& S : P.seps) {
     SepDev D{};
