@@ -135,12 +135,13 @@ def test_update_matches_oracle(case):
         assert np.array_equal(G.front_block(ell, b, k), v, equal_nan=True), (ell, b, k)
 
 
+@pytest.mark.parametrize("dims,maxs", [((31, 31, 31), 3), ((255, 255), 8)])
 @pytest.mark.parametrize("form", ["fp64", "fp32"])
-def test_fenced_sweeps_identical(form):
-    """The blocked sweeps' two publication variants (write-through atomics alone, or with
-    agent-scope release / acquire fences) give the same solution bit for bit."""
-    dims = (31, 31, 31)
-    _, A, G, O = _mk(dims, 3, "lognormal", 1)
+def test_fenced_sweeps_identical(form, dims, maxs):
+    """The blocked sweeps' two publication variants (epoch-tagged words polled by the readers,
+    or per-block flags between agent-scope release / acquire fences) give the same solution bit
+    for bit; the 2D case has blocked levels of 4 and 16 fronts with boundary blocks."""
+    _, A, G, O = _mk(dims, maxs, "lognormal", 1)
     G.SetPrecision(form, tol=1e-12)
     G.Factor()
     B = np.random.default_rng(3).standard_normal((A.n, 5))
