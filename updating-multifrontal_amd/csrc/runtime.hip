@@ -586,15 +586,14 @@ int factor_level(umf_plan* p, const Sched& S, const int* ids, int ell, T* arena,
   // columns and updates the group's later diagonal blocks -- instead of a group GEMM launch
   // after every step (one launch fewer on the step chain)
   bool lazy = false;
-  // FP32 left-looking levels take block columns in pairs (k_chol_diag, k_chol_diag2,
-  // k_chol_panel2) where every front of the plan's level reaching k0 has two full blocks left:
-  // the panel below both blocks then reads its rows' finished columns once per 128 columns, and
-  // the factors are bit-identical to single steps (2D 1023^2 FP32 step 19.8 -> 19.4 ms, 3D 127^3
-  // 1231 -> 1226 ms).  FP64 levels keep single steps: the paired FP64 panel holds three
-  // workgroups per CU instead of four, which cancels its saved traffic (2D 21.8 -> 21.7 ms, 3D
-  // 2354 -> 2361 ms).
+  // Left-looking levels take block columns in pairs (k_chol_diag, k_chol_diag2, k_chol_panel2)
+  // where every front of the plan's level reaching k0 has two full blocks left: the panel below
+  // both blocks then reads its rows' finished columns once per 128 columns, and the factors are
+  // bit-identical to single steps (2D 1023^2 FP32 step 19.8 -> 19.4 ms, 3D 127^3 1231 -> 1226
+  // ms; FP64, with the MFMA-phase priority of the 64 x 128 GEMM: 3D panels 296 -> 283 ms, 2D
+  // step 21.7 -> 21.3 ms).
   auto paired = [&](int k0) {
-    if (!left || sizeof(T) != 4) return false;
+    if (!left) return false;
     bool any = false;
     for (int f : level_fronts) {
       const int n = P.fronts[f].n;

@@ -181,7 +181,7 @@ class _MF:
         mod = np.ascontiguousarray(sorted(set(int(d) for d in modifiedDOFs)), dtype=np.int32)
         _check(_analyze_update(self._h, len(mod), _i(mod) if len(mod) else None))
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):  # bound now: module globals may be gone at interpreter exit
         h = getattr(self, "_h", None)
         if h:
             _destroy(h)
@@ -628,10 +628,10 @@ class Server:
         _check(_srv_stats(self._h, int(slot), C.byref(st)))
         return {f: getattr(st, f) for f, _ in Stats._fields_}
 
-    def close(self):
+    def close(self, _destroy=None):
         if getattr(self, "_h", None):
-            _srv_destroy(self._h)
+            (_destroy or _srv_destroy)(self._h)
             self._h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _destroy=_srv_destroy):  # bound now: module globals may be gone at interpreter exit
+        self.close(_destroy)
